@@ -1,0 +1,158 @@
+/*
+ * sfr3d.h -- MI355X (gfx950) linear-solve path for SFR3D-assembled systems:
+ * Jacobi-preconditioned CG and BiCGSTAB over CSR-family and ELLPACK matrices.
+ *
+ * The boundary keeps the contract of the reference's solver dispatch
+ * (solverC, reference include/Solv.h:59-70, src/Solv.c:64-501): a matrix in
+ * one of SFR3D's storages, a solver id, a tolerance, a maximum iteration
+ * count, b and x.  Solver ids are those of include/Define.h:392-397 and
+ * storage ids those of include/Define.h:407-412 (see sfr3d_setup.h).
+ *
+ * Stopping tests are the reference's:
+ *   PCG      sqrt|r.D^-1 r| < tol * sqrt|b.D^-1 b|   (src/Cg.c:73-78,141-142)
+ *   BiCGSTAB ||r||_2 < tol * ||b||_2, checked after each half step
+ *            (src/Bicgstab.c:79-81,129-131,153-154)
+ * and the iteration count reported is the reference's j+1.
+ *
+ * Every entry point returns an SFR_* status.  All arrays are caller-owned and
+ * never freed or reallocated; x is read (newX = 0) and written, b is read.
+ * One sfr_context owns one HIP stream and its device buffers; calls on one
+ * context must come from one host thread at a time, distinct contexts are
+ * independent.
+ */
+#ifndef SFR3D_H
+#define SFR3D_H
+
+#include <stdbool.h>
+#include <stdint.h>
+#include <stdio.h>
+#include "sfr3d_setup.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* solver ids (include/Define.h:392-397) */
+#define SFR_PCG        1
+#define SFR_PBICGSTAB  2
+
+/* status codes */
+#define SFR_OK             0
+#define SFR_MAXIT          1   /* j == maxIt: reference prints + exit()    */
+#define SFR_BREAKDOWN      2   /* zero / non-finite denominator            */
+#define SFR_BAD_ARG        3
+#define SFR_HIP_ERROR      4
+#define SFR_NO_DEVICE      5
+#define SFR_NOT_SUPPORTED  6
+
+/* arithmetic of a device solve */
+#define SFR_PREC_FP64   0      /* FP64 throughout, the reference algorithm */
+#define SFR_PREC_MIXED  1      /* FP32 inner Krylov, FP64 residual/update  */
+
+/* device matrix formats (chosen at upload, independent of the storage) */
+#define SFR_FMT_AUTO   0
+#define SFR_FMT_ELL    1       /* slot-major (column-major) ELL, lane/row  */
+#define SFR_FMT_CSR    2       /* full-row CSR, several lanes per row      */
+
+/* A matrix in one of the reference's host storages (SistEq, reference
+ * include/Sisteq.h:27-45).  For CSRC with unsym, the upper part is au, or
+ * al + nad when au is NULL (the reference keeps [al | au] contiguous). */
+typedef struct sfr_system {
+  int32_t        neq;
+  int32_t        nad;
+  const int32_t *ia;
+  const int32_t *ja;
+  const double  *al;
+  const double  *ad;
+  const double  *au;
+  short          storage;
+  short          unsym;
+} sfr_system;
+
+typedef struct sfr_stats {
+  int      status;
+  unsigned iters;     /* Krylov iterations (reference j+1); MIXED: sum of inner */
+  unsigned outer;     /* MIXED: FP64 refinement steps                          */
+  double   tol;
+  double   norm_b;    /* PCG: sqrt|b.D^-1 b|  BiCGSTAB: ||b||                   */
+  double   conv;      /* tol * norm_b                                          */
+  double   xKx;       /* x.Ax            (src/Cg.c:155-158)                    */
+  double   norm_x;    /* ||x||           (src/Cg.c:162)                        */
+  double   norm_r;    /* ||b - Ax||, FP64                                      */
+  double   norm_r_m;  /* sqrt|r.D^-1 r|, FP64                                  */
+  double   relres;    /* ||b - Ax|| / ||b||                                    */
+  double   time_s;    /* solve time on the device (events on the context stream) */
+} sfr_stats;
+
+typedef struct sfr_context sfr_context;
+typedef struct sfr_matrix  sfr_matrix;
+
+/* ---- context -------------------------------------------------------- */
+int         sfr_init(sfr_context **ctx, int device);
+void        sfr_finalize(sfr_context *ctx);
+const char *sfr_status_string(int status);
+const char *sfr_last_error(const sfr_context *ctx);
+int         sfr_synchronize(sfr_context *ctx);
+
+/* device memory helpers on the context's stream (plain hipMalloc'd memory) */
+int sfr_dev_alloc(sfr_context *ctx, size_t bytes, void **ptr);
+int sfr_dev_free(sfr_context *ctx, void *ptr);
+int sfr_copy_h2d(sfr_context *ctx, void *dst, const void *src, size_t bytes);
+int sfr_copy_d2h(sfr_context *ctx, void *dst, const void *src, size_t bytes);
+
+/* ---- device-resident matrices -------------------------------------------
+ * sfr_matrix_create uploads the pattern once (conversion to the device
+ * format, value gather maps) and the values; sfr_matrix_set_values uploads
+ * new values for the same pattern, as SFR3D's SIMPLE loop re-assembles every
+ * outer iteration.  format = SFR_FMT_* (AUTO lets the tuner decide). */
+int  sfr_matrix_create(sfr_context *ctx, const sfr_system *sys, int format,
+                       sfr_matrix **A);
+int  sfr_matrix_set_values(sfr_context *ctx, sfr_matrix *A, const double *al,
+                           const double *ad, const double *au);
+void sfr_matrix_destroy(sfr_context *ctx, sfr_matrix *A);
+/* neq, device format, ELL slots, stored nnz of the device format */
+int  sfr_matrix_info(const sfr_matrix *A, int32_t *neq, int *format,
+                     int *slots, int64_t *nnz);
+
+/* ---- solves ---------------------------------------------------------------
+ * sfr_solve: host arrays in and out, the drop-in for solverC's PCG and
+ * PBICGSTAB branches (reference src/Solv.c:93-212).
+ * sfr_solve_device: b and x already on the device (d_b, d_x: neq doubles).
+ * precision SFR_PREC_MIXED runs FP32 inner solves under an FP64 refinement
+ * loop whose residual, update and stopping test are FP64 and use the same
+ * test as the FP64 solver. */
+int sfr_solve(sfr_context *ctx, const sfr_system *sys, int solver, double tol,
+              unsigned maxit, const double *b, double *x, int newX,
+              sfr_stats *stats);
+int sfr_solve_device(sfr_context *ctx, sfr_matrix *A, int solver,
+                     int precision, double tol, unsigned maxit,
+                     const double *d_b, double *d_x, int newX,
+                     sfr_stats *stats);
+
+/* Measurement hook: average device time of one launch of a loop kernel,
+ * HIP events on the context stream.  which = 0: FP64 PCG SpMV + p.q;
+ * which = 1: FP32 inner SpMV + p.q. */
+int sfr_kernel_time(sfr_context *ctx, sfr_matrix *A, int which, int reps,
+                    double *ms_per_launch);
+/* Tuning knobs of a context: iterations per captured graph chunk (>= 1) and
+ * the relative residual reduction of each MIXED inner solve. */
+int sfr_set_option(sfr_context *ctx, const char *name, double value);
+
+/* Drop-in twin of solverC (reference include/Solv.h:59-70), same prototype
+ * with the reference's opaque Memoria/Interface/BufferOmp pointers passed
+ * through untouched.  Non-convergence and bad ids end the process with the
+ * reference's messages and exit codes (src/Cg.c:195-200,
+ * src/Bicgstab.c:225-231, src/Solv.c:496); with fLog one line in the
+ * reference's format goes to fSolvLog (src/Cg.c:201-208,
+ * src/Bicgstab.c:235-242). */
+void sfr_solverC(void *m, int nEq, int nEqNov, int nAd, int nAdR,
+                 int *ia, int *ja, double *al, double *ad, double *au,
+                 double *b, double *x, void *iNeq, void *bOmp,
+                 double tol, unsigned int maxIt, short storage, short solver,
+                 FILE *fSolvLog, bool fLog, bool newX, bool unSym,
+                 const char *name);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SFR3D_H */

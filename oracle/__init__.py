@@ -1,0 +1,178 @@
+"""ORACLE -- test infrastructure only.
+
+CPU restatement of the reference's linear-solve path (sfr_oracle.c: pcg,
+pbicgstab, the serial matvecs, dot, preCondDiag) and of its index builders
+(patterns.py: convGraph, GENRCM, numEqV1/V2, csrIa/csrJa, ellPackJa, the csr/
+ellPack value scatter), plus, where the reference tree exists, a harness
+around the reference's own compiled C (_ref/libsfr_ref.so, built by
+oracle/Makefile).
+
+Only the tests, ``__graft_entry__.smoke()`` and the CPU-baseline leg of
+``bench.py`` may import this package, and only as the checker / the timed
+baseline.  The product library (sfr3d_amd) never loads it.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ORACLE_LIB = os.path.join(HERE, "liboracle.so")
+REF_LIB = os.path.join(HERE, "_ref", "libsfr_ref.so")
+
+i32p = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+i64p = np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS")
+i16p = np.ctypeslib.ndpointer(dtype=np.int16, flags="C_CONTIGUOUS")
+f64p = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+
+
+class OrcStats(C.Structure):
+    _fields_ = [("status", C.c_int), ("iters", C.c_uint), ("xKx", C.c_double),
+                ("norm_x", C.c_double), ("norm_r", C.c_double),
+                ("norm_r_m", C.c_double), ("norm_b", C.c_double),
+                ("conv", C.c_double), ("time_s", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+_orc = None
+_ref = None
+
+
+def orc():
+    """liboracle.so (built by `make -C oracle`)."""
+    global _orc
+    if _orc is None:
+        if not os.path.exists(ORACLE_LIB):
+            raise ImportError(f"{ORACLE_LIB} missing: run `make -C oracle`")
+        lib = C.CDLL(ORACLE_LIB)
+        for name in ("orc_pcg", "orc_pbicgstab"):
+            f = getattr(lib, name)
+            f.restype = C.c_int
+            f.argtypes = [C.c_int, C.c_int, i32p, i32p, f64p, f64p, f64p, f64p,
+                          C.c_double, C.c_uint, C.c_int, C.c_int, C.c_int,
+                          C.POINTER(OrcStats)]
+        lib.orc_matvec.restype = C.c_int
+        lib.orc_matvec.argtypes = [C.c_int, C.c_int, C.c_int, i32p, i32p, f64p,
+                                   f64p, f64p, f64p]
+        lib.orc_dot.restype = C.c_double
+        lib.orc_dot.argtypes = [f64p, f64p, C.c_int]
+        lib.orc_full_rows.restype = C.c_int
+        lib.orc_full_rows.argtypes = [C.c_int, C.c_int, C.c_int, i32p, i32p,
+                                      f64p, i64p, i32p, f64p]
+        lib.orc_pcg_omp.restype = C.c_int
+        lib.orc_pcg_omp.argtypes = [C.c_int, i64p, i32p, f64p, f64p, f64p,
+                                    f64p, C.c_double, C.c_uint, C.c_int,
+                                    C.POINTER(OrcStats)]
+        lib.orc_max_threads.restype = C.c_int
+        _orc = lib
+    return _orc
+
+
+def ref_available():
+    return os.path.exists(REF_LIB)
+
+
+def ref():
+    """The reference's own compiled C (only where /root/reference exists)."""
+    global _ref
+    if _ref is None:
+        if not ref_available():
+            raise ImportError(f"{REF_LIB} missing: run `make -C oracle ref`")
+        lib = C.CDLL(REF_LIB)
+        lib.ref_init.argtypes = [C.c_long]
+        lib.ref_reord.argtypes = [i32p, i32p, i16p, C.c_short, C.c_int, C.c_int]
+        lib.ref_genrcm.argtypes = [C.c_int, i32p, i32p, i32p]
+        lib.ref_numeq_v1.argtypes = [i32p, i32p, C.c_int]
+        lib.ref_numeq_v2.argtypes = [i32p, i32p, i16p, i16p, C.c_int, C.c_short]
+        lib.ref_datastruct.argtypes = [C.c_short, C.c_int, i32p, i32p, i32p,
+                                       i16p, C.c_int, C.c_short, C.c_int,
+                                       np.ctypeslib.ndpointer(dtype=np.int64)]
+        lib.ref_datastruct_copy.argtypes = [i32p, i32p, C.c_long, C.c_long]
+        lib.ref_assemble.argtypes = [i32p, i32p, i16p, C.c_int, C.c_short,
+                                     f64p, f64p, f64p, f64p, f64p, C.c_long,
+                                     C.c_int]
+        lib.ref_solve.argtypes = [C.c_int, C.c_int, i32p, i32p, f64p, f64p,
+                                  C.c_void_p, f64p, f64p, C.c_double, C.c_uint,
+                                  C.c_short, C.c_short, C.c_int, C.c_int, f64p,
+                                  C.c_char_p, C.c_int]
+        lib.ref_matvec.argtypes = [C.c_short, C.c_int, C.c_int, i32p, i32p,
+                                   f64p, f64p, f64p, f64p]
+        _ref = lib
+    return _ref
+
+
+# ---- convenience wrappers ---------------------------------------------------
+
+def pcg(system, b, tol, maxit, x=None, newX=True):
+    """Reference-order serial PCG (oracle).  Returns (x, stats)."""
+    return _krylov("orc_pcg", system, b, tol, maxit, x, newX)
+
+
+def pbicgstab(system, b, tol, maxit, x=None, newX=True):
+    return _krylov("orc_pbicgstab", system, b, tol, maxit, x, newX)
+
+
+def _al_contig(system):
+    al = system.al
+    if system.au is not None:
+        al = np.concatenate([system.al[:system.nad], system.au[:system.nad]])
+    return np.ascontiguousarray(al, np.float64)
+
+
+def _krylov(name, system, b, tol, maxit, x, newX):
+    b = np.ascontiguousarray(b, np.float64)
+    x = np.zeros(system.neq) if x is None else np.array(x, np.float64)
+    st = OrcStats()
+    ad = system.ad if system.ad is not None else system.al
+    rc = getattr(orc(), name)(system.neq, system.nad, system.ia, system.ja,
+                              _al_contig(system),
+                              np.ascontiguousarray(ad, np.float64), b, x, tol,
+                              maxit, 1 if newX else 0, system.storage,
+                              1 if system.unsym else 0, C.byref(st))
+    if rc:
+        raise RuntimeError(f"{name} = {rc}")
+    return x, st.as_dict()
+
+
+def matvec(system, x):
+    y = np.zeros(system.neq)
+    ad = system.ad if system.ad is not None else system.al
+    orc().orc_matvec(system.storage, 1 if system.unsym else 0, system.neq,
+                     system.ia, system.ja, _al_contig(system),
+                     np.ascontiguousarray(ad, np.float64),
+                     np.ascontiguousarray(x, np.float64), y)
+    return y
+
+
+def full_rows(system):
+    """Full-row CSR (rp, cj, cv) of the off-diagonal part, for the OpenMP
+    baseline."""
+    n = system.neq
+    if system.storage == 4:
+        cnt = n * system.ia[0]
+    else:
+        cnt = int(system.ia[n]) * (1 if system.unsym and system.storage in (1, 2, 5) else 2)
+    rp = np.zeros(n + 1, np.int64)
+    cj = np.zeros(max(cnt, 1), np.int32)
+    cv = np.zeros(max(cnt, 1), np.float64)
+    rc = orc().orc_full_rows(system.storage, 1 if system.unsym else 0, n,
+                             system.ia, system.ja, _al_contig(system), rp, cj,
+                             cv)
+    if rc:
+        raise RuntimeError("orc_full_rows")
+    return rp, cj[:rp[n]].copy(), cv[:rp[n]].copy()
+
+
+def pcg_omp(rows, ad, b, tol, maxit, nthreads=0):
+    rp, cj, cv = rows
+    x = np.zeros(len(ad))
+    st = OrcStats()
+    rc = orc().orc_pcg_omp(len(ad), rp, cj, cv,
+                           np.ascontiguousarray(ad[:len(ad)], np.float64),
+                           np.ascontiguousarray(b, np.float64), x, tol, maxit,
+                           nthreads, C.byref(st))
+    if rc:
+        raise RuntimeError("orc_pcg_omp")
+    return x, st.as_dict()

@@ -1,0 +1,190 @@
+/*
+ * ORACLE -- test infrastructure only.
+ *
+ * Thin harness around the reference's own C sources (compiled where they
+ * lie by oracle/Makefile into oracle/_ref/libsfr_ref.so).  It sets the
+ * reference's process globals the way its main program does and calls its
+ * functions unchanged:
+ *   reord          src/Reord.c:18          numEqV1/V2  src/Numeq.c:70,109
+ *   dataStruct     src/Datastruct.c:28     assbly      src/Assbly.c:43
+ *   solverC        src/Solv.c:64 (PCG / PBICGSTAB, serial, log line captured)
+ *   genrcm         src/Rcm.c:56            matvecs     src/HccaBlas.c
+ * Used only to pin the oracle and to generate golden vectors.
+ */
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <Solv.h>
+#include <Reord.h>
+#include <Sisteq.h>
+#include <Assbly.h>
+
+static Memoria g_mem;
+static int g_init;
+static SistEq g_sys;
+static int g_have_sys;
+static int g_sys_ad_alloc;
+
+int ref_init(long arena_bytes)
+{
+  if (g_init) finalizeMem(&g_mem, false);
+  nmax = arena_bytes;
+  initMem(&g_mem, nmax, false);
+  fileLogExc = fopen("/dev/null", "w");
+  fileLogDebug = fileLogExc;
+  mpiVar.nPrcs = 1;
+  mpiVar.myId = 0;
+  ompVar.fSolver = false;
+  memset(&tm, 0, sizeof tm);
+  /* dalloc (src/Memoria.c:280-330) refuses to free the arena's only block,
+   * so one small block stays allocated for the life of the harness */
+  static char *sentinel;
+  HccaAlloc(char, &g_mem, sentinel, 64, "refSentinel", false);
+  g_init = 1;
+  g_have_sys = 0;
+  return 0;
+}
+
+int ref_reord(int *num, const int *nelcon, const short *nViz, short maxViz,
+              int numel, int flag)
+{
+  reord(&g_mem, num, nelcon, nViz, maxViz, numel, numel, flag != 0, 1);
+  return 0;
+}
+
+int ref_genrcm(int node_num, int *adj_row_1based, int *adj_1based, int *perm)
+{
+  genrcm(node_num, adj_row_1based, adj_1based, perm);
+  return 0;
+}
+
+int ref_numeq_v1(int *id, int *num, int numel) { return numEqV1(id, num, numel); }
+
+int ref_numeq_v2(int *id, int *num, short *rt, short *nFace, int numel,
+                 short maxViz)
+{
+  return numEqV2(id, num, rt, nFace, numel, maxViz);
+}
+
+static void drop_sys(void)
+{
+  if (!g_have_sys) return;
+  HccaDealloc(&g_mem, g_sys.al, "refA", false);
+  if (g_sys_ad_alloc) HccaDealloc(&g_mem, g_sys.ad, "refAd", false);
+  HccaDealloc(&g_mem, g_sys.ja, "refJa", false);
+  HccaDealloc(&g_mem, g_sys.ia, "refIa", false);
+  g_have_sys = 0;
+}
+
+/* Builds the structure with dataStruct; returns nad, sizes in out[0..2] =
+ * {len(ia), len(ja), len(al)}. */
+int ref_datastruct(short storage, int unsym, int *id, int *num, int *nelcon,
+                   short *nViz, int numel, short maxViz, int neq, long *out)
+{
+  drop_sys();
+  memset(&g_sys, 0, sizeof g_sys);
+  g_sys.storage = storage;
+  g_sys.unsym = unsym != 0;
+  g_sys.neq = neq;
+  g_sys.neqNov = neq;
+  dataStruct(&g_mem, id, num, nelcon, nViz, numel, maxViz, 1,
+             "refIa", "refJa", "refAd", "refA", &g_sys);
+  g_have_sys = 1;
+  g_sys_ad_alloc = storage != CSR;
+  long nia, nja, nal;
+  if (storage == ELLPACK) {
+    nia = 2; nja = (long)(neq + 1) * maxViz; nal = nja;
+  } else {
+    nia = neq + 1; nja = g_sys.ia[neq]; nal = nja;
+    if ((storage == CSRC || storage == CSRCCOO) && unsym) nal = 2 * nja;
+  }
+  out[0] = nia; out[1] = nja; out[2] = nal;
+  return g_sys.nad;
+}
+
+int ref_datastruct_copy(int *ia, int *ja, long nia, long nja)
+{
+  if (!g_have_sys) return -1;
+  memcpy(ia, g_sys.ia, sizeof(int) * (size_t)nia);
+  memcpy(ja, g_sys.ja, sizeof(int) * (size_t)nja);
+  return 0;
+}
+
+/* Scatter per-cell coefficients with assbly into the held structure.
+ * lA[c*(maxViz+1)+f], diagonal at f = maxViz; lB[c]. */
+int ref_assemble(int *id, int *nelcon, short *nViz, int numel, short maxViz,
+                 const double *lA, const double *lB, double *al, double *ad,
+                 double *b, long nal, int neqad)
+{
+  if (!g_have_sys) return -1;
+  const short storage = g_sys.storage;
+  zero(g_sys.al, nal, DOUBLEC);
+  if (storage != CSR) zero(g_sys.ad, neqad, DOUBLEC);
+  double *bb = calloc((size_t)neqad, sizeof(double));
+  INT lId[16];
+  DOUBLE la[16], lb[1];
+  for (int c = 0; c < numel; c++) {
+    if (id[c] < 1) continue;
+    const short nFace = nViz[c];
+    for (int f = 0; f < nFace; f++) {
+      const int viz = nelcon[c * maxViz + f] - 1;
+      lId[f] = -1;
+      if (viz != -2) lId[f] = id[viz] - 1;
+      la[f] = lA[(long)c * (maxViz + 1) + f];
+    }
+    lId[nFace] = id[c] - 1;
+    la[nFace] = lA[(long)c * (maxViz + 1) + maxViz];
+    lb[0] = lB[c];
+    assbly(g_sys.ia, g_sys.ja, g_sys.al, g_sys.ad, bb, lId, la, lb,
+           g_sys.neq, g_sys.neq, g_sys.nad, 0, nFace, 1, storage, true, true,
+           g_sys.unsym);
+  }
+  memcpy(al, g_sys.al, sizeof(double) * (size_t)nal);
+  if (storage != CSR) memcpy(ad, g_sys.ad, sizeof(double) * (size_t)neqad);
+  memcpy(b, bb, sizeof(double) * (size_t)neqad);
+  free(bb);
+  return 0;
+}
+
+/* solverC with the given arrays; out[0] = iterations, out[1] = xKx,
+ * out[2] = ||x|| (parsed back from the reference's own log line). */
+int ref_solve(int neq, int nad, int *ia, int *ja, double *al, double *ad,
+              double *au, double *b, double *x, double tol, unsigned maxit,
+              short storage, short solver, int newX, int unsym, double *out,
+              char *line, int linelen)
+{
+  FILE *f = tmpfile();
+  if (!f) return -1;
+  solverC(&g_mem, neq, neq, nad, 0, ia, ja, al, ad, au, b, x, NULL, NULL,
+          tol, maxit, storage, solver, f, true, newX != 0, unsym != 0, "ref");
+  fflush(f);
+  rewind(f);
+  char buf[512] = {0};
+  if (!fgets(buf, sizeof buf, f)) { fclose(f); return -2; }
+  fclose(f);
+  if (line && linelen > 0) { strncpy(line, buf, (size_t)linelen - 1); line[linelen - 1] = 0; }
+  double t, xkx, nx, tm_;
+  int it;
+  if (solver == PCG) {
+    if (sscanf(buf, "PCG: tol %lf iteration %d xKx %lf norma(x*x) %lf time %lf",
+               &t, &it, &xkx, &nx, &tm_) != 5) return -3;
+  } else {
+    if (sscanf(buf, "PBICGSTAB: tol %lf iteration %d xKx %lf ||x*x|| %lf time %lf",
+               &t, &it, &xkx, &nx, &tm_) != 5) return -3;
+  }
+  out[0] = it; out[1] = xkx; out[2] = nx; out[3] = tm_;
+  return 0;
+}
+
+/* y = A x with the reference's serial matvec for the storage. */
+int ref_matvec(short storage, int unsym, int neq, int *ia, int *ja,
+               double *al, double *ad, double *x, double *y)
+{
+  void (*mv)() = NULL;
+  const bool o = ompVar.fSolver;
+  ompVar.fSolver = false;
+  setMatVec(&mv, storage, unsym != 0, false);
+  ompVar.fSolver = o;
+  mv(neq, ia, ja, al, ad, x, y);
+  return 0;
+}

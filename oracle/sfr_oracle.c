@@ -1,0 +1,431 @@
+/*
+ * ORACLE -- test infrastructure only.
+ *
+ * CPU restatement of the reference's linear-solve path, used as the checker
+ * of the GPU implementation and as the timed CPU baseline.  Only the tests,
+ * the smoke check and the benchmark's CPU-baseline leg may load it; the
+ * product library never does.
+ *
+ * Followed line by line for operation order (serial dot is one left-to-right
+ * accumulator, so results are reproducible bit for bit):
+ *   dot              reference src/HccaBlas.c:393-427
+ *   matVecCsr        reference src/HccaBlas.c:3244-3268
+ *   matVecCsrSym     reference src/HccaBlas.c:3297-3331
+ *   matVecCsrD       reference src/HccaBlas.c:3358-3382
+ *   matVecCsrDSym    reference src/HccaBlas.c:2620-2651
+ *   matVecCsrC       reference src/HccaBlas.c:4223-4260
+ *   matVecEllPack    reference src/HccaBlas.c:4700-4752 (widths 6/4/3 only;
+ *                    other widths leave y untouched, as the reference does)
+ *   setMatVec        reference src/Solv.c:732-919 (serial branch)
+ *   preCondDiag      reference src/PreCond.c:18-26
+ *   pcg              reference src/Cg.c:47-209, driven as src/Solv.c:93-145
+ *   pbicgstab        reference src/Bicgstab.c:50-245, as src/Solv.c:150-212
+ * The OpenMP variants (orc_*_omp) keep the same recurrences with parallel
+ * loops and reductions over full rows; they are the all-cores CPU baseline
+ * (the reference's pcgOmp, src/Cg.c:490-715, plays that role there).
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+#include <time.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#define ORC_CSR 1
+#define ORC_CSRD 2
+#define ORC_CSRC 3
+#define ORC_ELLPACK 4
+#define ORC_CSRDCOO 5
+#define ORC_CSRCCOO 6
+
+typedef struct orc_stats {
+  int status;          /* 0 converged, 1 maxit */
+  unsigned iters;
+  double xKx, norm_x, norm_r, norm_r_m, norm_b, conv, time_s;
+} orc_stats;
+
+typedef void (*matvec_fn)(int, const int *, const int *, const double *,
+                          const double *, const double *, double *);
+
+static double now(void)
+{
+  struct timespec ts;
+  clock_gettime(CLOCK_MONOTONIC, &ts);
+  return ts.tv_sec + 1e-9 * ts.tv_nsec;
+}
+
+double orc_dot(const double *x1, const double *x2, int n)
+{
+  double tmp = 0.0;
+  for (int i = 0; i < n; i++) tmp += x1[i] * x2[i];
+  return tmp;
+}
+
+static void mv_csr(int n, const int *ia, const int *ja, const double *a,
+                   const double *ad, const double *x, double *y)
+{
+  (void)ad;
+  for (int i = 0; i < n; i++) {
+    double tmp = 0.0;
+    for (int j = ia[i]; j < ia[i + 1]; j++) tmp += a[j] * x[ja[j]];
+    y[i] = tmp;
+  }
+}
+
+static void mv_csr_sym(int n, const int *ia, const int *ja, const double *a,
+                       const double *ad, const double *x, double *y)
+{
+  (void)ad;
+  for (int i = 0; i < n; i++) {
+    const double xi = x[i];
+    double tmp = y[i] = 0.0;
+    for (int j = ia[i]; j < ia[i + 1]; j++) {
+      const int jak = ja[j];
+      const double s = a[j];
+      if (i == jak) tmp += s * xi;
+      else { tmp += s * x[jak]; y[jak] += s * xi; }
+    }
+    y[i] = tmp;
+  }
+}
+
+static void mv_csrd(int n, const int *ia, const int *ja, const double *a,
+                    const double *ad, const double *x, double *y)
+{
+  for (int i = 0; i < n; i++) {
+    double tmp = ad[i] * x[i];
+    for (int j = ia[i]; j < ia[i + 1]; j++) tmp += a[j] * x[ja[j]];
+    y[i] = tmp;
+  }
+}
+
+static void mv_csrd_sym(int n, const int *ia, const int *ja, const double *al,
+                        const double *ad, const double *x, double *y)
+{
+  y[0] = ad[0] * x[0];
+  for (int i = 1; i < n; i++) {
+    const double xi = x[i];
+    double tmp = ad[i] * xi;
+    for (int k = ia[i]; k < ia[i + 1]; k++) {
+      const int jak = ja[k];
+      const double s = al[k];
+      tmp += s * x[jak];
+      y[jak] += s * xi;
+    }
+    y[i] = tmp;
+  }
+}
+
+static void mv_csrc(int n, const int *ia, const int *ja, const double *a,
+                    const double *ad, const double *x, double *y)
+{
+  const int nad = ia[n] - ia[0];
+  const double *au = a + nad, *al = a;
+  y[0] = ad[0] * x[0];
+  for (int i = 1; i < n; i++) {
+    const double xi = x[i];
+    double tmp = ad[i] * xi;
+    for (int k = ia[i]; k < ia[i + 1]; k++) {
+      const int jak = ja[k];
+      tmp += al[k] * x[jak];
+      y[jak] += au[k] * xi;
+    }
+    y[i] = tmp;
+  }
+}
+
+static void mv_ell(int n, const int *ia, const int *ja, const double *a,
+                   const double *ad, const double *x, double *y)
+{
+  const int s = ia[0];
+  if (s != 6 && s != 4 && s != 3) return;
+  for (int i = 0; i < n; i++) {
+    double t = ad[i] * x[i];
+    for (int f = 0; f < s; f++) t += a[i * s + f] * x[ja[i * s + f]];
+    y[i] = t;
+  }
+}
+
+static matvec_fn pick(int storage, int unsym)
+{
+  switch (storage) {
+  case ORC_CSR: return unsym ? mv_csr : mv_csr_sym;
+  case ORC_CSRD: case ORC_CSRDCOO: return unsym ? mv_csrd : mv_csrd_sym;
+  case ORC_CSRC: case ORC_CSRCCOO: return unsym ? mv_csrc : mv_csrd_sym;
+  case ORC_ELLPACK: return mv_ell;
+  default: return NULL;
+  }
+}
+
+/* y = A x with the storage's reference matvec (for checks) */
+int orc_matvec(int storage, int unsym, int n, const int *ia, const int *ja,
+               const double *al, const double *ad, const double *x, double *y)
+{
+  matvec_fn mv = pick(storage, unsym);
+  if (!mv) return -1;
+  mv(n, ia, ja, al, ad, x, y);
+  return 0;
+}
+
+int orc_pcg(int n, int nad, const int *ia, const int *ja, const double *al,
+            const double *ad, const double *b, double *x, double tol,
+            unsigned maxit, int newX, int storage, int unsym, orc_stats *st)
+{
+  (void)nad;
+  matvec_fn matvec = pick(storage, unsym);
+  if (!matvec || n <= 0) return -1;
+  double *m = malloc(sizeof(double) * n), *z = malloc(sizeof(double) * n);
+  double *r = malloc(sizeof(double) * n), *p = malloc(sizeof(double) * n);
+  if (!m || !z || !r || !p) { free(m); free(z); free(r); free(p); return -2; }
+  for (int i = 0; i < n; i++) m[i] = 1.0 / ad[i];          /* preCondDiag */
+  memset(z, 0, sizeof(double) * n);
+  const double t0 = now();
+  unsigned j;
+  if (newX) for (int i = 0; i < n; i++) x[i] = 0.0;
+  for (int i = 0; i < n; i++) z[i] = b[i] * m[i];
+  double d = orc_dot(b, z, n);
+  const double norm_b = sqrt(fabs(d));
+  const double conv = tol * norm_b;
+  matvec(n, ia, ja, al, ad, x, z);
+  for (int i = 0; i < n; i++) {
+    r[i] = b[i] - z[i];
+    z[i] = r[i] * m[i];
+    p[i] = z[i];
+  }
+  d = orc_dot(r, z, n);
+  for (j = 0; j < maxit; j++) {
+    matvec(n, ia, ja, al, ad, p, z);
+    const double alpha = d / orc_dot(z, p, n);
+    for (int i = 0; i < n; i++) {
+      x[i] += alpha * p[i];
+      r[i] -= alpha * z[i];
+      z[i] = r[i] * m[i];
+    }
+    const double di = orc_dot(r, z, n);
+    const double beta = di / d;
+    for (int i = 0; i < n; i++) p[i] = z[i] + beta * p[i];
+    d = di;
+    if (sqrt(fabs(d)) < conv) break;
+  }
+  matvec(n, ia, ja, al, ad, x, z);
+  const double xKx = orc_dot(x, z, n);
+  const double norm = sqrt(orc_dot(x, x, n));
+  for (int i = 0; i < n; i++) {
+    r[i] = b[i] - z[i];
+    z[i] = r[i] * m[i];
+  }
+  const double norm_r_m = sqrt(fabs(orc_dot(r, z, n)));
+  const double norm_r = sqrt(orc_dot(r, r, n));
+  const double t1 = now();
+  if (st) {
+    st->status = (j == maxit) ? 1 : 0;
+    st->iters = j + 1;
+    st->xKx = xKx; st->norm_x = norm; st->norm_r = norm_r;
+    st->norm_r_m = norm_r_m; st->norm_b = norm_b; st->conv = conv;
+    st->time_s = t1 - t0;
+  }
+  free(m); free(z); free(r); free(p);
+  return 0;
+}
+
+int orc_pbicgstab(int n, int nad, const int *ia, const int *ja,
+                  const double *al, const double *ad, const double *b,
+                  double *x, double tol, unsigned maxit, int newX,
+                  int storage, int unsym, orc_stats *st)
+{
+  (void)nad;
+  matvec_fn matvec = pick(storage, unsym);
+  if (!matvec || n <= 0) return -1;
+  double *buf = calloc((size_t)n * 8, sizeof(double));
+  if (!buf) return -2;
+  double *m = buf, *z = buf + n, *r = buf + 2 * (size_t)n, *t = buf + 3 * (size_t)n;
+  double *v = buf + 4 * (size_t)n, *p = buf + 5 * (size_t)n, *r0 = buf + 6 * (size_t)n;
+  for (int i = 0; i < n; i++) m[i] = 1.0 / ad[i];
+  const double t0 = now();
+  unsigned j;
+  if (newX) for (int i = 0; i < n; i++) x[i] = 0.0;
+  double d = orc_dot(b, b, n);
+  const double norm_b = sqrt(d);
+  const double conv = tol * norm_b;
+  matvec(n, ia, ja, al, ad, x, z);
+  for (int i = 0; i < n; i++) {
+    r0[i] = b[i] - z[i];
+    p[i] = r0[i];
+    r[i] = r0[i];
+    z[i] = p[i] * m[i];
+  }
+  for (j = 0; j < maxit; j++) {
+    matvec(n, ia, ja, al, ad, z, v);
+    const double rr0 = orc_dot(r, r0, n);
+    const double alpha = rr0 / orc_dot(v, r0, n);
+    for (int i = 0; i < n; i++) {
+      x[i] += alpha * z[i];
+      r[i] -= alpha * v[i];
+      z[i] = r[i] * m[i];
+    }
+    d = orc_dot(r, r, n);
+    if (sqrt(d) < conv) break;
+    matvec(n, ia, ja, al, ad, z, t);
+    const double w = orc_dot(t, r, n) / orc_dot(t, t, n);
+    for (int i = 0; i < n; i++) {
+      x[i] += w * z[i];
+      r[i] -= w * t[i];
+    }
+    d = orc_dot(r, r, n);
+    if (sqrt(d) < conv) break;
+    const double beta = (orc_dot(r, r0, n) / rr0) * (alpha / w);
+    for (int i = 0; i < n; i++) {
+      p[i] = r[i] + beta * (p[i] - w * v[i]);
+      z[i] = p[i] * m[i];
+    }
+  }
+  matvec(n, ia, ja, al, ad, x, z);
+  const double xKx = orc_dot(x, z, n);
+  const double norm = sqrt(orc_dot(x, x, n));
+  for (int i = 0; i < n; i++) r[i] = b[i] - z[i];
+  const double norm_r = sqrt(orc_dot(r, r, n));
+  double nrm = 0.0;
+  for (int i = 0; i < n; i++) nrm += r[i] * (r[i] * m[i]);
+  const double t1 = now();
+  if (st) {
+    st->status = (j == maxit) ? 1 : 0;
+    st->iters = j + 1;
+    st->xKx = xKx; st->norm_x = norm; st->norm_r = norm_r;
+    st->norm_r_m = sqrt(fabs(nrm)); st->norm_b = norm_b; st->conv = conv;
+    st->time_s = t1 - t0;
+  }
+  free(buf);
+  return 0;
+}
+
+/* ---- all-cores analogue (OpenMP over full rows) ---------------------------
+ * Matrix given as full rows: rp[n+1], cj[], cv[] off-diagonal, ad[] diagonal
+ * (build it with orc_full_rows).  Same recurrences as orc_pcg. */
+
+int orc_full_rows(int storage, int unsym, int n, const int *ia, const int *ja,
+                  const double *al, int64_t *rp, int *cj, double *cv)
+{
+  /* counts */
+  memset(rp, 0, sizeof(int64_t) * ((size_t)n + 1));
+  const int csrc = storage == ORC_CSRC || storage == ORC_CSRCCOO;
+  const int nad = storage == ORC_ELLPACK ? 0 : ia[n];
+  for (int i = 0; i < n; i++) {
+    if (storage == ORC_ELLPACK) { rp[i + 1] += ia[0]; continue; }
+    for (int k = ia[i]; k < ia[i + 1]; k++) {
+      const int j = ja[k];
+      if (storage == ORC_CSR && j == i) continue;
+      rp[i + 1]++;
+      if (!unsym || csrc) rp[j + 1]++;
+    }
+  }
+  for (int i = 0; i < n; i++) rp[i + 1] += rp[i];
+  int64_t *pos = malloc(sizeof(int64_t) * (size_t)n);
+  if (!pos) return -1;
+  for (int i = 0; i < n; i++) pos[i] = rp[i];
+  for (int i = 0; i < n; i++) {
+    if (storage == ORC_ELLPACK) {
+      const int s = ia[0];
+      for (int f = 0; f < s; f++) {
+        cj[pos[i]] = ja[i * s + f]; cv[pos[i]++] = al[i * s + f];
+      }
+      continue;
+    }
+    for (int k = ia[i]; k < ia[i + 1]; k++) {
+      const int j = ja[k];
+      if (storage == ORC_CSR && j == i) continue;
+      cj[pos[i]] = j; cv[pos[i]++] = al[k];
+      if (csrc && unsym) { cj[pos[j]] = i; cv[pos[j]++] = al[nad + k]; }
+      else if (!unsym) { cj[pos[j]] = i; cv[pos[j]++] = al[k]; }
+    }
+  }
+  free(pos);
+  return 0;
+}
+
+static void mv_full_omp(int n, const int64_t *rp, const int *cj,
+                        const double *cv, const double *ad, const double *x,
+                        double *y)
+{
+#pragma omp parallel for schedule(static)
+  for (int i = 0; i < n; i++) {
+    double t = ad[i] * x[i];
+    for (int64_t k = rp[i]; k < rp[i + 1]; k++) t += cv[k] * x[cj[k]];
+    y[i] = t;
+  }
+}
+
+static double dot_omp(const double *a, const double *b, int n)
+{
+  double s = 0.0;
+#pragma omp parallel for reduction(+ : s) schedule(static)
+  for (int i = 0; i < n; i++) s += a[i] * b[i];
+  return s;
+}
+
+int orc_pcg_omp(int n, const int64_t *rp, const int *cj, const double *cv,
+                const double *ad, const double *b, double *x, double tol,
+                unsigned maxit, int nthreads, orc_stats *st)
+{
+#ifdef _OPENMP
+  if (nthreads > 0) omp_set_num_threads(nthreads);
+#else
+  (void)nthreads;
+#endif
+  double *m = malloc(sizeof(double) * n), *z = malloc(sizeof(double) * n);
+  double *r = malloc(sizeof(double) * n), *p = malloc(sizeof(double) * n);
+  if (!m || !z || !r || !p) { free(m); free(z); free(r); free(p); return -2; }
+#pragma omp parallel for schedule(static)
+  for (int i = 0; i < n; i++) { m[i] = 1.0 / ad[i]; x[i] = 0.0; z[i] = 0.0; r[i] = 0.0; p[i] = 0.0; }
+  const double t0 = now();
+  unsigned j;
+#pragma omp parallel for schedule(static)
+  for (int i = 0; i < n; i++) z[i] = b[i] * m[i];
+  double d = dot_omp(b, z, n);
+  const double norm_b = sqrt(fabs(d)), conv = tol * norm_b;
+  mv_full_omp(n, rp, cj, cv, ad, x, z);
+#pragma omp parallel for schedule(static)
+  for (int i = 0; i < n; i++) { r[i] = b[i] - z[i]; z[i] = r[i] * m[i]; p[i] = z[i]; }
+  d = dot_omp(r, z, n);
+  for (j = 0; j < maxit; j++) {
+    mv_full_omp(n, rp, cj, cv, ad, p, z);
+    const double alpha = d / dot_omp(z, p, n);
+    double di = 0.0;
+#pragma omp parallel for reduction(+ : di) schedule(static)
+    for (int i = 0; i < n; i++) {
+      x[i] += alpha * p[i];
+      r[i] -= alpha * z[i];
+      z[i] = r[i] * m[i];
+      di += r[i] * z[i];
+    }
+    const double beta = di / d;
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < n; i++) p[i] = z[i] + beta * p[i];
+    d = di;
+    if (sqrt(fabs(d)) < conv) break;
+  }
+  const double t1 = now();
+  mv_full_omp(n, rp, cj, cv, ad, x, z);
+  double rr = 0.0;
+  for (int i = 0; i < n; i++) { const double ri = b[i] - z[i]; rr += ri * ri; }
+  if (st) {
+    memset(st, 0, sizeof *st);
+    st->status = (j == maxit) ? 1 : 0;
+    st->iters = j + 1;
+    st->norm_r = sqrt(rr);
+    st->norm_b = norm_b; st->conv = conv;
+    st->time_s = t1 - t0;
+  }
+  free(m); free(z); free(r); free(p);
+  return 0;
+}
+
+int orc_max_threads(void)
+{
+#ifdef _OPENMP
+  return omp_get_max_threads();
+#else
+  return 1;
+#endif
+}

@@ -1,0 +1,225 @@
+// Device helpers shared by the gfx950 kernels: XCD-aware workgroup mapping,
+// wave64/workgroup/grid reductions with a fixed summation order, and the
+// sparse row operators for the two device formats.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include "../sfr_internal.h"
+
+#define SFR_WAVES (SFR_BLOCK / 64)
+
+// Workgroup b of a grid of g (g % 8 == 0) works on chunk xcd_chunk(b, g).
+// Workgroups are handed to the 8 XCDs round-robin, so XCD x receives the
+// contiguous chunk range [x*g/8, (x+1)*g/8): rows that are neighbours in the
+// ordering share that XCD's L2.
+__device__ __forceinline__ int xcd_chunk(int b, int g)
+{
+  return (b & 7) * (g >> 3) + (b >> 3);
+}
+
+__device__ __forceinline__ double wave_sum(double v)
+{
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+// Sum NV values over the workgroup; totals are valid in thread 0.
+template <int NV>
+__device__ __forceinline__ void block_sum(double (&v)[NV])
+{
+  __shared__ double sh[NV][SFR_WAVES];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+  for (int k = 0; k < NV; k++) v[k] = wave_sum(v[k]);
+  if (lane == 0) {
+#pragma unroll
+    for (int k = 0; k < NV; k++) sh[k][w] = v[k];
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int k = 0; k < NV; k++) {
+      double s = sh[k][0];
+#pragma unroll
+      for (int j = 1; j < SFR_WAVES; j++) s += sh[k][j];
+      v[k] = s;
+    }
+  }
+  __syncthreads();
+}
+
+// Deterministic grid-wide sum of NV values.  Each workgroup stores its sums
+// in part[k*gridDim.x + blockIdx.x]; the workgroup that arrives last adds all
+// partials in workgroup order and returns true, totals in thread 0.  That
+// workgroup resets the arrival counter, so the kernel can be relaunched or
+// replayed from a graph with no memset in between.
+template <int NV>
+__device__ __forceinline__ bool grid_sum(double (&v)[NV], double *part,
+                                         unsigned *counter)
+{
+  __shared__ bool last;
+  block_sum<NV>(v);
+  const int g = gridDim.x;
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int k = 0; k < NV; k++) part[(size_t)k * g + blockIdx.x] = v[k];
+    __threadfence();
+    const unsigned prev = atomicAdd(counter, 1u);
+    last = (prev == (unsigned)g - 1);
+  }
+  __syncthreads();
+  if (!last) return false;
+  __threadfence();
+  double t[NV];
+#pragma unroll
+  for (int k = 0; k < NV; k++) {
+    double s = 0.0;
+    for (int j = threadIdx.x; j < g; j += SFR_BLOCK)
+      s += __hip_atomic_load(&part[(size_t)k * g + j], __ATOMIC_RELAXED,
+                             __HIP_MEMORY_SCOPE_AGENT);
+    t[k] = s;
+  }
+  block_sum<NV>(t);
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int k = 0; k < NV; k++) v[k] = t[k];
+    __hip_atomic_store(counter, 0u, __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_AGENT);
+  }
+  return true;
+}
+
+// ---- sparse row operators ---------------------------------------------------
+// row(i, valid, x) returns (A x)_i for the L lanes that share row i; lanes of
+// an invalid row return 0 but still take part in the cross-lane shuffles.
+
+// Slot-major ELL: val[f*n + i], col[f*n + i]; one lane per row.  S > 0 fixes
+// the slot count at compile time, S == 0 reads it at run time.  UNIT = the
+// diagonal is 1 (Jacobi-scaled copy) and ad is not read.
+template <typename T, int S, bool UNIT>
+struct EllOp {
+  static constexpr int L = 1;
+  const T *val;
+  const int32_t *col;
+  const T *ad;
+  int64_t n;
+  int s;
+  __device__ __forceinline__ T row(int64_t i, bool valid, const T *__restrict__ x) const
+  {
+    if (!valid) return T(0);
+    T acc = UNIT ? x[i] : ad[i] * x[i];
+    if constexpr (S > 0) {
+#pragma unroll
+      for (int f = 0; f < S; f++) acc += val[f * n + i] * x[col[f * n + i]];
+    } else {
+      for (int f = 0; f < s; f++) acc += val[f * n + i] * x[col[f * n + i]];
+    }
+    return acc;
+  }
+};
+
+// Full-row CSR (off-diagonals only, diagonal apart); L consecutive lanes per
+// row stride through the row and are combined with xor shuffles.
+template <typename T, int L_, bool UNIT>
+struct CsrOp {
+  static constexpr int L = L_;
+  const T *val;
+  const int32_t *col;
+  const int32_t *ptr;
+  const T *ad;
+  int64_t n;
+  int s;
+  __device__ __forceinline__ T row(int64_t i, bool valid, const T *__restrict__ x) const
+  {
+    T acc = T(0);
+    if (valid) {
+      const int sub = threadIdx.x & (L - 1);
+      const int e = ptr[i + 1];
+      for (int k = ptr[i] + sub; k < e; k += L) acc += val[k] * x[col[k]];
+    }
+#pragma unroll
+    for (int off = L / 2; off > 0; off >>= 1) acc += __shfl_xor(acc, off, L);
+    if (valid) acc += UNIT ? x[i] : ad[i] * x[i];
+    return acc;
+  }
+};
+
+// Row of pass r of this thread: chunks of rpt*(BLOCK/L) consecutive rows.
+template <int L>
+__device__ __forceinline__ int64_t row_base(int rpt)
+{
+  return (int64_t)xcd_chunk(blockIdx.x, gridDim.x) * rpt * (SFR_BLOCK / L) +
+         threadIdx.x / L;
+}
+
+// Call f(op) with the FP64 operator matching the view.
+template <typename F>
+__host__ inline void with_op64(const sfr_mat_view *A, F &&f)
+{
+  if (A->fmt == SFR_FMT_ELL) {
+    switch (A->slots) {
+    case 6: f(EllOp<double, 6, false>{A->val, A->col, A->ad, A->n, 6}); return;
+    case 4: f(EllOp<double, 4, false>{A->val, A->col, A->ad, A->n, 4}); return;
+    case 3: f(EllOp<double, 3, false>{A->val, A->col, A->ad, A->n, 3}); return;
+    default: f(EllOp<double, 0, false>{A->val, A->col, A->ad, A->n, A->slots}); return;
+    }
+  }
+  switch (A->lanes) {
+  case 1: f(CsrOp<double, 1, false>{A->val, A->col, A->rowptr, A->ad, A->n, 0}); return;
+  case 2: f(CsrOp<double, 2, false>{A->val, A->col, A->rowptr, A->ad, A->n, 0}); return;
+  case 4: f(CsrOp<double, 4, false>{A->val, A->col, A->rowptr, A->ad, A->n, 0}); return;
+  default: f(CsrOp<double, 8, false>{A->val, A->col, A->rowptr, A->ad, A->n, 0}); return;
+  }
+}
+
+// Call f(op) with the FP32 Jacobi-scaled (unit diagonal) operator.
+template <typename F>
+__host__ inline void with_op32(const sfr_mat_view *A, F &&f)
+{
+  if (A->fmt == SFR_FMT_ELL) {
+    switch (A->slots) {
+    case 6: f(EllOp<float, 6, true>{A->val32, A->col, nullptr, A->n, 6}); return;
+    case 4: f(EllOp<float, 4, true>{A->val32, A->col, nullptr, A->n, 4}); return;
+    case 3: f(EllOp<float, 3, true>{A->val32, A->col, nullptr, A->n, 3}); return;
+    default: f(EllOp<float, 0, true>{A->val32, A->col, nullptr, A->n, A->slots}); return;
+    }
+  }
+  switch (A->lanes) {
+  case 1: f(CsrOp<float, 1, true>{A->val32, A->col, A->rowptr, nullptr, A->n, 0}); return;
+  case 2: f(CsrOp<float, 2, true>{A->val32, A->col, A->rowptr, nullptr, A->n, 0}); return;
+  case 4: f(CsrOp<float, 4, true>{A->val32, A->col, A->rowptr, nullptr, A->n, 0}); return;
+  default: f(CsrOp<float, 8, true>{A->val32, A->col, A->rowptr, nullptr, A->n, 0}); return;
+  }
+}
+
+// Workgroups for n rows at rpt rows per thread and `lanes` lanes per row,
+// a multiple of 8 (XCD remap) and at most SFR_MAXG (kernels grid-stride).
+static inline int grid_for(int64_t n, int rpt, int lanes)
+{
+  const int64_t rows = (int64_t)rpt * (SFR_BLOCK / lanes);
+  int64_t g = (n + rows - 1) / rows;
+  g = ((g + 7) / 8) * 8;
+  if (g < 8) g = 8;
+  if (g > SFR_MAXG) g = SFR_MAXG;
+  return (int)g;
+}
+
+static inline int op_lanes(const sfr_mat_view *A)
+{
+  return A->fmt == SFR_FMT_ELL ? 1 : A->lanes;
+}
+
+// Launch kernel template K<Op, RPT> with the rows-per-thread of g.
+#define SFR_LAUNCH_RPT(K, Op, g, s, ...)                                      \
+  do {                                                                        \
+    switch ((g).rpt) {                                                        \
+    case 1: hipLaunchKernelGGL((K<Op, 1>), dim3((g).grid), dim3(SFR_BLOCK),   \
+                               0, s, __VA_ARGS__); break;                     \
+    case 2: hipLaunchKernelGGL((K<Op, 2>), dim3((g).grid), dim3(SFR_BLOCK),   \
+                               0, s, __VA_ARGS__); break;                     \
+    default: hipLaunchKernelGGL((K<Op, 4>), dim3((g).grid), dim3(SFR_BLOCK),  \
+                                0, s, __VA_ARGS__); break;                    \
+    }                                                                         \
+  } while (0)

@@ -1,0 +1,445 @@
+/*
+ * Context and device-resident matrices.
+ *
+ * A host matrix in one of the reference's storages (src/Datastruct.c:28-613)
+ * is converted once, on the host, to a device format that holds complete
+ * rows:
+ *   ELL  slot-major val[f*n+i]/col[f*n+i], every row padded to `slots`
+ *        entries with (col = i, value 0); lane per row, coalesced per slot;
+ *   CSR  row pointers + columns, several lanes per row.
+ * Symmetric storages (lower or upper triangle, src/HccaBlas.c:2620-2651,
+ * 3297-3331) are expanded to full rows, so the device never scatters.
+ * CSRC unsym (lower pattern, [al|au], src/HccaBlas.c:4223-4260) contributes
+ * al to row i and au to row ja[k].  ELLPACK input (src/EllPack.c:26-79,
+ * src/HccaBlas.c:4700-4752) keeps its slot order.
+ * Each stored device entry remembers which raw host value feeds it (vmap), so
+ * a re-assembly with the same pattern uploads the raw arrays and one gather
+ * kernel rebuilds the device values.
+ */
+#include <stdarg.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include "../sfr_internal.h"
+
+int sfr_set_error(sfr_context *ctx, int code, const char *fmt, ...)
+{
+  if (ctx) {
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(ctx->err, sizeof ctx->err, fmt, ap);
+    va_end(ap);
+  }
+  return code;
+}
+
+const char *sfr_last_error(const sfr_context *ctx)
+{
+  return ctx ? ctx->err : "no context";
+}
+
+const char *sfr_status_string(int s)
+{
+  switch (s) {
+  case SFR_OK: return "ok";
+  case SFR_MAXIT: return "maximum iterations reached without convergence";
+  case SFR_BREAKDOWN: return "Krylov breakdown (zero or non-finite denominator)";
+  case SFR_BAD_ARG: return "invalid argument";
+  case SFR_HIP_ERROR: return "HIP runtime error";
+  case SFR_NO_DEVICE: return "no HIP device";
+  case SFR_NOT_SUPPORTED: return "not supported";
+  default: return "unknown status";
+  }
+}
+
+#define TRY(call)                                                            \
+  do {                                                                       \
+    hipError_t e_ = (call);                                                  \
+    if (e_ != hipSuccess)                                                    \
+      return sfr_set_error(ctx, SFR_HIP_ERROR, "%s: %s (%s:%d)", #call,      \
+                           hipGetErrorString(e_), __FILE__, __LINE__);       \
+  } while (0)
+
+int sfr_init(sfr_context **out, int device)
+{
+  if (!out) return SFR_BAD_ARG;
+  *out = NULL;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return SFR_NO_DEVICE;
+  if (device < 0 || device >= ndev) return SFR_BAD_ARG;
+  sfr_context *ctx = (sfr_context *)calloc(1, sizeof *ctx);
+  if (!ctx) return SFR_BAD_ARG;
+  ctx->device = device;
+  ctx->chunk = 16;
+  ctx->inner_tol = 1e-3;
+  if (hipSetDevice(device) != hipSuccess) { free(ctx); return SFR_NO_DEVICE; }
+  int rc = SFR_OK;
+#define INIT_TRY(call) do { if ((call) != hipSuccess) { rc = SFR_HIP_ERROR; goto fail; } } while (0)
+  INIT_TRY(hipDeviceGetAttribute(&ctx->ncu, hipDeviceAttributeMultiprocessorCount, device));
+  INIT_TRY(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+  INIT_TRY(hipMalloc((void **)&ctx->d_sc, sizeof(sfr_scalars) * SFR_MAXSYS));
+  INIT_TRY(hipMemset(ctx->d_sc, 0, sizeof(sfr_scalars) * SFR_MAXSYS));
+  INIT_TRY(hipHostMalloc((void **)&ctx->h_sc, sizeof(sfr_scalars) * SFR_MAXSYS, 0));
+  INIT_TRY(hipHostMalloc((void **)&ctx->h_poll, sizeof(unsigned) * 2 * 8 * SFR_MAXSYS, 0));
+  INIT_TRY(hipMalloc((void **)&ctx->part, sizeof(double) * SFR_MAXDOT * SFR_MAXG * SFR_MAXSYS));
+  INIT_TRY(hipEventCreate(&ctx->ev_start));
+  INIT_TRY(hipEventCreate(&ctx->ev_stop));
+  INIT_TRY(hipEventCreateWithFlags(&ctx->ev_poll[0], hipEventDisableTiming));
+  INIT_TRY(hipEventCreateWithFlags(&ctx->ev_poll[1], hipEventDisableTiming));
+#undef INIT_TRY
+  *out = ctx;
+  return SFR_OK;
+fail:
+  sfr_finalize(ctx);
+  return rc;
+}
+
+static void drop_graphs(sfr_context *ctx)
+{
+  for (int i = 0; i < ctx->ngraph; i++)
+    if (ctx->graphs[i].exec) hipGraphExecDestroy(ctx->graphs[i].exec);
+  ctx->ngraph = 0;
+}
+
+void sfr_finalize(sfr_context *ctx)
+{
+  if (!ctx) return;
+  hipSetDevice(ctx->device);
+  if (ctx->stream) hipStreamSynchronize(ctx->stream);
+  drop_graphs(ctx);
+  if (ctx->cached) sfr_matrix_destroy(ctx, ctx->cached);
+  for (int i = 0; i < 10; i++) hipFree(ctx->w64[i]);
+  for (int i = 0; i < 6; i++) hipFree(ctx->w32[i]);
+  hipFree(ctx->d_sc);
+  hipFree(ctx->part);
+  if (ctx->h_sc) hipHostFree(ctx->h_sc);
+  if (ctx->h_poll) hipHostFree(ctx->h_poll);
+  if (ctx->ev_start) hipEventDestroy(ctx->ev_start);
+  if (ctx->ev_stop) hipEventDestroy(ctx->ev_stop);
+  if (ctx->ev_poll[0]) hipEventDestroy(ctx->ev_poll[0]);
+  if (ctx->ev_poll[1]) hipEventDestroy(ctx->ev_poll[1]);
+  if (ctx->stream) hipStreamDestroy(ctx->stream);
+  free(ctx);
+}
+
+int sfr_synchronize(sfr_context *ctx)
+{
+  if (!ctx) return SFR_BAD_ARG;
+  TRY(hipStreamSynchronize(ctx->stream));
+  return SFR_OK;
+}
+
+int sfr_dev_alloc(sfr_context *ctx, size_t bytes, void **ptr)
+{
+  if (!ctx || !ptr) return SFR_BAD_ARG;
+  TRY(hipMalloc(ptr, bytes ? bytes : 1));
+  return SFR_OK;
+}
+
+int sfr_dev_free(sfr_context *ctx, void *ptr)
+{
+  if (!ctx) return SFR_BAD_ARG;
+  TRY(hipStreamSynchronize(ctx->stream));
+  TRY(hipFree(ptr));
+  return SFR_OK;
+}
+
+int sfr_copy_h2d(sfr_context *ctx, void *dst, const void *src, size_t bytes)
+{
+  if (!ctx) return SFR_BAD_ARG;
+  if (!bytes) return SFR_OK;
+  TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream));
+  TRY(hipStreamSynchronize(ctx->stream));
+  return SFR_OK;
+}
+
+int sfr_copy_d2h(sfr_context *ctx, void *dst, const void *src, size_t bytes)
+{
+  if (!ctx) return SFR_BAD_ARG;
+  if (!bytes) return SFR_OK;
+  TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
+  TRY(hipStreamSynchronize(ctx->stream));
+  return SFR_OK;
+}
+
+/* Workspace vectors for systems of up to n rows (grow-only). */
+int sfr_workspace(sfr_context *ctx, int64_t n)
+{
+  if (n <= ctx->ws_n) return SFR_OK;
+  TRY(hipStreamSynchronize(ctx->stream));
+  drop_graphs(ctx);                /* captured graphs hold the old buffers */
+  for (int i = 0; i < 10; i++) { hipFree(ctx->w64[i]); ctx->w64[i] = NULL; }
+  for (int i = 0; i < 6; i++) { hipFree(ctx->w32[i]); ctx->w32[i] = NULL; }
+  ctx->ws_n = 0;
+  for (int i = 0; i < 10; i++) TRY(hipMalloc((void **)&ctx->w64[i], sizeof(double) * (size_t)n));
+  for (int i = 0; i < 6; i++) TRY(hipMalloc((void **)&ctx->w32[i], sizeof(float) * (size_t)n));
+  ctx->ws_n = n;
+  return SFR_OK;
+}
+
+sfr_mat_view sfr_view(const sfr_matrix *A)
+{
+  sfr_mat_view v;
+  v.n = A->n; v.fmt = A->fmt; v.slots = A->slots; v.lanes = A->lanes;
+  v.col = A->col; v.rowptr = A->rowptr; v.val = A->val; v.ad = A->ad;
+  v.dinv = A->dinv; v.val32 = A->val32; v.dscale = A->dscale;
+  return v;
+}
+
+/* ---- host conversion ------------------------------------------------------ */
+
+typedef struct { int32_t col; int32_t raw; } entry;
+
+static int cmp_entry(const void *a, const void *b)
+{
+  const entry *x = (const entry *)a, *y = (const entry *)b;
+  if (x->col != y->col) return (x->col > y->col) - (x->col < y->col);
+  return (x->raw > y->raw) - (x->raw < y->raw);
+}
+
+static int valid_storage(short s)
+{
+  return s >= SFR_CSR && s <= SFR_CSRCCOO;
+}
+
+/* Full rows of the system as (col, raw index) lists: off[rowptr[i]..]. */
+static int full_rows(const sfr_system *sys, int64_t **rowptr_out,
+                     entry **ent_out, int32_t **dmap_out, int64_t *nraw_out)
+{
+  const int32_t n = sys->neq;
+  const int32_t *ia = sys->ia, *ja = sys->ja;
+  const short st = sys->storage;
+  const int unsym = sys->unsym != 0;
+  int64_t *cnt = (int64_t *)calloc((size_t)n + 1, sizeof(int64_t));
+  int32_t *dmap = NULL;
+  if (!cnt) return -1;
+  int64_t nraw = 0;
+  const int csrc = (st == SFR_CSRC || st == SFR_CSRCCOO);
+  const int csrd = (st == SFR_CSRD || st == SFR_CSRDCOO);
+  /* every raw value index must fall inside the staged arrays */
+  if (st != SFR_ELLPACK && (ia[0] != 0 || ia[n] != sys->nad || sys->nad < 0)) {
+    free(cnt);
+    return -2;
+  }
+  if (st == SFR_CSR) {
+    dmap = (int32_t *)malloc(sizeof(int32_t) * (size_t)n);
+    if (!dmap) { free(cnt); return -1; }
+    for (int32_t i = 0; i < n; i++) dmap[i] = -1;
+  }
+  /* pass 1: counts */
+  for (int32_t i = 0; i < n; i++) {
+    if (st == SFR_ELLPACK) { cnt[i + 1] += ia[0]; continue; }
+    for (int32_t k = ia[i]; k < ia[i + 1]; k++) {
+      const int32_t j = ja[k];
+      if (j < 0 || j >= n) { free(cnt); free(dmap); return -2; }
+      if (st == SFR_CSR && j == i) { dmap[i] = k; continue; }
+      cnt[i + 1]++;
+      if (st == SFR_CSR && !unsym) cnt[j + 1]++;
+      else if (csrd && !unsym) cnt[j + 1]++;
+      else if (csrc) cnt[j + 1]++;         /* CSRC: sym mirror or au entry */
+    }
+  }
+  if (st == SFR_ELLPACK) nraw = (int64_t)n * ia[0];
+  else if (csrc && unsym) nraw = 2 * (int64_t)sys->nad;
+  else nraw = sys->nad;
+  for (int32_t i = 0; i < n; i++) cnt[i + 1] += cnt[i];
+  entry *ent = (entry *)malloc(sizeof(entry) * (size_t)(cnt[n] > 0 ? cnt[n] : 1));
+  int64_t *pos = (int64_t *)malloc(sizeof(int64_t) * (size_t)n);
+  if (!ent || !pos) { free(cnt); free(dmap); free(ent); free(pos); return -1; }
+  for (int32_t i = 0; i < n; i++) pos[i] = cnt[i];
+  /* pass 2: fill */
+  for (int32_t i = 0; i < n; i++) {
+    if (st == SFR_ELLPACK) {
+      const int s = ia[0];
+      for (int f = 0; f < s; f++) {
+        const int64_t k = (int64_t)i * s + f;
+        const int32_t j = ja[k];
+        if (j < 0 || j >= n) { free(cnt); free(dmap); free(ent); free(pos); return -2; }
+        ent[pos[i]++] = (entry){j, (int32_t)k};
+      }
+      continue;
+    }
+    for (int32_t k = ia[i]; k < ia[i + 1]; k++) {
+      const int32_t j = ja[k];
+      if (st == SFR_CSR && j == i) continue;
+      ent[pos[i]++] = (entry){j, k};
+      if ((st == SFR_CSR || csrd) && !unsym) ent[pos[j]++] = (entry){i, k};
+      else if (csrc) ent[pos[j]++] = (entry){i, unsym ? sys->nad + k : k};
+    }
+  }
+  free(pos);
+  if (st != SFR_ELLPACK)               /* ELLPACK keeps the face order */
+    for (int32_t i = 0; i < n; i++)
+      qsort(ent + cnt[i], (size_t)(cnt[i + 1] - cnt[i]), sizeof(entry), cmp_entry);
+  if (st == SFR_CSR)
+    for (int32_t i = 0; i < n; i++)
+      if (dmap[i] < 0) { free(cnt); free(dmap); free(ent); return -3; }
+  *rowptr_out = cnt;
+  *ent_out = ent;
+  *dmap_out = dmap;
+  *nraw_out = nraw;
+  return 0;
+}
+
+void sfr_matrix_destroy(sfr_context *ctx, sfr_matrix *A)
+{
+  if (!A) return;
+  if (ctx) {
+    hipStreamSynchronize(ctx->stream);
+    if (ctx->cached == A) ctx->cached = NULL;
+    /* graphs captured on this matrix are stale */
+    int w = 0;
+    for (int i = 0; i < ctx->ngraph; i++) {
+      if (ctx->graphs[i].key[0] == (const void *)A) hipGraphExecDestroy(ctx->graphs[i].exec);
+      else ctx->graphs[w++] = ctx->graphs[i];
+    }
+    ctx->ngraph = w;
+  }
+  hipFree(A->col); hipFree(A->rowptr); hipFree(A->vmap); hipFree(A->dmap);
+  hipFree(A->raw); hipFree(A->raw_ad); hipFree(A->val); hipFree(A->ad);
+  hipFree(A->dinv); hipFree(A->dscale); hipFree(A->val32);
+  free(A);
+}
+
+int sfr_matrix_create(sfr_context *ctx, const sfr_system *sys, int format,
+                      sfr_matrix **out)
+{
+  if (!ctx || !sys || !out) return SFR_BAD_ARG;
+  *out = NULL;
+  if (sys->neq <= 0 || !sys->ia || !valid_storage(sys->storage))
+    return sfr_set_error(ctx, SFR_BAD_ARG, "bad system (neq=%d storage=%d)",
+                         sys->neq, sys->storage);
+  if (sys->storage != SFR_CSR && !sys->ad)
+    return sfr_set_error(ctx, SFR_BAD_ARG, "ad is required for storage %d",
+                         sys->storage);
+  if (sys->storage == SFR_ELLPACK && (sys->ia[0] < 0 || sys->ia[0] > 64))
+    return sfr_set_error(ctx, SFR_BAD_ARG, "ELLPACK width %d", sys->ia[0]);
+  int64_t *rp = NULL; entry *ent = NULL; int32_t *dmap = NULL; int64_t nraw = 0;
+  const int frc = full_rows(sys, &rp, &ent, &dmap, &nraw);
+  if (frc == -2)
+    return sfr_set_error(ctx, SFR_BAD_ARG,
+                         "index arrays inconsistent (ia[0] != 0, ia[neq] != nad or a column out of range)");
+  if (frc == -3) return sfr_set_error(ctx, SFR_BAD_ARG, "CSR row without diagonal");
+  if (frc) return sfr_set_error(ctx, SFR_BAD_ARG, "host allocation failed");
+  const int32_t n = sys->neq;
+  int maxrow = 0;
+  for (int32_t i = 0; i < n; i++) {
+    const int64_t l = rp[i + 1] - rp[i];
+    if (l > maxrow) maxrow = (int)l;
+  }
+  int fmt = format;
+  if (fmt == SFR_FMT_AUTO)
+    fmt = (maxrow <= 16 && (int64_t)maxrow * n <= 2 * rp[n] + 16 * (int64_t)n)
+              ? SFR_FMT_ELL : SFR_FMT_CSR;
+  if (fmt != SFR_FMT_ELL && fmt != SFR_FMT_CSR) {
+    free(rp); free(ent); free(dmap);
+    return sfr_set_error(ctx, SFR_BAD_ARG, "format %d", format);
+  }
+  sfr_matrix *A = (sfr_matrix *)calloc(1, sizeof *A);
+  if (!A) { free(rp); free(ent); free(dmap); return SFR_BAD_ARG; }
+  A->n = n; A->fmt = fmt; A->storage = sys->storage; A->unsym = sys->unsym;
+  A->nad = sys->nad; A->nraw = nraw; A->diag_in_al = sys->storage == SFR_CSR;
+  A->h_ia = sys->ia; A->h_ja = sys->ja;
+  A->rpt = 2;
+  A->lanes = 4;
+  int32_t *hcol = NULL, *hmap = NULL, *hptr = NULL;
+  int rc = SFR_OK;
+  if (fmt == SFR_FMT_ELL) {
+    const int s = maxrow > 0 ? maxrow : 1;
+    A->slots = s;
+    A->nnz = (int64_t)s * n;
+    hcol = (int32_t *)malloc(sizeof(int32_t) * (size_t)A->nnz);
+    hmap = (int32_t *)malloc(sizeof(int32_t) * (size_t)A->nnz);
+    if (!hcol || !hmap) { rc = SFR_BAD_ARG; goto done; }
+    for (int32_t i = 0; i < n; i++) {
+      const int64_t b = rp[i], l = rp[i + 1] - rp[i];
+      for (int f = 0; f < s; f++) {
+        const int64_t k = (int64_t)f * n + i;
+        if (f < l) { hcol[k] = ent[b + f].col; hmap[k] = ent[b + f].raw; }
+        else { hcol[k] = i; hmap[k] = -1; }
+      }
+    }
+  } else {
+    A->slots = maxrow;
+    A->nnz = rp[n];
+    hcol = (int32_t *)malloc(sizeof(int32_t) * (size_t)(A->nnz + 1));
+    hmap = (int32_t *)malloc(sizeof(int32_t) * (size_t)(A->nnz + 1));
+    hptr = (int32_t *)malloc(sizeof(int32_t) * ((size_t)n + 1));
+    if (!hcol || !hmap || !hptr) { rc = SFR_BAD_ARG; goto done; }
+    for (int32_t i = 0; i <= n; i++) hptr[i] = (int32_t)rp[i];
+    for (int64_t k = 0; k < A->nnz; k++) { hcol[k] = ent[k].col; hmap[k] = ent[k].raw; }
+    A->lanes = maxrow <= 4 ? 2 : (maxrow <= 12 ? 4 : 8);
+  }
+#define MTRY(call) do { hipError_t e_ = (call); if (e_ != hipSuccess) { \
+    rc = sfr_set_error(ctx, SFR_HIP_ERROR, "%s: %s", #call, hipGetErrorString(e_)); goto done; } } while (0)
+  MTRY(hipMalloc((void **)&A->col, sizeof(int32_t) * (size_t)(A->nnz + 1)));
+  MTRY(hipMalloc((void **)&A->vmap, sizeof(int32_t) * (size_t)(A->nnz + 1)));
+  MTRY(hipMalloc((void **)&A->val, sizeof(double) * (size_t)(A->nnz + 1)));
+  MTRY(hipMalloc((void **)&A->ad, sizeof(double) * (size_t)n));
+  MTRY(hipMalloc((void **)&A->dinv, sizeof(double) * (size_t)n));
+  MTRY(hipMalloc((void **)&A->raw, sizeof(double) * (size_t)(nraw + 1)));
+  MTRY(hipMalloc((void **)&A->raw_ad, sizeof(double) * (size_t)n));
+  MTRY(hipMemcpyAsync(A->col, hcol, sizeof(int32_t) * (size_t)A->nnz, hipMemcpyHostToDevice, ctx->stream));
+  MTRY(hipMemcpyAsync(A->vmap, hmap, sizeof(int32_t) * (size_t)A->nnz, hipMemcpyHostToDevice, ctx->stream));
+  if (hptr) {
+    MTRY(hipMalloc((void **)&A->rowptr, sizeof(int32_t) * ((size_t)n + 1)));
+    MTRY(hipMemcpyAsync(A->rowptr, hptr, sizeof(int32_t) * ((size_t)n + 1), hipMemcpyHostToDevice, ctx->stream));
+  }
+  if (dmap) {
+    MTRY(hipMalloc((void **)&A->dmap, sizeof(int32_t) * (size_t)n));
+    MTRY(hipMemcpyAsync(A->dmap, dmap, sizeof(int32_t) * (size_t)n, hipMemcpyHostToDevice, ctx->stream));
+  }
+  MTRY(hipStreamSynchronize(ctx->stream));
+#undef MTRY
+  rc = sfr_matrix_set_values(ctx, A, sys->al, sys->ad, sys->au);
+done:
+  free(rp); free(ent); free(dmap); free(hcol); free(hmap); free(hptr);
+  if (rc != SFR_OK) { sfr_matrix_destroy(ctx, A); return rc; }
+  *out = A;
+  return SFR_OK;
+}
+
+int sfr_matrix_set_values(sfr_context *ctx, sfr_matrix *A, const double *al,
+                          const double *ad, const double *au)
+{
+  if (!ctx || !A) return SFR_BAD_ARG;
+  if ((A->nraw > 0 && !al) || (!A->diag_in_al && !ad))
+    return sfr_set_error(ctx, SFR_BAD_ARG, "missing values");
+  const int csrc_unsym = (A->storage == SFR_CSRC || A->storage == SFR_CSRCCOO) && A->unsym;
+  if (csrc_unsym) {
+    TRY(hipMemcpyAsync(A->raw, al, sizeof(double) * (size_t)A->nad, hipMemcpyHostToDevice, ctx->stream));
+    TRY(hipMemcpyAsync(A->raw + A->nad, au ? au : al + A->nad, sizeof(double) * (size_t)A->nad,
+                       hipMemcpyHostToDevice, ctx->stream));
+  } else if (A->nraw > 0) {
+    TRY(hipMemcpyAsync(A->raw, al, sizeof(double) * (size_t)A->nraw, hipMemcpyHostToDevice, ctx->stream));
+  }
+  if (!A->diag_in_al)
+    TRY(hipMemcpyAsync(A->raw_ad, ad, sizeof(double) * (size_t)A->n, hipMemcpyHostToDevice, ctx->stream));
+  TRY(sfr_k_gather(ctx->stream, A->nnz, A->vmap, A->raw, A->val));
+  TRY(sfr_k_diag(ctx->stream, A->n, A->diag_in_al ? A->dmap : NULL, A->raw, A->raw_ad, A->ad, A->dinv));
+  TRY(hipStreamSynchronize(ctx->stream));
+  A->have32 = 0;
+  return SFR_OK;
+}
+
+int sfr_matrix_prepare_mixed(sfr_context *ctx, sfr_matrix *A)
+{
+  if (A->have32) return SFR_OK;
+  if (!A->val32) TRY(hipMalloc((void **)&A->val32, sizeof(float) * (size_t)(A->nnz + 1)));
+  if (!A->dscale) TRY(hipMalloc((void **)&A->dscale, sizeof(double) * (size_t)A->n));
+  sfr_mat_view v = sfr_view(A);
+  TRY(sfr_k_scale32(ctx->stream, &v, A->dscale, A->val32));
+  A->have32 = 1;
+  return SFR_OK;
+}
+
+int sfr_matrix_info(const sfr_matrix *A, int32_t *neq, int *format, int *slots,
+                    int64_t *nnz)
+{
+  if (!A) return SFR_BAD_ARG;
+  if (neq) *neq = A->n;
+  if (format) *format = A->fmt;
+  if (slots) *slots = A->slots;
+  if (nnz) *nnz = A->nnz;
+  return SFR_OK;
+}

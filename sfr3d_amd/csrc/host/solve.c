@@ -1,0 +1,389 @@
+/*
+ * Krylov drivers (C host side of the device solve) and the public solves.
+ *
+ * A solve is: setup kernels, then the iteration body captured once as a HIP
+ * graph of `chunk` iterations and replayed; the device decides convergence
+ * itself (scalars never leave the GPU inside the loop) and every kernel of an
+ * iteration that starts after convergence returns at once.  The host keeps
+ * two chunks in flight and polls a 32-byte status copy after each, so the GPU
+ * is never idle waiting for the host.  The final kernel computes x.Ax, ||x||,
+ * the FP64 residual and its M-norm, as the reference's pcg does after its
+ * loop (src/Cg.c:155-176).
+ */
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include "../sfr_internal.h"
+
+#define TRY(call)                                                            \
+  do {                                                                       \
+    hipError_t e_ = (hipError_t)(call);                                      \
+    if (e_ != hipSuccess)                                                    \
+      return sfr_set_error(ctx, SFR_HIP_ERROR, "%s: %s (%s:%d)", #call,      \
+                           hipGetErrorString(e_), __FILE__, __LINE__);       \
+  } while (0)
+
+#define CHECK(call) do { int rc_ = (call); if (rc_ != SFR_OK) return rc_; } while (0)
+
+int sfr_workspace(sfr_context *ctx, int64_t n);
+int sfr_matrix_prepare_mixed(sfr_context *ctx, sfr_matrix *A);
+
+enum { G_PCG = 1, G_BICG = 2, G_MX = 3 };
+
+/* Capture `chunk` iterations of one loop body, or reuse a captured one. */
+static int get_graph(sfr_context *ctx, int kind, const void *k0,
+                     const void *k1, const void *k2, const sfr_mat_view *v,
+                     sfr_geom g, hipGraphExec_t *exec)
+{
+  const int chunk = ctx->chunk;
+  for (int i = 0; i < ctx->ngraph; i++) {
+    sfr_graph *e = &ctx->graphs[i];
+    if (e->kind == kind && e->chunk == chunk && e->key[0] == k0 &&
+        e->key[1] == k1 && e->key[2] == k2 &&
+        e->key[3] == (const void *)ctx->w64[0] && e->key[4] == (const void *)(intptr_t)g.rpt) {
+      *exec = e->exec;
+      return SFR_OK;
+    }
+  }
+  if (ctx->ngraph == (int)(sizeof ctx->graphs / sizeof ctx->graphs[0])) {
+    hipGraphExecDestroy(ctx->graphs[0].exec);
+    memmove(&ctx->graphs[0], &ctx->graphs[1], sizeof(sfr_graph) * (size_t)(ctx->ngraph - 1));
+    ctx->ngraph--;
+  }
+  hipStream_t s = ctx->stream;
+  double *const *w = ctx->w64;
+  float *const *w32 = ctx->w32;
+  TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+  int lrc = hipSuccess;
+  for (int it = 0; it < chunk && lrc == hipSuccess; it++) {
+    if (kind == G_PCG)
+      lrc = sfr_k_pcg_iter(s, v, g, (double *)k1, w[0], w[1], w[2], ctx->d_sc, ctx->part);
+    else if (kind == G_BICG)
+      lrc = sfr_k_bicg_iter(s, v, g, (double *)k1, w[0], w[1], w[2], w[3], w[4], w[5],
+                            ctx->d_sc, ctx->part);
+    else
+      lrc = sfr_k_mx_inner_iter(s, v, g, w32[0], w32[1], w32[2], w32[3], ctx->d_sc, ctx->part);
+  }
+  hipGraph_t graph = NULL;
+  const hipError_t erc = hipStreamEndCapture(s, &graph);
+  if (lrc != hipSuccess || erc != hipSuccess) {
+    if (graph) hipGraphDestroy(graph);
+    return sfr_set_error(ctx, SFR_HIP_ERROR, "graph capture: %s",
+                         hipGetErrorString(lrc != hipSuccess ? (hipError_t)lrc : erc));
+  }
+  hipGraphExec_t ex = NULL;
+  const hipError_t irc = hipGraphInstantiate(&ex, graph, NULL, NULL, 0);
+  hipGraphDestroy(graph);
+  if (irc != hipSuccess)
+    return sfr_set_error(ctx, SFR_HIP_ERROR, "graph instantiate: %s", hipGetErrorString(irc));
+  sfr_graph *e = &ctx->graphs[ctx->ngraph++];
+  memset(e, 0, sizeof *e);
+  e->exec = ex; e->kind = kind; e->chunk = chunk;
+  e->key[0] = k0; e->key[1] = k1; e->key[2] = k2;
+  e->key[3] = (const void *)ctx->w64[0];
+  e->key[4] = (const void *)(intptr_t)g.rpt;
+  *exec = ex;
+  return SFR_OK;
+}
+
+/* Replay the loop graph until the device reports the watched flag set.
+ * which = 0: sc->done; which = 1: sc->inner_done.  Two chunks in flight. */
+static int run_loop(sfr_context *ctx, hipGraphExec_t ex, int which,
+                    unsigned max_chunks)
+{
+  hipStream_t s = ctx->stream;
+  unsigned launched = 0, seen = 0;
+  for (;;) {
+    while (launched - seen < 2 && launched < max_chunks) {
+      const int slot = (int)(launched & 1);
+      TRY(hipGraphLaunch(ex, s));
+      TRY(hipMemcpyAsync(ctx->h_poll + 8 * slot, &ctx->d_sc->iter, 8 * sizeof(unsigned),
+                         hipMemcpyDeviceToHost, s));
+      TRY(hipEventRecord(ctx->ev_poll[slot], s));
+      launched++;
+    }
+    if (seen == launched) break;                 /* chunk budget exhausted */
+    const int slot = (int)(seen & 1);
+    TRY(hipEventSynchronize(ctx->ev_poll[slot]));
+    seen++;
+    /* layout from &iter: iter, done, maxit, seal, counter, inner_iter, inner_done */
+    const unsigned *p = ctx->h_poll + 8 * slot;
+    const unsigned flag = which ? p[6] : p[1];
+    if (flag) break;
+  }
+  return SFR_OK;
+}
+
+static void fill_stats(const sfr_scalars *sc, sfr_stats *st, float ms)
+{
+  if (!st) return;
+  memset(st, 0, sizeof *st);
+  st->iters = sc->iter;
+  st->outer = sc->outer;
+  st->tol = sc->tol;
+  st->norm_b = sc->norm_b;
+  st->conv = sc->conv;
+  st->xKx = sc->res[0];
+  st->norm_x = sqrt(sc->res[1]);
+  st->norm_r_m = sqrt(fabs(sc->res[2]));
+  st->norm_r = sqrt(sc->res[3]);
+  st->relres = sc->bb > 0.0 ? st->norm_r / sqrt(sc->bb) : 0.0;
+  st->time_s = ms * 1e-3;
+  switch (sc->done) {
+  case SFR_DONE_CONV: st->status = SFR_OK; break;
+  case SFR_DONE_MAXIT: st->status = SFR_MAXIT; break;
+  case SFR_DONE_BREAK: st->status = SFR_BREAKDOWN; break;
+  default: st->status = SFR_MAXIT; break;
+  }
+}
+
+static int finish(sfr_context *ctx, const sfr_mat_view *v, sfr_geom g,
+                  const double *d_b, const double *d_x, sfr_stats *st)
+{
+  TRY(sfr_k_final(ctx->stream, v, g, d_b, d_x, ctx->d_sc, ctx->part));
+  TRY(hipEventRecord(ctx->ev_stop, ctx->stream));
+  TRY(hipMemcpyAsync(ctx->h_sc, ctx->d_sc, sizeof(sfr_scalars), hipMemcpyDeviceToHost,
+                     ctx->stream));
+  TRY(hipStreamSynchronize(ctx->stream));
+  float ms = 0.0f;
+  TRY(hipEventElapsedTime(&ms, ctx->ev_start, ctx->ev_stop));
+  sfr_stats tmp;
+  fill_stats(ctx->h_sc, &tmp, ms);
+  if (st) *st = tmp;
+  return tmp.status;
+}
+
+static unsigned chunk_budget(const sfr_context *ctx, unsigned maxit)
+{
+  const unsigned c = (unsigned)ctx->chunk;
+  return maxit / c + 2;
+}
+
+static int solve_pcg64(sfr_context *ctx, sfr_matrix *A, double tol,
+                       unsigned maxit, const double *d_b, double *d_x, int newX,
+                       sfr_stats *st)
+{
+  CHECK(sfr_workspace(ctx, A->n));
+  sfr_mat_view v = sfr_view(A);
+  sfr_geom g = {A->rpt, 0};
+  hipGraphExec_t ex;
+  CHECK(get_graph(ctx, G_PCG, A, d_x, NULL, &v, g, &ex));
+  TRY(hipEventRecord(ctx->ev_start, ctx->stream));
+  TRY(sfr_k_pcg_setup(ctx->stream, &v, g, d_b, d_x, ctx->w64[0], ctx->w64[1], newX, tol,
+                      maxit, ctx->d_sc, ctx->part));
+  CHECK(run_loop(ctx, ex, 0, chunk_budget(ctx, maxit)));
+  return finish(ctx, &v, g, d_b, d_x, st);
+}
+
+static int solve_bicg64(sfr_context *ctx, sfr_matrix *A, double tol,
+                        unsigned maxit, const double *d_b, double *d_x, int newX,
+                        sfr_stats *st)
+{
+  CHECK(sfr_workspace(ctx, A->n));
+  sfr_mat_view v = sfr_view(A);
+  sfr_geom g = {A->rpt, 0};
+  hipGraphExec_t ex;
+  CHECK(get_graph(ctx, G_BICG, A, d_x, NULL, &v, g, &ex));
+  double *const *w = ctx->w64;
+  TRY(hipEventRecord(ctx->ev_start, ctx->stream));
+  TRY(sfr_k_bicg_setup(ctx->stream, &v, g, d_b, d_x, w[0], w[1], w[2], w[3], newX, tol,
+                       maxit, ctx->d_sc, ctx->part));
+  CHECK(run_loop(ctx, ex, 0, chunk_budget(ctx, maxit)));
+  return finish(ctx, &v, g, d_b, d_x, st);
+}
+
+static int solve_pcg_mixed(sfr_context *ctx, sfr_matrix *A, double tol,
+                           unsigned maxit, const double *d_b, double *d_x,
+                           int newX, sfr_stats *st)
+{
+  CHECK(sfr_workspace(ctx, A->n));
+  CHECK(sfr_matrix_prepare_mixed(ctx, A));
+  sfr_mat_view v = sfr_view(A);
+  sfr_geom g = {A->rpt, 0};
+  hipGraphExec_t ex;
+  CHECK(get_graph(ctx, G_MX, A, NULL, NULL, &v, g, &ex));
+  float *const *w32 = ctx->w32;
+  hipStream_t s = ctx->stream;
+  TRY(hipEventRecord(ctx->ev_start, s));
+  /* norms of b, tolerance (k_pcg_setup's first kernel), x = 0 if newX */
+  TRY(sfr_k_pcg_norm(s, &v, g, d_b, d_x, newX, tol, maxit, ctx->d_sc, ctx->part));
+  int first = 1;
+  for (unsigned outer = 0;; outer++) {
+    TRY(sfr_k_mx_outer(s, &v, g, d_b, d_x, w32[0], first, newX, w32[1], w32[2], w32[0],
+                       ctx->inner_tol, maxit, ctx->d_sc, ctx->part));
+    first = 0;
+    TRY(hipMemcpyAsync(ctx->h_poll, &ctx->d_sc->iter, 8 * sizeof(unsigned),
+                       hipMemcpyDeviceToHost, s));
+    TRY(hipStreamSynchronize(s));
+    if (ctx->h_poll[1] != SFR_DONE_RUN) break;
+    CHECK(run_loop(ctx, ex, 1, chunk_budget(ctx, maxit)));
+    if (outer > maxit) break;
+  }
+  return finish(ctx, &v, g, d_b, d_x, st);
+}
+
+int sfr_solve_device(sfr_context *ctx, sfr_matrix *A, int solver,
+                     int precision, double tol, unsigned maxit,
+                     const double *d_b, double *d_x, int newX, sfr_stats *st)
+{
+  if (!ctx || !A || !d_b || !d_x) return SFR_BAD_ARG;
+  if (maxit == 0) return sfr_set_error(ctx, SFR_BAD_ARG, "maxit = 0");
+  TRY(hipSetDevice(ctx->device));
+  if (solver == SFR_PCG)
+    return precision == SFR_PREC_MIXED
+               ? solve_pcg_mixed(ctx, A, tol, maxit, d_b, d_x, newX, st)
+               : solve_pcg64(ctx, A, tol, maxit, d_b, d_x, newX, st);
+  if (solver == SFR_PBICGSTAB) {
+    if (precision == SFR_PREC_MIXED)
+      return sfr_set_error(ctx, SFR_NOT_SUPPORTED, "mixed BiCGSTAB");
+    return solve_bicg64(ctx, A, tol, maxit, d_b, d_x, newX, st);
+  }
+  return sfr_set_error(ctx, SFR_BAD_ARG, "solver id %d", solver);
+}
+
+/* Average duration of one launch of a loop kernel, timed with HIP events on
+ * the context stream: which = 0 FP64 PCG SpMV (+p.q), 1 FP32 inner SpMV. */
+int sfr_kernel_time(sfr_context *ctx, sfr_matrix *A, int which, int reps,
+                    double *ms_per_launch)
+{
+  if (!ctx || !A || reps <= 0 || !ms_per_launch) return SFR_BAD_ARG;
+  TRY(hipSetDevice(ctx->device));
+  CHECK(sfr_workspace(ctx, A->n));
+  if (which == 1) CHECK(sfr_matrix_prepare_mixed(ctx, A));
+  sfr_mat_view v = sfr_view(A);
+  sfr_geom g = {A->rpt, 0};
+  hipStream_t s = ctx->stream;
+  TRY(hipMemsetAsync(ctx->d_sc, 0, sizeof(sfr_scalars), s));
+  TRY(sfr_k_fill(s, A->n, 1.0, ctx->w64[1]));
+  TRY(sfr_k_fill32(s, A->n, 1.0f, ctx->w32[2]));
+  for (int warm = 0; warm < 2; warm++) {
+    if (which == 0) TRY(sfr_k_pcg_spmv_only(s, &v, g, ctx->w64[1], ctx->w64[2], ctx->d_sc, ctx->part));
+    else TRY(sfr_k_mx_spmv_only(s, &v, g, ctx->w32[2], ctx->w32[3], ctx->d_sc, ctx->part));
+  }
+  TRY(hipEventRecord(ctx->ev_start, s));
+  for (int r = 0; r < reps; r++) {
+    if (which == 0) TRY(sfr_k_pcg_spmv_only(s, &v, g, ctx->w64[1], ctx->w64[2], ctx->d_sc, ctx->part));
+    else TRY(sfr_k_mx_spmv_only(s, &v, g, ctx->w32[2], ctx->w32[3], ctx->d_sc, ctx->part));
+  }
+  TRY(hipEventRecord(ctx->ev_stop, s));
+  TRY(hipEventSynchronize(ctx->ev_stop));
+  float ms = 0.0f;
+  TRY(hipEventElapsedTime(&ms, ctx->ev_start, ctx->ev_stop));
+  *ms_per_launch = ms / reps;
+  return SFR_OK;
+}
+
+/* Host-pointer solve: the pattern is uploaded once and recognised again by
+ * the identity of ia/ja (SFR3D re-assembles values on a fixed pattern). */
+int sfr_solve(sfr_context *ctx, const sfr_system *sys, int solver, double tol,
+              unsigned maxit, const double *b, double *x, int newX,
+              sfr_stats *st)
+{
+  if (!ctx || !sys || !b || !x) return SFR_BAD_ARG;
+  TRY(hipSetDevice(ctx->device));
+  sfr_matrix *A = ctx->cached;
+  if (A && (A->h_ia != sys->ia || A->h_ja != sys->ja || A->n != sys->neq ||
+            A->nad != sys->nad || A->storage != sys->storage ||
+            A->unsym != sys->unsym)) {
+    sfr_matrix_destroy(ctx, A);
+    A = NULL;
+  }
+  if (A) {
+    CHECK(sfr_matrix_set_values(ctx, A, sys->al, sys->ad, sys->au));
+  } else {
+    CHECK(sfr_matrix_create(ctx, sys, SFR_FMT_AUTO, &A));
+    ctx->cached = A;
+  }
+  CHECK(sfr_workspace(ctx, A->n));
+  double *d_b = ctx->w64[8], *d_x = ctx->w64[9];
+  const size_t bytes = sizeof(double) * (size_t)A->n;
+  TRY(hipMemcpyAsync(d_b, b, bytes, hipMemcpyHostToDevice, ctx->stream));
+  if (!newX) TRY(hipMemcpyAsync(d_x, x, bytes, hipMemcpyHostToDevice, ctx->stream));
+  const int rc = sfr_solve_device(ctx, A, solver, SFR_PREC_FP64, tol, maxit, d_b, d_x,
+                                  newX, st);
+  if (rc == SFR_HIP_ERROR || rc == SFR_BAD_ARG || rc == SFR_NOT_SUPPORTED) return rc;
+  TRY(hipMemcpyAsync(x, d_x, bytes, hipMemcpyDeviceToHost, ctx->stream));
+  TRY(hipStreamSynchronize(ctx->stream));
+  return rc;
+}
+
+/* ---- solverC drop-in -------------------------------------------------------- */
+
+static sfr_context *g_ctx;
+
+void sfr_solverC(void *m, int nEq, int nEqNov, int nAd, int nAdR, int *ia,
+                 int *ja, double *al, double *ad, double *au, double *b,
+                 double *x, void *iNeq, void *bOmp, double tol,
+                 unsigned int maxIt, short storage, short solver,
+                 FILE *fSolvLog, bool fLog, bool newX, bool unSym,
+                 const char *name)
+{
+  (void)m; (void)iNeq; (void)bOmp;
+  if (solver != SFR_PCG && solver != SFR_PBICGSTAB) {
+    fprintf(stderr, "Opcao %d e invalida!!\n", solver);
+    fprintf(stderr, "Arquivo:%s\nFonte:  %s\n", __FILE__, __func__);
+    exit(EXIT_FAILURE);
+  }
+  if (storage < SFR_CSR || storage > SFR_CSRCCOO) {
+    fprintf(stderr, "Opcao %d e invalida!!\n", storage);
+    fprintf(stderr, "Arquivo:%s\nFonte:  %s\n", __FILE__, __func__);
+    exit(EXIT_FAILURE);
+  }
+  if (nAdR != 0 || nEqNov != nEq) {
+    fprintf(stderr, "sfr_solverC: distributed systems (nAdR=%d, nEqNov=%d != nEq=%d)"
+            " are not supported\n", nAdR, nEqNov, nEq);
+    exit(EXIT_FAILURE);
+  }
+  if (!g_ctx) {
+    const char *dev = getenv("SFR_DEVICE");
+    const int rc = sfr_init(&g_ctx, dev ? atoi(dev) : 0);
+    if (rc != SFR_OK) {
+      fprintf(stderr, "sfr_solverC: %s\n", sfr_status_string(rc));
+      exit(EXIT_FAILURE);
+    }
+  }
+  const sfr_system sys = {nEq, nAd, ia, ja, al, ad, au, storage, (short)unSym};
+  sfr_stats st;
+  const int rc = sfr_solve(g_ctx, &sys, solver, tol, maxIt, b, x, newX, &st);
+  if (rc == SFR_HIP_ERROR || rc == SFR_BAD_ARG || rc == SFR_NOT_SUPPORTED ||
+      rc == SFR_NO_DEVICE) {
+    fprintf(stderr, "sfr_solverC: %s: %s\n", sfr_status_string(rc), sfr_last_error(g_ctx));
+    exit(EXIT_FAILURE);
+  }
+  if (rc != SFR_OK) {                 /* j == maxIt (and breakdowns) */
+    printf("name : %s\n", name ? name : "");
+    if (solver == SFR_PCG) {
+      printf("PCG: *** WARNING: no convergende reached !\n");
+      printf("MAXIT = %d \n", maxIt);
+      exit(EXIT_FAILURE);
+    }
+    printf(" PBICGSTAB *** WARNING: no convergende reached !\n");
+    printf("MAXIT = %d \n", maxIt);
+    exit(-2);                          /* EXIT_SOLVER */
+  }
+  if (fLog && fSolvLog) {
+    if (solver == SFR_PCG)
+      fprintf(fSolvLog, "PCG: tol %20.2e iteration %d xKx %20.12e norma(x*x) %20.12e "
+              "time %20.5lf\n", tol, (int)st.iters, st.xKx, st.norm_x, st.time_s);
+    else
+      fprintf(fSolvLog, "PBICGSTAB: tol %20.2e iteration %d xKx %20.12e ||x*x|| %20.12e "
+              "time %20.5lf\n", tol, (int)st.iters, st.xKx, st.norm_x, st.time_s);
+  }
+}
+
+int sfr_set_option(sfr_context *ctx, const char *name, double value)
+{
+  if (!ctx || !name) return SFR_BAD_ARG;
+  if (!strcmp(name, "chunk")) {
+    if (value < 1 || value > 256) return sfr_set_error(ctx, SFR_BAD_ARG, "chunk %g", value);
+    ctx->chunk = (int)value;
+    return SFR_OK;
+  }
+  if (!strcmp(name, "inner_tol")) {
+    if (!(value > 0 && value < 1)) return sfr_set_error(ctx, SFR_BAD_ARG, "inner_tol %g", value);
+    ctx->inner_tol = value;
+    return SFR_OK;
+  }
+  return sfr_set_error(ctx, SFR_BAD_ARG, "unknown option %s", name);
+}

@@ -1,0 +1,173 @@
+/*
+ * Internal interface between the C host layer (drivers, matrix conversion,
+ * public ABI) and the HIP kernel layer (kernels + extern "C" launchers).
+ * Plain C: included by gcc-compiled host files and by hipcc-compiled .hip
+ * files alike.
+ */
+#ifndef SFR_INTERNAL_H
+#define SFR_INTERNAL_H
+
+#include <stddef.h>
+#include <stdint.h>
+#include <hip/hip_runtime_api.h>
+#include "sfr3d.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SFR_BLOCK   256          /* threads per workgroup (4 wave64)        */
+#define SFR_MAXG    16384        /* max workgroups of a reducing kernel     */
+#define SFR_MAXDOT  8            /* partial sums one kernel may reduce      */
+#define SFR_MAXSYS  8            /* systems of one shared-pattern batch     */
+
+/* Scalars of one Krylov solve, resident on the device.  The last workgroup
+ * of each reducing kernel writes them; later kernels read them. */
+typedef struct sfr_scalars {
+  double   d;          /* PCG: r.D^-1 r   BiCGSTAB: r.r0   inner CG: r.r   */
+  double   alpha, beta, omega;
+  double   norm_b;     /* reference norm of b                               */
+  double   conv;       /* tol * norm_b                                      */
+  double   bb;         /* b.b                                               */
+  double   tol;
+  double   rho0;       /* inner: initial r.r                                */
+  double   inner_tol2; /* inner: stop when r.r <= inner_tol2 * rho0         */
+  double   res[6];     /* final / outer dots                                */
+  unsigned iter;       /* completed iterations                              */
+  unsigned done;       /* SFR_DONE_*                                        */
+  unsigned maxit;
+  unsigned seal;       /* set once the loop has been seen finished          */
+  unsigned counter;    /* last-workgroup arrival counter                    */
+  unsigned inner_iter, inner_done, inner_maxit;
+  unsigned outer;      /* mixed: FP64 refinement steps taken                */
+  unsigned pad[7];
+} sfr_scalars;
+
+enum { SFR_DONE_RUN = 0, SFR_DONE_CONV = 1, SFR_DONE_BREAK = 2,
+       SFR_DONE_MAXIT = 3 };
+
+/* Read-only view of a device matrix handed to the launchers. */
+typedef struct sfr_mat_view {
+  int64_t        n;
+  int            fmt;       /* SFR_FMT_ELL / SFR_FMT_CSR                 */
+  int            slots;     /* ELL slots                                 */
+  int            lanes;     /* CSR lanes per row                         */
+  const int32_t *col;
+  const int32_t *rowptr;
+  const double  *val;
+  const double  *ad;
+  const double  *dinv;
+  const float   *val32;     /* D^-1/2 A D^-1/2 off-diagonals (unit diag) */
+  const double  *dscale;    /* 1/sqrt(ad)                                */
+} sfr_mat_view;
+
+typedef struct sfr_geom { int rpt; int grid; } sfr_geom;
+
+struct sfr_matrix {
+  int32_t  n;
+  int      fmt, slots, lanes;
+  int64_t  nnz;          /* stored entries of the device format            */
+  short    storage, unsym;
+  int32_t  nad;
+  int64_t  nraw;         /* raw off-diagonal values of the host storage    */
+  int      diag_in_al;   /* storage CSR: diagonal inside al                */
+  int      have32;       /* FP32 scaled copy is current                    */
+  int      rpt;          /* tuned rows per thread                          */
+  int32_t *col, *rowptr, *vmap, *dmap;
+  double  *raw, *raw_ad, *val, *ad, *dinv, *dscale;
+  float   *val32;
+  const int32_t *h_ia, *h_ja;   /* host pattern identity (value-only uploads) */
+};
+
+typedef struct sfr_graph {
+  hipGraphExec_t exec;
+  const void    *key[8];
+  int            kind, chunk;
+} sfr_graph;
+
+struct sfr_context {
+  int          device, ncu;
+  hipStream_t  stream;
+  char         err[512];
+  sfr_scalars *d_sc, *h_sc;     /* SFR_MAXSYS each; h_sc is pinned        */
+  double      *part;            /* SFR_MAXDOT * SFR_MAXG * SFR_MAXSYS      */
+  hipEvent_t   ev_start, ev_stop, ev_poll[2];
+  unsigned    *h_poll;          /* pinned [2][4]: done, iter, inner_done   */
+  int64_t      ws_n;
+  double      *w64[10];
+  float       *w32[6];
+  sfr_graph    graphs[16];
+  int          ngraph;
+  sfr_matrix  *cached;          /* pattern cache of the host-pointer path  */
+  int          chunk;           /* iterations per captured chunk           */
+  double       inner_tol;       /* MIXED inner relative reduction          */
+};
+
+int sfr_set_error(sfr_context *ctx, int code, const char *fmt, ...);
+sfr_mat_view sfr_view(const sfr_matrix *A);
+sfr_geom     sfr_geom_for(const sfr_context *ctx, int64_t n, int rpt);
+
+/* ---- launchers (the .hip files); each enqueues on `s`, returns a hipError_t */
+/* matrix values */
+int sfr_k_gather(hipStream_t s, int64_t nnz, const int32_t *vmap,
+                 const double *raw, double *val);
+int sfr_k_diag(hipStream_t s, int64_t n, const int32_t *dmap,
+               const double *raw, const double *raw_ad, double *ad,
+               double *dinv);
+int sfr_k_scale32(hipStream_t s, const sfr_mat_view *A, double *dscale,
+                  float *val32);
+int sfr_k_fill(hipStream_t s, int64_t n, double v, double *x);
+int sfr_k_spmv(hipStream_t s, const sfr_mat_view *A, const double *x,
+               double *y);
+int sfr_k_fill32(hipStream_t s, int64_t n, float v, float *x);
+/* single SpMV kernels of the PCG / inner loops (measurement hooks) */
+int sfr_k_pcg_spmv_only(hipStream_t s, const sfr_mat_view *A, sfr_geom g,
+                        const double *p, double *q, sfr_scalars *sc,
+                        double *part);
+int sfr_k_mx_spmv_only(hipStream_t s, const sfr_mat_view *A, sfr_geom g,
+                       const float *p, float *q, sfr_scalars *sc,
+                       double *part);
+
+/* PCG, FP64 (reference src/Cg.c:47-209) */
+int sfr_k_pcg_setup(hipStream_t s, const sfr_mat_view *A, sfr_geom g,
+                    const double *b, double *x, double *r, double *p,
+                    int newX, double tol, unsigned maxit, sfr_scalars *sc,
+                    double *part);
+int sfr_k_pcg_iter(hipStream_t s, const sfr_mat_view *A, sfr_geom g,
+                   double *x, double *r, double *p, double *q,
+                   sfr_scalars *sc, double *part);
+/* b.D^-1 b, b.b, conv = tol*sqrt|b.D^-1 b|, x = 0 when newX */
+int sfr_k_pcg_norm(hipStream_t s, const sfr_mat_view *A, sfr_geom g,
+                   const double *b, double *x, int newX, double tol,
+                   unsigned maxit, sfr_scalars *sc, double *part);
+/* final quantities: x.Ax, x.x, r.D^-1 r, r.r with r = b - Ax -> sc->res */
+int sfr_k_final(hipStream_t s, const sfr_mat_view *A, sfr_geom g,
+                const double *b, const double *x, sfr_scalars *sc,
+                double *part);
+
+/* BiCGSTAB, FP64 (reference src/Bicgstab.c:50-245) */
+int sfr_k_bicg_setup(hipStream_t s, const sfr_mat_view *A, sfr_geom g,
+                     const double *b, double *x, double *r, double *r0,
+                     double *p, double *z, int newX, double tol,
+                     unsigned maxit, sfr_scalars *sc, double *part);
+int sfr_k_bicg_iter(hipStream_t s, const sfr_mat_view *A, sfr_geom g,
+                    double *x, double *r, const double *r0, double *p,
+                    double *z, double *v, double *t, sfr_scalars *sc,
+                    double *part);
+
+/* mixed precision: FP64 outer residual + FP32 inner CG on D^-1/2 A D^-1/2 */
+/* outer step: x += D^-1/2 y (skipped when first), r = b - Ax in FP64, stop
+ * test, next inner right-hand side rh = ph = D^-1/2 r, y = 0 */
+int sfr_k_mx_outer(hipStream_t s, const sfr_mat_view *A, sfr_geom g,
+                   const double *b, double *x, const float *y, int first,
+                   int newX, float *rh, float *ph, float *yz,
+                   double inner_tol, unsigned inner_maxit, sfr_scalars *sc,
+                   double *part);
+int sfr_k_mx_inner_iter(hipStream_t s, const sfr_mat_view *A, sfr_geom g,
+                        float *y, float *rh, float *ph, float *qh,
+                        sfr_scalars *sc, double *part);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SFR_INTERNAL_H */

@@ -1,0 +1,154 @@
+"""Python face of the device solve (C ABI of include/sfr3d.h).
+
+Mirrors the reference's solver boundary, solverC (reference
+include/Solv.h:59-70): a system in one of SFR3D's storages, a solver id
+(PCG = 1, PBICGSTAB = 2), tol, maxit, b and x.  Everything here calls the
+HIP library; there is no CPU path.
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import _lib as L
+from ._lib import lib, SfrSystem, SfrStats, ptr
+
+
+class SolverError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"{lib.sfr_status_string(code).decode()}: {msg}")
+        self.code = code
+
+
+class System:
+    """Host arrays of one system in a reference storage (SistEq fields)."""
+
+    def __init__(self, neq, nad, ia, ja, al, ad, storage, unsym, au=None):
+        self.neq, self.nad = int(neq), int(nad)
+        self.ia = np.ascontiguousarray(ia, np.int32)
+        self.ja = np.ascontiguousarray(ja, np.int32)
+        self.al = np.ascontiguousarray(al, np.float64)
+        self.ad = None if ad is None else np.ascontiguousarray(ad, np.float64)
+        self.au = None if au is None else np.ascontiguousarray(au, np.float64)
+        self.storage, self.unsym = int(storage), bool(unsym)
+
+    def c(self):
+        s = SfrSystem()
+        s.neq, s.nad = self.neq, self.nad
+        s.ia, s.ja, s.al = ptr(self.ia), ptr(self.ja), ptr(self.al)
+        s.ad = ptr(self.ad) if self.ad is not None else ptr(self.al)
+        s.au = ptr(self.au)
+        s.storage, s.unsym = self.storage, 1 if self.unsym else 0
+        return s
+
+
+class DeviceArray:
+    """Plain device allocation owned by a Context."""
+
+    def __init__(self, ctx, nbytes, dtype, shape):
+        self.ctx, self.nbytes, self.dtype, self.shape = ctx, nbytes, dtype, shape
+        p = C.c_void_p()
+        ctx._check(lib.sfr_dev_alloc(ctx.h, nbytes, C.byref(p)))
+        self.ptr = p.value
+
+    def set(self, arr):
+        arr = np.ascontiguousarray(arr, self.dtype)
+        assert arr.nbytes == self.nbytes
+        self.ctx._check(lib.sfr_copy_h2d(self.ctx.h, self.ptr, ptr(arr),
+                                         arr.nbytes))
+        return self
+
+    def get(self):
+        out = np.empty(self.shape, self.dtype)
+        self.ctx._check(lib.sfr_copy_d2h(self.ctx.h, ptr(out), self.ptr,
+                                         out.nbytes))
+        return out
+
+    def free(self):
+        if self.ptr:
+            lib.sfr_dev_free(self.ctx.h, self.ptr)
+            self.ptr = None
+
+
+class DeviceMatrix:
+    def __init__(self, ctx, system, fmt=L.FMT_AUTO):
+        self.ctx = ctx
+        self.system = system            # keeps host arrays alive
+        h = C.c_void_p()
+        s = system.c()
+        ctx._check(lib.sfr_matrix_create(ctx.h, C.byref(s), fmt, C.byref(h)))
+        self.h = h.value
+
+    def set_values(self, al, ad, au=None):
+        al = np.ascontiguousarray(al, np.float64)
+        ad = None if ad is None else np.ascontiguousarray(ad, np.float64)
+        self._keep = (al, ad, au)
+        self.ctx._check(lib.sfr_matrix_set_values(self.ctx.h, self.h, ptr(al),
+                                                  ptr(ad), ptr(au)))
+
+    def info(self):
+        n, f, s, nnz = C.c_int32(), C.c_int(), C.c_int(), C.c_int64()
+        lib.sfr_matrix_info(self.h, C.byref(n), C.byref(f), C.byref(s),
+                            C.byref(nnz))
+        return {"neq": n.value, "format": f.value, "slots": s.value,
+                "nnz": nnz.value}
+
+    def destroy(self):
+        if self.h:
+            lib.sfr_matrix_destroy(self.ctx.h, self.h)
+            self.h = None
+
+
+class Context:
+    def __init__(self, device=0):
+        h = C.c_void_p()
+        rc = lib.sfr_init(C.byref(h), device)
+        if rc != L.OK:
+            raise SolverError(rc, "sfr_init")
+        self.h = h.value
+
+    def _check(self, rc, allow=()):
+        if rc != L.OK and rc not in allow:
+            raise SolverError(rc, lib.sfr_last_error(self.h).decode())
+        return rc
+
+    def close(self):
+        if self.h:
+            lib.sfr_finalize(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def array(self, arr):
+        arr = np.ascontiguousarray(arr)
+        return DeviceArray(self, arr.nbytes, arr.dtype, arr.shape).set(arr)
+
+    def empty(self, shape, dtype=np.float64):
+        nbytes = int(np.prod(shape)) * np.dtype(dtype).itemsize
+        return DeviceArray(self, nbytes, np.dtype(dtype), shape)
+
+    def matrix(self, system, fmt=L.FMT_AUTO):
+        return DeviceMatrix(self, system, fmt)
+
+    def solve(self, system, solver, tol, maxit, b, x=None, newX=True):
+        """Host-array solve (sfr_solve).  Returns (x, stats dict)."""
+        b = np.ascontiguousarray(b, np.float64)
+        x = np.zeros(system.neq) if x is None else np.array(x, np.float64)
+        st = SfrStats()
+        s = system.c()
+        rc = lib.sfr_solve(self.h, C.byref(s), solver, tol, maxit, ptr(b),
+                           ptr(x), 1 if newX else 0, C.byref(st))
+        self._check(rc, allow=(L.MAXIT, L.BREAKDOWN))
+        return x, st.as_dict()
+
+    def solve_device(self, A, solver, tol, maxit, d_b, d_x, newX=True,
+                     precision=L.PREC_FP64):
+        st = SfrStats()
+        rc = lib.sfr_solve_device(self.h, A.h, solver, precision, tol, maxit,
+                                  d_b.ptr, d_x.ptr, 1 if newX else 0,
+                                  C.byref(st))
+        self._check(rc, allow=(L.MAXIT, L.BREAKDOWN))
+        return st.as_dict()
