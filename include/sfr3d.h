@@ -110,6 +110,9 @@ int  sfr_matrix_create(sfr_context *ctx, const sfr_system *sys, int format,
 int  sfr_matrix_set_values(sfr_context *ctx, sfr_matrix *A, const double *al,
                            const double *ad, const double *au);
 void sfr_matrix_destroy(sfr_context *ctx, sfr_matrix *A);
+/* launch shape of the solve kernels: rows per thread (1, 2, 4, 8) and, for
+ * the CSR format, lanes per row (1, 2, 4, 8) */
+int  sfr_matrix_tune(sfr_context *ctx, sfr_matrix *A, int rpt, int lanes);
 /* neq, device format, ELL slots, stored nnz of the device format */
 int  sfr_matrix_info(const sfr_matrix *A, int32_t *neq, int *format,
                      int *slots, int64_t *nnz);
@@ -134,8 +137,9 @@ int sfr_solve_device(sfr_context *ctx, sfr_matrix *A, int solver,
  * which = 1: FP32 inner SpMV + p.q. */
 int sfr_kernel_time(sfr_context *ctx, sfr_matrix *A, int which, int reps,
                     double *ms_per_launch);
-/* Tuning knobs of a context: iterations per captured graph chunk (>= 1) and
- * the relative residual reduction of each MIXED inner solve. */
+/* Tuning knobs of a context: "chunk" = iterations per captured graph chunk
+ * (1..256); "ru_delta" = MIXED: drop of the FP32 residual (relative to its
+ * largest value since the last fold) that triggers an FP64 reliable update. */
 int sfr_set_option(sfr_context *ctx, const char *name, double value);
 
 /* Drop-in twin of solverC (reference include/Solv.h:59-70), same prototype

@@ -83,6 +83,7 @@ _sig("sfr_matrix_create", C.c_int, vp, C.POINTER(SfrSystem), C.c_int,
      C.POINTER(vp))
 _sig("sfr_matrix_set_values", C.c_int, vp, vp, vp, vp, vp)
 _sig("sfr_matrix_destroy", None, vp, vp)
+_sig("sfr_matrix_tune", C.c_int, vp, vp, C.c_int, C.c_int)
 _sig("sfr_matrix_info", C.c_int, vp, C.POINTER(C.c_int32),
      C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int64))
 _sig("sfr_solve", C.c_int, vp, C.POINTER(SfrSystem), C.c_int, C.c_double,

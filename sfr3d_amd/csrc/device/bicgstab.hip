@@ -6,14 +6,23 @@
 //          t = Az, w = (t.s)/(t.t)
 //          x += w z, r = s - w t                      stop if ||r|| < conv
 //          beta = (r.r0/rr0)(alpha/w), p = r + beta(p - w v), z = D^-1 p
-// Five kernels per iteration, two of them with the SpMV.  r.r0 of the beta
-// step is the next iteration's rr0 (the reference computes it twice).
+// Five kernels per iteration, two with the SpMV:
+//   B1 k_bicg_v  v = Az, partials v.r0
+//   B2 k_bicg_s  alpha; x += alpha z, s = r - alpha v, z = D^-1 s; s.s
+//   B3 k_bicg_t  half-step stop test; t = Az, partials t.s, t.t
+//   B4 k_bicg_r  w; x += w z, r = s - w t; r.r, r.r0
+//   B5 k_bicg_p  full-step stop test; beta; p = r + beta(p - w v), z = D^-1 p
+// r.r0 of the beta step is the next iteration's rr0 (the reference computes
+// it twice).  Reductions and flag hand-over as in pcg.hip: B1 alone reads
+// `done`, each later kernel reads the run flag its predecessor set.
 #include "dev_common.hpp"
 
+#define REGION(part, k) ((part) + (size_t)(k) * 4 * SFR_MAXG)
+
 template <int RPT>
-__global__ __launch_bounds__(SFR_BLOCK) void k_bicg_setup1(int64_t n,
+__global__ __launch_bounds__(SFR_BLOCK) void k_bicg_s1(int64_t n,
     const double *__restrict__ b, double *__restrict__ x, int newX,
-    double tol, unsigned maxit, sfr_scalars *sc, double *part)
+    double *pout)
 {
   constexpr int RPC = RPT * SFR_BLOCK;
   double acc[1] = {0.0};
@@ -29,25 +38,30 @@ __global__ __launch_bounds__(SFR_BLOCK) void k_bicg_setup1(int64_t n,
       }
     }
   }
-  if (grid_sum<1>(acc, part, &sc->counter) && threadIdx.x == 0) {
-    sc->bb = acc[0];
-    sc->norm_b = sqrt(acc[0]);
-    sc->conv = tol * sc->norm_b;
-    sc->tol = tol;
-    sc->maxit = maxit;
-  }
+  store_partials<1>(acc, pout);
 }
 
 template <class Op, int RPT>
-__global__ __launch_bounds__(SFR_BLOCK) void k_bicg_setup2(Op A,
+__global__ __launch_bounds__(SFR_BLOCK) void k_bicg_s2(Op A,
     const double *__restrict__ b, const double *__restrict__ dinv,
     double *__restrict__ x, double *__restrict__ r, double *__restrict__ r0,
-    double *__restrict__ p, double *__restrict__ z, int newX,
-    sfr_scalars *sc, double *part)
+    double *__restrict__ p, double *__restrict__ z, int newX, double tol,
+    unsigned maxit, const double *pin, int gin, double *pout,
+    sfr_scalars *sc)
 {
+  double bb[1];
+  reduce_partials<1>(pin, gin, bb);
+  const double norm_b = sqrt(bb[0]);
+  const bool zero_b = norm_b == 0.0;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    sc->bb = bb[0];
+    sc->norm_b = norm_b;
+    sc->conv = tol * norm_b;
+    sc->tol = tol;
+    sc->maxit = maxit;
+  }
   constexpr int L = Op::L, RPC = RPT * (SFR_BLOCK / L);
   const bool lead = (threadIdx.x % L) == 0;
-  const bool zero_b = sc->norm_b == 0.0;
   double acc[1] = {0.0};
   for (int64_t c = xcd_chunk(blockIdx.x, gridDim.x); c * RPC < A.n;
        c += gridDim.x) {
@@ -67,21 +81,35 @@ __global__ __launch_bounds__(SFR_BLOCK) void k_bicg_setup2(Op A,
       }
     }
   }
-  if (grid_sum<1>(acc, part, &sc->counter) && threadIdx.x == 0) {
-    sc->d = acc[0];
+  store_partials<1>(acc, pout);
+}
+
+__global__ __launch_bounds__(SFR_BLOCK) void k_bicg_start(const double *part,
+    int g, sfr_scalars *sc)
+{
+  double t[1];
+  reduce_partials<1>(part, g, t);
+  if (threadIdx.x == 0) {
+    sc->d = t[0];
     sc->iter = 0;
-    sc->seal = 0;
-    sc->done = zero_b ? SFR_DONE_CONV : SFR_DONE_RUN;
+    sc->outer = 0;
+    sc->ru = 0;
+    sc->done = (sc->norm_b == 0.0 || t[0] == 0.0) ? SFR_DONE_CONV : SFR_DONE_RUN;
   }
 }
 
-// v = Az, alpha = rr0 / (v.r0)
+// B1: v = Az, partials v.r0
 template <class Op, int RPT>
 __global__ __launch_bounds__(SFR_BLOCK) void k_bicg_v(Op A,
     const double *__restrict__ z, const double *__restrict__ r0,
-    double *__restrict__ v, sfr_scalars *sc, double *part)
+    double *__restrict__ v, double *pout, sfr_scalars *sc)
 {
-  if (sc->done) return;
+  const bool first = blockIdx.x == 0 && threadIdx.x == 0;
+  if (sc->done) {
+    if (first) sc->run2 = 0;
+    return;
+  }
+  if (first) sc->run2 = 1;
   constexpr int L = Op::L, RPC = RPT * (SFR_BLOCK / L);
   const bool lead = (threadIdx.x % L) == 0;
   double acc[1] = {0.0};
@@ -98,23 +126,30 @@ __global__ __launch_bounds__(SFR_BLOCK) void k_bicg_v(Op A,
       }
     }
   }
-  if (grid_sum<1>(acc, part, &sc->counter) && threadIdx.x == 0) {
-    const double alpha = sc->d / acc[0];
-    sc->alpha = alpha;
-    if (!(acc[0] != 0.0) || !isfinite(alpha)) sc->done = SFR_DONE_BREAK;
-  }
+  store_partials<1>(acc, pout);
 }
 
-// x += alpha z, s = r - alpha v, z = D^-1 s, stop if ||s|| < conv
+// B2: alpha = rr0/(v.r0); x += alpha z, s = r - alpha v, z = D^-1 s; s.s
 template <int RPT>
 __global__ __launch_bounds__(SFR_BLOCK) void k_bicg_s(int64_t n,
     const double *__restrict__ v, const double *__restrict__ dinv,
     double *__restrict__ x, double *__restrict__ r, double *__restrict__ z,
-    sfr_scalars *sc, double *part)
+    const double *pin, int gin, double *pout, sfr_scalars *sc)
 {
-  if (sc->done) return;
+  const bool first = blockIdx.x == 0 && threadIdx.x == 0;
+  if (!sc->run2) {
+    if (first) sc->run3 = 0;
+    return;
+  }
+  double vr0[1];
+  reduce_partials<1>(pin, gin, vr0);
+  const double alpha = sc->d / vr0[0];
+  if (!(vr0[0] != 0.0) || !isfinite(alpha)) {
+    if (first) { sc->done = SFR_DONE_BREAK; sc->run3 = 0; }
+    return;
+  }
+  if (first) { sc->alpha = alpha; sc->run3 = 1; }
   constexpr int RPC = RPT * SFR_BLOCK;
-  const double alpha = sc->alpha;
   double acc[1] = {0.0};
   for (int64_t c = xcd_chunk(blockIdx.x, gridDim.x); c * RPC < n;
        c += gridDim.x) {
@@ -130,23 +165,32 @@ __global__ __launch_bounds__(SFR_BLOCK) void k_bicg_s(int64_t n,
       }
     }
   }
-  if (grid_sum<1>(acc, part, &sc->counter) && threadIdx.x == 0) {
-    if (sqrt(acc[0]) < sc->conv) {
-      sc->iter = sc->iter + 1;
-      sc->done = SFR_DONE_CONV;
-    } else if (!isfinite(acc[0])) {
-      sc->done = SFR_DONE_BREAK;
-    }
-  }
+  store_partials<1>(acc, pout);
 }
 
-// t = Az, w = (t.s)/(t.t)
+// B3: stop if ||s|| < conv; t = Az, partials t.s, t.t
 template <class Op, int RPT>
 __global__ __launch_bounds__(SFR_BLOCK) void k_bicg_t(Op A,
     const double *__restrict__ z, const double *__restrict__ r,
-    double *__restrict__ t, sfr_scalars *sc, double *part)
+    double *__restrict__ t, const double *pin, int gin, double *pout,
+    sfr_scalars *sc)
 {
-  if (sc->done) return;
+  const bool first = blockIdx.x == 0 && threadIdx.x == 0;
+  if (!sc->run3) {
+    if (first) sc->run4 = 0;
+    return;
+  }
+  double ss[1];
+  reduce_partials<1>(pin, gin, ss);
+  if (sqrt(ss[0]) < sc->conv || ss[0] == 0.0) {
+    if (first) { sc->iter = sc->iter + 1; sc->done = SFR_DONE_CONV; sc->run4 = 0; }
+    return;
+  }
+  if (!isfinite(ss[0])) {
+    if (first) { sc->done = SFR_DONE_BREAK; sc->run4 = 0; }
+    return;
+  }
+  if (first) sc->run4 = 1;
   constexpr int L = Op::L, RPC = RPT * (SFR_BLOCK / L);
   const bool lead = (threadIdx.x % L) == 0;
   double acc[2] = {0.0, 0.0};
@@ -164,24 +208,31 @@ __global__ __launch_bounds__(SFR_BLOCK) void k_bicg_t(Op A,
       }
     }
   }
-  if (grid_sum<2>(acc, part, &sc->counter) && threadIdx.x == 0) {
-    const double w = acc[0] / acc[1];
-    sc->omega = w;
-    if (!(acc[1] != 0.0) || !(w != 0.0) || !isfinite(w))
-      sc->done = SFR_DONE_BREAK;
-  }
+  store_partials<2>(acc, pout);
 }
 
-// x += w z, r = s - w t, stop if ||r|| < conv, beta = (r.r0/rr0)(alpha/w)
+// B4: w = t.s / t.t; x += w z, r = s - w t; partials r.r, r.r0
 template <int RPT>
 __global__ __launch_bounds__(SFR_BLOCK) void k_bicg_r(int64_t n,
     const double *__restrict__ z, const double *__restrict__ t,
     const double *__restrict__ r0, double *__restrict__ x,
-    double *__restrict__ r, sfr_scalars *sc, double *part)
+    double *__restrict__ r, const double *pin, int gin, double *pout,
+    sfr_scalars *sc)
 {
-  if (sc->done) return;
+  const bool first = blockIdx.x == 0 && threadIdx.x == 0;
+  if (!sc->run4) {
+    if (first) sc->run5 = 0;
+    return;
+  }
+  double tt[2];
+  reduce_partials<2>(pin, gin, tt);
+  const double w = tt[0] / tt[1];
+  if (!(tt[1] != 0.0) || !(w != 0.0) || !isfinite(w)) {
+    if (first) { sc->done = SFR_DONE_BREAK; sc->run5 = 0; }
+    return;
+  }
+  if (first) { sc->omega = w; sc->d_prev = sc->d; sc->run5 = 1; }
   constexpr int RPC = RPT * SFR_BLOCK;
-  const double w = sc->omega;
   double acc[2] = {0.0, 0.0};
   for (int64_t c = xcd_chunk(blockIdx.x, gridDim.x); c * RPC < n;
        c += gridDim.x) {
@@ -197,31 +248,34 @@ __global__ __launch_bounds__(SFR_BLOCK) void k_bicg_r(int64_t n,
       }
     }
   }
-  if (grid_sum<2>(acc, part, &sc->counter) && threadIdx.x == 0) {
-    const unsigned it = sc->iter + 1;
-    sc->iter = it;
-    if (sqrt(acc[0]) < sc->conv) {
-      sc->done = SFR_DONE_CONV;
-    } else if (!isfinite(acc[0]) || !(sc->d != 0.0)) {
-      sc->done = SFR_DONE_BREAK;
-    } else {
-      sc->beta = (acc[1] / sc->d) * (sc->alpha / w);
-      sc->d = acc[1];
-      if (it >= sc->maxit) sc->done = SFR_DONE_MAXIT;
-    }
-  }
+  store_partials<2>(acc, pout);
 }
 
-// p = r + beta (p - w v), z = D^-1 p
+// B5: stop if ||r|| < conv; beta; p = r + beta (p - w v), z = D^-1 p
 template <int RPT>
 __global__ __launch_bounds__(SFR_BLOCK) void k_bicg_p(int64_t n,
     const double *__restrict__ r, const double *__restrict__ v,
     const double *__restrict__ dinv, double *__restrict__ p,
-    double *__restrict__ z, sfr_scalars *sc)
+    double *__restrict__ z, const double *pin, int gin, sfr_scalars *sc)
 {
-  if (sc->done) return;
+  if (!sc->run5) return;
+  const bool first = blockIdx.x == 0 && threadIdx.x == 0;
+  double rr[2];
+  reduce_partials<2>(pin, gin, rr);
+  const double rr0 = sc->d_prev, w = sc->omega;
+  const double beta = (rr[1] / rr0) * (sc->alpha / w);
+  const bool conv = sqrt(rr[0]) < sc->conv || rr[0] == 0.0;
+  const bool brk = !isfinite(rr[0]) || !(rr0 != 0.0) || !isfinite(beta);
+  if (first) {
+    const unsigned it = sc->iter + 1;
+    sc->iter = it;
+    sc->d = rr[1];
+    if (conv) sc->done = SFR_DONE_CONV;
+    else if (brk) sc->done = SFR_DONE_BREAK;
+    else if (it >= sc->maxit) sc->done = SFR_DONE_MAXIT;
+  }
+  if (conv || brk) return;
   constexpr int RPC = RPT * SFR_BLOCK;
-  const double beta = sc->beta, w = sc->omega;
   for (int64_t c = xcd_chunk(blockIdx.x, gridDim.x); c * RPC < n;
        c += gridDim.x) {
 #pragma unroll
@@ -236,18 +290,6 @@ __global__ __launch_bounds__(SFR_BLOCK) void k_bicg_p(int64_t n,
   }
 }
 
-#define LAUNCH_V(K, g, s, ...)                                                 \
-  do {                                                                         \
-    switch ((g).rpt) {                                                         \
-    case 1: hipLaunchKernelGGL(K<1>, dim3((g).grid), dim3(SFR_BLOCK), 0, s,    \
-                               __VA_ARGS__); break;                            \
-    case 2: hipLaunchKernelGGL(K<2>, dim3((g).grid), dim3(SFR_BLOCK), 0, s,    \
-                               __VA_ARGS__); break;                            \
-    default: hipLaunchKernelGGL(K<4>, dim3((g).grid), dim3(SFR_BLOCK), 0, s,   \
-                                __VA_ARGS__); break;                           \
-    }                                                                          \
-  } while (0)
-
 extern "C" int sfr_k_bicg_setup(hipStream_t s, const sfr_mat_view *A,
                                 sfr_geom g0, const double *b, double *x,
                                 double *r, double *r0, double *p, double *z,
@@ -256,12 +298,15 @@ extern "C" int sfr_k_bicg_setup(hipStream_t s, const sfr_mat_view *A,
 {
   const sfr_geom gv = {g0.rpt, grid_for(A->n, g0.rpt, 1)};
   const sfr_geom gm = {g0.rpt, grid_for(A->n, g0.rpt, op_lanes(A))};
-  LAUNCH_V(k_bicg_setup1, gv, s, A->n, b, x, newX, tol, maxit, sc, part);
+  double *rc = REGION(part, 2), *rb = REGION(part, 1);
+  SFR_LAUNCH_V(k_bicg_s1, gv, s, A->n, b, x, newX, rc);
   with_op64(A, [&](auto op) {
     using Op = decltype(op);
-    SFR_LAUNCH_RPT(k_bicg_setup2, Op, gm, s, op, b, A->dinv, x, r, r0, p, z,
-                   newX, sc, part);
+    SFR_LAUNCH_RPT(k_bicg_s2, Op, gm, s, op, b, A->dinv, x, r, r0, p, z,
+                   newX, tol, maxit, (const double *)rc, gv.grid, rb, sc);
   });
+  hipLaunchKernelGGL(k_bicg_start, dim3(1), dim3(SFR_BLOCK), 0, s,
+                     (const double *)rb, gm.grid, sc);
   return hipGetLastError();
 }
 
@@ -273,16 +318,22 @@ extern "C" int sfr_k_bicg_iter(hipStream_t s, const sfr_mat_view *A,
 {
   const sfr_geom gv = {g0.rpt, grid_for(A->n, g0.rpt, 1)};
   const sfr_geom gm = {g0.rpt, grid_for(A->n, g0.rpt, op_lanes(A))};
+  double *ra = REGION(part, 0), *rb = REGION(part, 1);
+  double *rc = REGION(part, 2), *rd = REGION(part, 3);
   with_op64(A, [&](auto op) {
     using Op = decltype(op);
-    SFR_LAUNCH_RPT(k_bicg_v, Op, gm, s, op, z, r0, v, sc, part);
+    SFR_LAUNCH_RPT(k_bicg_v, Op, gm, s, op, z, r0, v, ra, sc);
   });
-  LAUNCH_V(k_bicg_s, gv, s, A->n, v, A->dinv, x, r, z, sc, part);
+  SFR_LAUNCH_V(k_bicg_s, gv, s, A->n, v, A->dinv, x, r, z, (const double *)ra,
+               gm.grid, rb, sc);
   with_op64(A, [&](auto op) {
     using Op = decltype(op);
-    SFR_LAUNCH_RPT(k_bicg_t, Op, gm, s, op, z, r, t, sc, part);
+    SFR_LAUNCH_RPT(k_bicg_t, Op, gm, s, op, z, r, t, (const double *)rb,
+                   gv.grid, rc, sc);
   });
-  LAUNCH_V(k_bicg_r, gv, s, A->n, z, t, r0, x, r, sc, part);
-  LAUNCH_V(k_bicg_p, gv, s, A->n, r, v, A->dinv, p, z, sc);
+  SFR_LAUNCH_V(k_bicg_r, gv, s, A->n, z, t, r0, x, r, (const double *)rc,
+               gm.grid, rd, sc);
+  SFR_LAUNCH_V(k_bicg_p, gv, s, A->n, r, v, A->dinv, p, z, (const double *)rd,
+               gv.grid, sc);
   return hipGetLastError();
 }

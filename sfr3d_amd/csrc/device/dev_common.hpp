@@ -50,45 +50,46 @@ __device__ __forceinline__ void block_sum(double (&v)[NV])
   __syncthreads();
 }
 
-// Deterministic grid-wide sum of NV values.  Each workgroup stores its sums
-// in part[k*gridDim.x + blockIdx.x]; the workgroup that arrives last adds all
-// partials in workgroup order and returns true, totals in thread 0.  That
-// workgroup resets the arrival counter, so the kernel can be relaunched or
-// replayed from a graph with no memset in between.
+// Block partial sums: the workgroup's sum of each of the NV values goes to
+// part[k*gridDim.x + blockIdx.x] with a plain store.  No atomics and no
+// fences: the partials are read by a later kernel on the same stream, and
+// the kernel boundary makes them visible on every XCD.
 template <int NV>
-__device__ __forceinline__ bool grid_sum(double (&v)[NV], double *part,
-                                         unsigned *counter)
+__device__ __forceinline__ void store_partials(double (&v)[NV], double *part)
 {
-  __shared__ bool last;
   block_sum<NV>(v);
-  const int g = gridDim.x;
   if (threadIdx.x == 0) {
 #pragma unroll
-    for (int k = 0; k < NV; k++) part[(size_t)k * g + blockIdx.x] = v[k];
-    __threadfence();
-    const unsigned prev = atomicAdd(counter, 1u);
-    last = (prev == (unsigned)g - 1);
+    for (int k = 0; k < NV; k++) part[(size_t)k * gridDim.x + blockIdx.x] = v[k];
   }
-  __syncthreads();
-  if (!last) return false;
-  __threadfence();
+}
+
+// Totals of the g partials a previous kernel stored (slot-major), summed in
+// a fixed order by the whole workgroup and handed to every thread.  Every
+// workgroup that calls it gets bitwise the same totals, so scalars derived
+// from them (alpha, beta, stop tests) agree across the grid without any
+// grid-wide synchronisation.
+template <int NV>
+__device__ __forceinline__ void reduce_partials(const double *__restrict__ part,
+                                                int g, double (&out)[NV])
+{
+  __shared__ double bc[NV];
   double t[NV];
 #pragma unroll
   for (int k = 0; k < NV; k++) {
     double s = 0.0;
-    for (int j = threadIdx.x; j < g; j += SFR_BLOCK)
-      s += __hip_atomic_load(&part[(size_t)k * g + j], __ATOMIC_RELAXED,
-                             __HIP_MEMORY_SCOPE_AGENT);
+    for (int j = threadIdx.x; j < g; j += SFR_BLOCK) s += part[(size_t)k * g + j];
     t[k] = s;
   }
   block_sum<NV>(t);
   if (threadIdx.x == 0) {
 #pragma unroll
-    for (int k = 0; k < NV; k++) v[k] = t[k];
-    __hip_atomic_store(counter, 0u, __ATOMIC_RELAXED,
-                       __HIP_MEMORY_SCOPE_AGENT);
+    for (int k = 0; k < NV; k++) bc[k] = t[k];
   }
-  return true;
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < NV; k++) out[k] = bc[k];
+  __syncthreads();
 }
 
 // ---- sparse row operators ---------------------------------------------------
@@ -146,14 +147,6 @@ struct CsrOp {
   }
 };
 
-// Row of pass r of this thread: chunks of rpt*(BLOCK/L) consecutive rows.
-template <int L>
-__device__ __forceinline__ int64_t row_base(int rpt)
-{
-  return (int64_t)xcd_chunk(blockIdx.x, gridDim.x) * rpt * (SFR_BLOCK / L) +
-         threadIdx.x / L;
-}
-
 // Call f(op) with the FP64 operator matching the view.
 template <typename F>
 __host__ inline void with_op64(const sfr_mat_view *A, F &&f)
@@ -195,14 +188,15 @@ __host__ inline void with_op32(const sfr_mat_view *A, F &&f)
 }
 
 // Workgroups for n rows at rpt rows per thread and `lanes` lanes per row,
-// a multiple of 8 (XCD remap) and at most SFR_MAXG (kernels grid-stride).
+// a multiple of 8 (XCD remap) and at most SFR_GRIDCAP (kernels grid-stride
+// beyond that), so a consumer never reduces more than SFR_GRIDCAP partials.
 static inline int grid_for(int64_t n, int rpt, int lanes)
 {
   const int64_t rows = (int64_t)rpt * (SFR_BLOCK / lanes);
   int64_t g = (n + rows - 1) / rows;
   g = ((g + 7) / 8) * 8;
   if (g < 8) g = 8;
-  if (g > SFR_MAXG) g = SFR_MAXG;
+  if (g > SFR_GRIDCAP) g = SFR_GRIDCAP;
   return (int)g;
 }
 
@@ -211,7 +205,7 @@ static inline int op_lanes(const sfr_mat_view *A)
   return A->fmt == SFR_FMT_ELL ? 1 : A->lanes;
 }
 
-// Launch kernel template K<Op, RPT> with the rows-per-thread of g.
+// Launch kernel template K<Op, RPT> / K<RPT> with the rows-per-thread of g.
 #define SFR_LAUNCH_RPT(K, Op, g, s, ...)                                      \
   do {                                                                        \
     switch ((g).rpt) {                                                        \
@@ -219,7 +213,23 @@ static inline int op_lanes(const sfr_mat_view *A)
                                0, s, __VA_ARGS__); break;                     \
     case 2: hipLaunchKernelGGL((K<Op, 2>), dim3((g).grid), dim3(SFR_BLOCK),   \
                                0, s, __VA_ARGS__); break;                     \
-    default: hipLaunchKernelGGL((K<Op, 4>), dim3((g).grid), dim3(SFR_BLOCK),  \
+    case 4: hipLaunchKernelGGL((K<Op, 4>), dim3((g).grid), dim3(SFR_BLOCK),   \
+                               0, s, __VA_ARGS__); break;                     \
+    default: hipLaunchKernelGGL((K<Op, 8>), dim3((g).grid), dim3(SFR_BLOCK),  \
                                 0, s, __VA_ARGS__); break;                    \
+    }                                                                         \
+  } while (0)
+
+#define SFR_LAUNCH_V(K, g, s, ...)                                            \
+  do {                                                                        \
+    switch ((g).rpt) {                                                        \
+    case 1: hipLaunchKernelGGL(K<1>, dim3((g).grid), dim3(SFR_BLOCK), 0, s,   \
+                               __VA_ARGS__); break;                           \
+    case 2: hipLaunchKernelGGL(K<2>, dim3((g).grid), dim3(SFR_BLOCK), 0, s,   \
+                               __VA_ARGS__); break;                           \
+    case 4: hipLaunchKernelGGL(K<4>, dim3((g).grid), dim3(SFR_BLOCK), 0, s,   \
+                               __VA_ARGS__); break;                           \
+    default: hipLaunchKernelGGL(K<8>, dim3((g).grid), dim3(SFR_BLOCK), 0, s,  \
+                                __VA_ARGS__); break;                          \
     }                                                                         \
   } while (0)

@@ -1,32 +1,89 @@
-// Mixed-precision iterative refinement for the Jacobi-PCG path.
+// Mixed-precision Jacobi-PCG: FP32 CG iterations with FP64 reliable updates.
 //
-// Outer loop, FP64: x += D^-1/2 y, r = b - A x, stop when
-// sqrt|r.D^-1 r| < tol * sqrt|b.D^-1 b| -- the reference PCG's own test
-// (src/Cg.c:73-78,141-142), evaluated on the true FP64 residual.
-// Inner loop, FP32 storage: CG on the Jacobi-scaled system
-//   (D^-1/2 A D^-1/2) y = D^-1/2 r,
-// which is Jacobi-PCG on A written with a unit diagonal: no diagonal and no
-// preconditioner vector is read, and the correction is D^-1/2 y.  Matrix and
-// vectors are FP32; dot products are accumulated in FP64.  The inner solve
-// stops when its residual has dropped by inner_tol (r.r <= inner_tol^2 r0.r0)
-// or after inner_maxit iterations.
-// Per inner iteration (three kernels):
-//   k_mx_spmv   q = Ap, p.q -> alpha       matrix + 2 words
-//   k_mx_resid  r -= alpha q, r.r -> beta   3 words
-//   k_mx_dir    y += alpha p, p = r + beta p  5 words
+// The Krylov iteration runs on the Jacobi-scaled system
+//     (D^-1/2 A D^-1/2) y = D^-1/2 r,
+// i.e. Jacobi-PCG on A written with a unit diagonal, with the off-diagonals
+// and the vectors y, r, p, q stored in FP32 (dot products accumulated in
+// FP64).  The correction y is folded into the FP64 solution x every time the
+// FP32 residual has dropped by a factor delta below its largest value since
+// the previous fold (reliable updates: G. Sleijpen, H. van der Vorst,
+// "Reliable updated residuals in hybrid Bi-CG methods", Computing 56 (1996);
+// the CG form used by M. Clark et al., "Solving lattice QCD systems of
+// equations using mixed precision solvers on GPUs", CPC 181 (2010)):
+//   x += D^-1/2 y, y = 0               (FP64)
+//   r64 = b - A x                      (FP64 matrix, FP64 arithmetic)
+//   stop if sqrt|r64.D^-1 r64| < tol * sqrt|b.D^-1 b|   -- the reference
+//        PCG's own test (src/Cg.c:73-78,141-142) on the true FP64 residual
+//   r = D^-1/2 r64 (to FP32); p = r + beta (p - (r.p / r.r) r),
+//   beta = r.r / (r.r before the fold)
+// Unlike restarted refinement, the search direction survives the fold, so
+// the FP32 loop keeps CG's convergence rate.
+// Per FP32 iteration (three kernels):
+//   M1 k_mx_spmv   q = A^ p; partials p.q             matrix + 2 words
+//   M2 k_mx_resid  alpha; r -= alpha q; partials r.r   3 words
+//   M3 k_mx_dir    beta; y += alpha p; p = r + beta p  5 words
+// A fold is requested by M3 (flag `ru`); the host sees it at its next poll
+// and enqueues the fold kernels R1-R3.
 #include "dev_common.hpp"
 
+#define REGION(part, k) ((part) + (size_t)(k) * 4 * SFR_MAXG)
+
+// one workgroup: norms of b (partials of k_pcg_s1) -> tolerance, loop state
+__global__ __launch_bounds__(SFR_BLOCK) void k_mx_start(const double *pin,
+    int gin, double tol, unsigned maxit, double delta, sfr_scalars *sc)
+{
+  double nb[2];
+  reduce_partials<2>(pin, gin, nb);
+  if (threadIdx.x == 0) {
+    const double norm_b = sqrt(fabs(nb[0]));
+    sc->norm_b = norm_b;
+    sc->conv = tol * norm_b;
+    sc->bb = nb[1];
+    sc->tol = tol;
+    sc->maxit = maxit;
+    sc->delta2 = delta * delta;
+    sc->iter = 0;
+    sc->outer = 0;
+    sc->ru = 0;
+    sc->done = norm_b == 0.0 ? SFR_DONE_CONV : SFR_DONE_RUN;
+  }
+}
+
+// R1: x += D^-1/2 y, y = 0
+template <int RPT>
+__global__ __launch_bounds__(SFR_BLOCK) void k_mx_fold(int64_t n,
+    float *__restrict__ y, const double *__restrict__ dscale,
+    double *__restrict__ x)
+{
+  constexpr int RPC = RPT * SFR_BLOCK;
+  for (int64_t c = xcd_chunk(blockIdx.x, gridDim.x); c * RPC < n;
+       c += gridDim.x) {
+#pragma unroll
+    for (int r_ = 0; r_ < RPT; r_++) {
+      const int64_t i = c * RPC + r_ * SFR_BLOCK + threadIdx.x;
+      if (i < n) {
+        x[i] += dscale[i] * (double)y[i];
+        y[i] = 0.0f;
+      }
+    }
+  }
+}
+
+// R2: r64 = b - A x in FP64; r = D^-1/2 r64 in FP32 (and p = r, y = 0 on the
+// first call); partials of r64.D^-1 r64, r64.r64, r.r, r.p
 template <class Op, int RPT, bool ZERO_X>
-__global__ __launch_bounds__(SFR_BLOCK) void k_mx_resid64(Op A,
+__global__ __launch_bounds__(SFR_BLOCK) void k_mx_true(Op A,
     const double *__restrict__ b, const double *__restrict__ x,
     const double *__restrict__ dinv, const double *__restrict__ dscale,
-    float *__restrict__ rh, float *__restrict__ ph, float *__restrict__ y,
-    int first, double inner_tol, unsigned inner_maxit, sfr_scalars *sc,
-    double *part)
+    float *__restrict__ r, float *__restrict__ p, float *__restrict__ y,
+    int init, double *pout, sfr_scalars *sc)
 {
+  if (blockIdx.x == 0 && threadIdx.x == 0)
+    sc->run2 = sc->done ? 0u : ((init || sc->ru == 2) ? 2u : 1u);
+  if (sc->done) return;
   constexpr int L = Op::L, RPC = RPT * (SFR_BLOCK / L);
   const bool lead = (threadIdx.x % L) == 0;
-  double acc[3] = {0.0, 0.0, 0.0};
+  double acc[4] = {0.0, 0.0, 0.0, 0.0};
   for (int64_t c = xcd_chunk(blockIdx.x, gridDim.x); c * RPC < A.n;
        c += gridDim.x) {
 #pragma unroll
@@ -37,67 +94,93 @@ __global__ __launch_bounds__(SFR_BLOCK) void k_mx_resid64(Op A,
       if (valid && lead) {
         const double ri = b[i] - ax;
         const float ci = (float)(ri * dscale[i]);
-        rh[i] = ci;
-        ph[i] = ci;
-        y[i] = 0.0f;
+        r[i] = ci;
         acc[0] += ri * (ri * dinv[i]);
         acc[1] += ri * ri;
         acc[2] += (double)ci * (double)ci;
+        if (init) {
+          p[i] = ci;
+          y[i] = 0.0f;
+        } else {
+          acc[3] += (double)ci * (double)p[i];
+        }
       }
     }
   }
-  if (grid_sum<3>(acc, part, &sc->counter) && threadIdx.x == 0) {
-    sc->res[2] = acc[0];
-    sc->res[3] = acc[1];
-    if (first) {
-      sc->outer = 0;
-      sc->iter = 0;
-    } else {
-      sc->outer = sc->outer + 1;
-    }
-    if (sqrt(fabs(acc[0])) < sc->conv) {
-      sc->done = SFR_DONE_CONV;
-    } else if (!isfinite(acc[0])) {
-      sc->done = SFR_DONE_BREAK;
-    } else {
-      sc->done = sc->iter >= sc->maxit ? SFR_DONE_MAXIT : SFR_DONE_RUN;
-    }
-    sc->d = acc[2];
-    sc->rho0 = acc[2];
-    sc->inner_tol2 = inner_tol * inner_tol;
-    sc->inner_iter = 0;
-    sc->inner_maxit = inner_maxit;
-    sc->inner_done = sc->done != SFR_DONE_RUN ? 1u : (acc[2] == 0.0 ? 1u : 0u);
-    sc->seal = 0;
-  }
+  store_partials<4>(acc, pout);
 }
 
-// x += D^-1/2 y
+// restart after an FP32 breakdown: p = r (R2 writes p only on the first call)
 template <int RPT>
-__global__ __launch_bounds__(SFR_BLOCK) void k_mx_update(int64_t n,
-    const float *__restrict__ y, const double *__restrict__ dscale,
-    double *__restrict__ x)
+__global__ __launch_bounds__(SFR_BLOCK) void k_mx_copy(int64_t n,
+    const float *__restrict__ r, float *__restrict__ p, int init,
+    sfr_scalars *sc)
 {
+  if (init || sc->run2 != 2) return;
   constexpr int RPC = RPT * SFR_BLOCK;
   for (int64_t c = xcd_chunk(blockIdx.x, gridDim.x); c * RPC < n;
        c += gridDim.x) {
 #pragma unroll
     for (int r_ = 0; r_ < RPT; r_++) {
       const int64_t i = c * RPC + r_ * SFR_BLOCK + threadIdx.x;
-      if (i < n) x[i] += dscale[i] * (double)y[i];
+      if (i < n) p[i] = r[i];
     }
   }
 }
 
+// R3: FP64 stop test; p = r + beta (p - (r.p/r.r) r) after a fold
+template <int RPT>
+__global__ __launch_bounds__(SFR_BLOCK) void k_mx_restart(int64_t n,
+    const float *__restrict__ r, float *__restrict__ p, const double *pin,
+    int gin, int init, sfr_scalars *sc)
+{
+  const unsigned mode = sc->run2;          // 0 skip, 1 fold, 2 restart (p = r)
+  if (!mode) return;
+  double t[4];
+  reduce_partials<4>(pin, gin, t);
+  const double rmr = t[0], rr = t[2];
+  const bool conv = sqrt(fabs(rmr)) < sc->conv || rmr == 0.0;
+  const bool brk = !isfinite(rmr) || !isfinite(rr);
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    sc->res[2] = rmr;
+    sc->res[3] = t[1];
+    if (!init) sc->outer = sc->outer + 1;
+    if (conv) sc->done = SFR_DONE_CONV;
+    else if (brk) sc->done = SFR_DONE_BREAK;
+    else if (sc->iter >= sc->maxit) sc->done = SFR_DONE_MAXIT;
+    sc->d = rr;
+    sc->rho_max = rr;
+    sc->ru = 0;
+  }
+  if (conv || brk || mode == 2) return;
+  const double beta = rr / sc->d_prev;
+  const float fb = (float)beta, fg = (float)(t[3] / rr);
+  constexpr int RPC = RPT * SFR_BLOCK;
+  for (int64_t c = xcd_chunk(blockIdx.x, gridDim.x); c * RPC < n;
+       c += gridDim.x) {
+#pragma unroll
+    for (int r_ = 0; r_ < RPT; r_++) {
+      const int64_t i = c * RPC + r_ * SFR_BLOCK + threadIdx.x;
+      if (i < n) {
+        const float ri = r[i];
+        p[i] = ri + fb * (p[i] - fg * ri);
+      }
+    }
+  }
+}
+
+// M1: q = A^ p (unit diagonal), partials p.q
 template <class Op, int RPT>
 __global__ __launch_bounds__(SFR_BLOCK) void k_mx_spmv(Op A,
-    const float *__restrict__ p, float *__restrict__ q, sfr_scalars *sc,
-    double *part)
+    const float *__restrict__ p, float *__restrict__ q, double *pout,
+    sfr_scalars *sc)
 {
-  if (sc->inner_done) {
-    if (blockIdx.x == 0 && threadIdx.x == 0) sc->seal = 1;
+  const bool first = blockIdx.x == 0 && threadIdx.x == 0;
+  if (sc->done || sc->ru) {
+    if (first) sc->run4 = 0;
     return;
   }
+  if (first) sc->run4 = 1;
   constexpr int L = Op::L, RPC = RPT * (SFR_BLOCK / L);
   const bool lead = (threadIdx.x % L) == 0;
   double acc[1] = {0.0};
@@ -114,21 +197,37 @@ __global__ __launch_bounds__(SFR_BLOCK) void k_mx_spmv(Op A,
       }
     }
   }
-  if (grid_sum<1>(acc, part, &sc->counter) && threadIdx.x == 0) {
-    const double alpha = sc->d / acc[0];
-    sc->alpha = alpha;
-    if (!(acc[0] > 0.0) || !isfinite(alpha)) sc->inner_done = 2;
-  }
+  store_partials<1>(acc, pout);
 }
 
+// M2: alpha = r.r / p.q; r -= alpha q; partials r.r
 template <int RPT>
-__global__ __launch_bounds__(SFR_BLOCK) void k_mx_resid32(int64_t n,
-    const float *__restrict__ q, float *__restrict__ r, sfr_scalars *sc,
-    double *part)
+__global__ __launch_bounds__(SFR_BLOCK) void k_mx_resid(int64_t n,
+    const float *__restrict__ q, float *__restrict__ r, const double *pin,
+    int gin, double *pout, sfr_scalars *sc)
 {
-  if (sc->inner_done) return;
+  const bool first = blockIdx.x == 0 && threadIdx.x == 0;
+  if (!sc->run4) {
+    if (first) sc->run5 = 0;
+    return;
+  }
+  double pq[1];
+  reduce_partials<1>(pin, gin, pq);
+  const double d = sc->d;
+  const double alpha = d / pq[0];
+  if (!(pq[0] > 0.0) || !isfinite(alpha)) {
+    // FP32 lost positive definiteness: restart from the true residual
+    if (first) { sc->ru = 2; sc->run5 = 0; }
+    return;
+  }
+  if (first) {
+    sc->alpha = alpha;
+    sc->d_prev = d;
+    if (d > sc->rho_max) sc->rho_max = d;
+    sc->run5 = 1;
+  }
+  const float fa = (float)alpha;
   constexpr int RPC = RPT * SFR_BLOCK;
-  const float alpha = (float)sc->alpha;
   double acc[1] = {0.0};
   for (int64_t c = xcd_chunk(blockIdx.x, gridDim.x); c * RPC < n;
        c += gridDim.x) {
@@ -136,35 +235,36 @@ __global__ __launch_bounds__(SFR_BLOCK) void k_mx_resid32(int64_t n,
     for (int r_ = 0; r_ < RPT; r_++) {
       const int64_t i = c * RPC + r_ * SFR_BLOCK + threadIdx.x;
       if (i < n) {
-        const float ri = r[i] - alpha * q[i];
+        const float ri = r[i] - fa * q[i];
         r[i] = ri;
         acc[0] += (double)ri * (double)ri;
       }
     }
   }
-  if (grid_sum<1>(acc, part, &sc->counter) && threadIdx.x == 0) {
-    const double rr = acc[0];
-    sc->beta = rr / sc->d;
-    sc->d = rr;
-    const unsigned it = sc->inner_iter + 1;
-    sc->inner_iter = it;
-    sc->iter = sc->iter + 1;
-    if (rr <= sc->inner_tol2 * sc->rho0 || it >= sc->inner_maxit ||
-        sc->iter >= sc->maxit)
-      sc->inner_done = 1;
-    else if (!isfinite(rr))
-      sc->inner_done = 2;
-  }
+  store_partials<1>(acc, pout);
 }
 
+// M3: beta = r.r / d; y += alpha p; p = r + beta p unless a fold is due
 template <int RPT>
 __global__ __launch_bounds__(SFR_BLOCK) void k_mx_dir(int64_t n,
     const float *__restrict__ r, float *__restrict__ y,
-    float *__restrict__ p, sfr_scalars *sc)
+    float *__restrict__ p, const double *pin, int gin, sfr_scalars *sc)
 {
-  if (sc->seal || sc->inner_done == 2) return;
+  if (!sc->run5) return;
+  double t[1];
+  reduce_partials<1>(pin, gin, t);
+  const double rr = t[0];
+  const double conv = sc->conv;
+  const bool fold = rr < sc->delta2 * sc->rho_max || rr <= conv * conv ||
+                    !isfinite(rr);
+  const float fa = (float)sc->alpha, fb = (float)(rr / sc->d_prev);
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    const unsigned it = sc->iter + 1;
+    sc->iter = it;
+    sc->d = rr;
+    if (fold || it >= sc->maxit) sc->ru = 1;
+  }
   constexpr int RPC = RPT * SFR_BLOCK;
-  const float alpha = (float)sc->alpha, beta = (float)sc->beta;
   for (int64_t c = xcd_chunk(blockIdx.x, gridDim.x); c * RPC < n;
        c += gridDim.x) {
 #pragma unroll
@@ -172,73 +272,73 @@ __global__ __launch_bounds__(SFR_BLOCK) void k_mx_dir(int64_t n,
       const int64_t i = c * RPC + r_ * SFR_BLOCK + threadIdx.x;
       if (i < n) {
         const float pi = p[i];
-        y[i] += alpha * pi;
-        p[i] = r[i] + beta * pi;
+        y[i] += fa * pi;
+        if (!fold) p[i] = r[i] + fb * pi;
       }
     }
   }
 }
 
-#define LAUNCH_V(K, g, s, ...)                                                 \
-  do {                                                                         \
-    switch ((g).rpt) {                                                         \
-    case 1: hipLaunchKernelGGL(K<1>, dim3((g).grid), dim3(SFR_BLOCK), 0, s,    \
-                               __VA_ARGS__); break;                            \
-    case 2: hipLaunchKernelGGL(K<2>, dim3((g).grid), dim3(SFR_BLOCK), 0, s,    \
-                               __VA_ARGS__); break;                            \
-    default: hipLaunchKernelGGL(K<4>, dim3((g).grid), dim3(SFR_BLOCK), 0, s,   \
-                                __VA_ARGS__); break;                           \
-    }                                                                          \
-  } while (0)
+extern "C" int sfr_k_mx_start(hipStream_t s, const double *part, int nparts,
+                              double tol, unsigned maxit, double delta,
+                              sfr_scalars *sc)
+{
+  hipLaunchKernelGGL(k_mx_start, dim3(1), dim3(SFR_BLOCK), 0, s,
+                     (const double *)REGION(part, 2), nparts, tol, maxit,
+                     delta, sc);
+  return hipGetLastError();
+}
 
-extern "C" int sfr_k_mx_outer(hipStream_t s, const sfr_mat_view *A,
-                              sfr_geom g0, const double *b, double *x,
-                              const float *y, int first, int newX, float *rh,
-                              float *ph, float *yz, double inner_tol,
-                              unsigned inner_maxit, sfr_scalars *sc,
-                              double *part)
+// fold kernels R1-R3 (init: first call -- no fold, p = r, y = 0)
+extern "C" int sfr_k_mx_fold(hipStream_t s, const sfr_mat_view *A,
+                             sfr_geom g0, const double *b, double *x,
+                             float *y, float *r, float *p, int init,
+                             int zero_x, sfr_scalars *sc, double *part)
 {
   const sfr_geom gv = {g0.rpt, grid_for(A->n, g0.rpt, 1)};
   const sfr_geom gm = {g0.rpt, grid_for(A->n, g0.rpt, op_lanes(A))};
-  if (!first) LAUNCH_V(k_mx_update, gv, s, A->n, y, A->dscale, x);
-  const bool zero_x = first && newX;
+  double *rc = REGION(part, 2);
+  if (!init) SFR_LAUNCH_V(k_mx_fold, gv, s, A->n, y, A->dscale, x);
   with_op64(A, [&](auto op) {
     using Op = decltype(op);
     switch (gm.rpt) {
 #define CASE(R)                                                                \
     case R:                                                                    \
-      if (zero_x)                                                              \
-        hipLaunchKernelGGL((k_mx_resid64<Op, R, true>), dim3(gm.grid),         \
+      if (zero_x && init)                                                      \
+        hipLaunchKernelGGL((k_mx_true<Op, R, true>), dim3(gm.grid),            \
                            dim3(SFR_BLOCK), 0, s, op, b, x, A->dinv,           \
-                           A->dscale, rh, ph, yz, first, inner_tol, inner_maxit,\
-                           sc, part);                                          \
+                           A->dscale, r, p, y, init, rc, sc);                  \
       else                                                                     \
-        hipLaunchKernelGGL((k_mx_resid64<Op, R, false>), dim3(gm.grid),        \
+        hipLaunchKernelGGL((k_mx_true<Op, R, false>), dim3(gm.grid),           \
                            dim3(SFR_BLOCK), 0, s, op, b, x, A->dinv,           \
-                           A->dscale, rh, ph, yz, first, inner_tol, inner_maxit,\
-                           sc, part);                                          \
+                           A->dscale, r, p, y, init, rc, sc);                  \
       break;
-    CASE(1) CASE(2)
-    default: CASE(4)
+    CASE(1) CASE(2) CASE(4)
+    default: CASE(8)
 #undef CASE
     }
   });
+  SFR_LAUNCH_V(k_mx_copy, gv, s, A->n, (const float *)r, p, init, sc);
+  SFR_LAUNCH_V(k_mx_restart, gv, s, A->n, (const float *)r, p,
+               (const double *)rc, gm.grid, init, sc);
   return hipGetLastError();
 }
 
 extern "C" int sfr_k_mx_inner_iter(hipStream_t s, const sfr_mat_view *A,
-                                   sfr_geom g0, float *y, float *rh,
-                                   float *ph, float *qh, sfr_scalars *sc,
-                                   double *part)
+                                   sfr_geom g0, float *y, float *r, float *p,
+                                   float *q, sfr_scalars *sc, double *part)
 {
   const sfr_geom gv = {g0.rpt, grid_for(A->n, g0.rpt, 1)};
   const sfr_geom gm = {g0.rpt, grid_for(A->n, g0.rpt, op_lanes(A))};
+  double *ra = REGION(part, 0), *rb = REGION(part, 1);
   with_op32(A, [&](auto op) {
     using Op = decltype(op);
-    SFR_LAUNCH_RPT(k_mx_spmv, Op, gm, s, op, ph, qh, sc, part);
+    SFR_LAUNCH_RPT(k_mx_spmv, Op, gm, s, op, p, q, ra, sc);
   });
-  LAUNCH_V(k_mx_resid32, gv, s, A->n, qh, rh, sc, part);
-  LAUNCH_V(k_mx_dir, gv, s, A->n, rh, y, ph, sc);
+  SFR_LAUNCH_V(k_mx_resid, gv, s, A->n, q, r, (const double *)ra, gm.grid, rb,
+               sc);
+  SFR_LAUNCH_V(k_mx_dir, gv, s, A->n, r, y, p, (const double *)rb, gv.grid,
+               sc);
   return hipGetLastError();
 }
 
@@ -250,7 +350,7 @@ extern "C" int sfr_k_mx_spmv_only(hipStream_t s, const sfr_mat_view *A,
   const sfr_geom gm = {g0.rpt, grid_for(A->n, g0.rpt, op_lanes(A))};
   with_op32(A, [&](auto op) {
     using Op = decltype(op);
-    SFR_LAUNCH_RPT(k_mx_spmv, Op, gm, s, op, p, q, sc, part);
+    SFR_LAUNCH_RPT(k_mx_spmv, Op, gm, s, op, p, q, REGION(part, 0), sc);
   });
   return hipGetLastError();
 }

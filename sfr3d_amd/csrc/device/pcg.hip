@@ -5,17 +5,24 @@
 //   iter   q = Ap, alpha = d/(p.q), x += alpha p, r -= alpha q, z = D^-1 r,
 //          di = r.z, beta = di/d, p = z + beta p, d = di, stop when
 //          sqrt|d| < conv
-// One iteration is three kernels, each one streaming pass over its vectors:
-//   k_pcg_spmv   q = Ap, p.q            -> alpha            (matrix + 2 words)
-//   k_pcg_resid  r -= alpha q, r.D^-1 r -> beta, stop test  (4 words)
-//   k_pcg_dir    x += alpha p, p = D^-1 r + beta p           (6 words)
-// The x update rides with the direction update (same alpha, same old p, the
-// same rounding as the reference order) so z never has to be stored.  The
-// direction kernel of the iteration that converged still runs; the `seal`
-// flag, raised by the first spmv kernel that finds the loop finished, stops
-// it afterwards.  Reductions are deterministic (fixed order), so a solve is
-// bitwise reproducible run to run.
+// One iteration is three streaming kernels:
+//   K1 k_pcg_spmv   q = Ap; partials of p.q                 (matrix + 2 words)
+//   K2 k_pcg_resid  alpha = d/(p.q); r -= alpha q; partials of r.D^-1 r
+//                                                           (4 words)
+//   K3 k_pcg_dir    beta = r.D^-1 r / d; x += alpha p; p = D^-1 r + beta p;
+//                   stop test                               (6 words)
+// The x update rides with the direction update (same alpha, same old p,
+// same rounding as the reference's order), so z is never stored.
+//
+// Reductions: each workgroup stores its partial sum with a plain store and
+// every workgroup of the next kernel adds all partials in the same fixed
+// order, so all of them derive bitwise the same alpha / beta / stop decision
+// without atomics, fences or grid barriers.  A scalar is never written by the
+// kernel that reads it: K1 alone reads `done`; K2 reads `run2` (set by K1)
+// and sets `run3`; K3 reads `run3`; only workgroup 0 writes scalars.
 #include "dev_common.hpp"
+
+#define REGION(part, k) ((part) + (size_t)(k) * 4 * SFR_MAXG)
 
 // ---- utility kernels --------------------------------------------------------
 
@@ -50,7 +57,14 @@ __global__ __launch_bounds__(SFR_BLOCK) void k_fill(int64_t n, double v,
   if (i < n) x[i] = v;
 }
 
-// Jacobi-scaled FP32 copy: a_ij / sqrt(a_ii a_jj); ELL rows are i = k mod n.
+__global__ __launch_bounds__(SFR_BLOCK) void k_fill32(int64_t n, float v,
+                                                      float *__restrict__ x)
+{
+  const int64_t i = (int64_t)blockIdx.x * SFR_BLOCK + threadIdx.x;
+  if (i < n) x[i] = v;
+}
+
+// Jacobi-scaled FP32 copy for the mixed solve: a_ij / sqrt(a_ii a_jj).
 __global__ __launch_bounds__(SFR_BLOCK) void k_dscale(int64_t n,
     const double *__restrict__ ad, double *__restrict__ dscale)
 {
@@ -65,7 +79,7 @@ __global__ __launch_bounds__(SFR_BLOCK) void k_scale32_ell(int64_t n,
 {
   const int64_t k = (int64_t)blockIdx.x * SFR_BLOCK + threadIdx.x;
   if (k < nnz) {
-    const int64_t i = k % n;
+    const int64_t i = k % n;           // slot-major: row of entry k
     val32[k] = (float)(val[k] * dscale[i] * dscale[col[k]]);
   }
 }
@@ -99,14 +113,39 @@ __global__ __launch_bounds__(SFR_BLOCK) void k_spmv(Op A,
   }
 }
 
-// ---- PCG --------------------------------------------------------------------
+// one workgroup: r.D^-1 r of the setup -> d, and the loop state
+__global__ __launch_bounds__(SFR_BLOCK) void k_pcg_start(const double *part,
+    int g, sfr_scalars *sc)
+{
+  double t[1];
+  reduce_partials<1>(part, g, t);
+  if (threadIdx.x == 0) {
+    sc->d = t[0];
+    sc->iter = 0;
+    sc->outer = 0;
+    sc->ru = 0;
+    // b = 0, or an exact initial guess: nothing to iterate
+    sc->done = (sc->norm_b == 0.0 || t[0] == 0.0) ? SFR_DONE_CONV : SFR_DONE_RUN;
+  }
+}
 
-// z = D^-1 b; b.z and b.b; x = 0 when newX
+// one workgroup: x.Ax, x.x, r.D^-1 r, r.r of the final kernel -> res
+__global__ __launch_bounds__(SFR_BLOCK) void k_final_sum(const double *part,
+    int g, sfr_scalars *sc)
+{
+  double t[4];
+  reduce_partials<4>(part, g, t);
+  if (threadIdx.x == 0)
+    for (int k = 0; k < 4; k++) sc->res[k] = t[k];
+}
+
+// ---- PCG ----------------------------------------------------------------------
+
+// partials of b.D^-1 b and b.b; x = 0 when newX
 template <int RPT>
-__global__ __launch_bounds__(SFR_BLOCK) void k_pcg_setup1(int64_t n,
+__global__ __launch_bounds__(SFR_BLOCK) void k_pcg_s1(int64_t n,
     const double *__restrict__ b, const double *__restrict__ dinv,
-    double *__restrict__ x, int newX, double tol, unsigned maxit,
-    sfr_scalars *sc, double *part)
+    double *__restrict__ x, int newX, double *part)
 {
   constexpr int RPC = RPT * SFR_BLOCK;
   double acc[2] = {0.0, 0.0};
@@ -123,25 +162,30 @@ __global__ __launch_bounds__(SFR_BLOCK) void k_pcg_setup1(int64_t n,
       }
     }
   }
-  if (grid_sum<2>(acc, part, &sc->counter) && threadIdx.x == 0) {
-    sc->norm_b = sqrt(fabs(acc[0]));
-    sc->conv = tol * sc->norm_b;
-    sc->bb = acc[1];
+  store_partials<2>(acc, part);
+}
+
+// norms of b; r = b - Ax, p = D^-1 r; partials of r.p
+template <class Op, int RPT>
+__global__ __launch_bounds__(SFR_BLOCK) void k_pcg_s2(Op A,
+    const double *__restrict__ b, const double *__restrict__ dinv,
+    double *__restrict__ x, double *__restrict__ r, double *__restrict__ p,
+    int newX, double tol, unsigned maxit, const double *pin, int gin,
+    double *pout, sfr_scalars *sc)
+{
+  double nb[2];
+  reduce_partials<2>(pin, gin, nb);
+  const double norm_b = sqrt(fabs(nb[0]));
+  const bool zero_b = norm_b == 0.0;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    sc->norm_b = norm_b;
+    sc->conv = tol * norm_b;
+    sc->bb = nb[1];
     sc->tol = tol;
     sc->maxit = maxit;
   }
-}
-
-// r = b - Ax, p = D^-1 r, d = r.p   (b == 0: x = r = p = 0, done at once)
-template <class Op, int RPT>
-__global__ __launch_bounds__(SFR_BLOCK) void k_pcg_setup2(Op A,
-    const double *__restrict__ b, const double *__restrict__ dinv,
-    double *__restrict__ x, double *__restrict__ r, double *__restrict__ p,
-    int newX, sfr_scalars *sc, double *part)
-{
   constexpr int L = Op::L, RPC = RPT * (SFR_BLOCK / L);
   const bool lead = (threadIdx.x % L) == 0;
-  const bool zero_b = sc->norm_b == 0.0;
   double acc[1] = {0.0};
   for (int64_t c = xcd_chunk(blockIdx.x, gridDim.x); c * RPC < A.n;
        c += gridDim.x) {
@@ -163,23 +207,20 @@ __global__ __launch_bounds__(SFR_BLOCK) void k_pcg_setup2(Op A,
       }
     }
   }
-  if (grid_sum<1>(acc, part, &sc->counter) && threadIdx.x == 0) {
-    sc->d = acc[0];
-    sc->iter = 0;
-    sc->seal = 0;
-    sc->done = zero_b ? SFR_DONE_CONV : SFR_DONE_RUN;
-  }
+  store_partials<1>(acc, pout);
 }
 
 template <class Op, int RPT>
 __global__ __launch_bounds__(SFR_BLOCK) void k_pcg_spmv(Op A,
-    const double *__restrict__ p, double *__restrict__ q, sfr_scalars *sc,
-    double *part)
+    const double *__restrict__ p, double *__restrict__ q, double *pout,
+    sfr_scalars *sc)
 {
+  const bool first = blockIdx.x == 0 && threadIdx.x == 0;
   if (sc->done) {
-    if (blockIdx.x == 0 && threadIdx.x == 0) sc->seal = 1;
+    if (first) sc->run2 = 0;
     return;
   }
+  if (first) sc->run2 = 1;
   constexpr int L = Op::L, RPC = RPT * (SFR_BLOCK / L);
   const bool lead = (threadIdx.x % L) == 0;
   double acc[1] = {0.0};
@@ -196,21 +237,30 @@ __global__ __launch_bounds__(SFR_BLOCK) void k_pcg_spmv(Op A,
       }
     }
   }
-  if (grid_sum<1>(acc, part, &sc->counter) && threadIdx.x == 0) {
-    const double alpha = sc->d / acc[0];
-    sc->alpha = alpha;
-    if (!(acc[0] != 0.0) || !isfinite(alpha)) sc->done = SFR_DONE_BREAK;
-  }
+  store_partials<1>(acc, pout);
 }
 
 template <int RPT>
 __global__ __launch_bounds__(SFR_BLOCK) void k_pcg_resid(int64_t n,
     const double *__restrict__ q, const double *__restrict__ dinv,
-    double *__restrict__ r, sfr_scalars *sc, double *part)
+    double *__restrict__ r, const double *pin, int gin, double *pout,
+    sfr_scalars *sc)
 {
-  if (sc->done) return;
+  const bool first = blockIdx.x == 0 && threadIdx.x == 0;
+  if (!sc->run2) {
+    if (first) sc->run3 = 0;
+    return;
+  }
+  double pq[1];
+  reduce_partials<1>(pin, gin, pq);
+  const double d = sc->d;
+  const double alpha = d / pq[0];
+  if (!(pq[0] != 0.0) || !isfinite(alpha)) {
+    if (first) { sc->done = SFR_DONE_BREAK; sc->run3 = 0; }
+    return;
+  }
+  if (first) { sc->alpha = alpha; sc->d_prev = d; sc->run3 = 1; }
   constexpr int RPC = RPT * SFR_BLOCK;
-  const double alpha = sc->alpha;
   double acc[1] = {0.0};
   for (int64_t c = xcd_chunk(blockIdx.x, gridDim.x); c * RPC < n;
        c += gridDim.x) {
@@ -224,26 +274,29 @@ __global__ __launch_bounds__(SFR_BLOCK) void k_pcg_resid(int64_t n,
       }
     }
   }
-  if (grid_sum<1>(acc, part, &sc->counter) && threadIdx.x == 0) {
-    const double di = acc[0];
-    sc->beta = di / sc->d;
-    sc->d = di;
-    const unsigned it = sc->iter + 1;
-    sc->iter = it;
-    if (sqrt(fabs(di)) < sc->conv) sc->done = SFR_DONE_CONV;
-    else if (!isfinite(di)) sc->done = SFR_DONE_BREAK;
-    else if (it >= sc->maxit) sc->done = SFR_DONE_MAXIT;
-  }
+  store_partials<1>(acc, pout);
 }
 
 template <int RPT>
 __global__ __launch_bounds__(SFR_BLOCK) void k_pcg_dir(int64_t n,
     const double *__restrict__ r, const double *__restrict__ dinv,
-    double *__restrict__ x, double *__restrict__ p, sfr_scalars *sc)
+    double *__restrict__ x, double *__restrict__ p, const double *pin,
+    int gin, sfr_scalars *sc)
 {
-  if (sc->seal || sc->done == SFR_DONE_BREAK) return;
+  if (!sc->run3) return;
+  double rz[1];
+  reduce_partials<1>(pin, gin, rz);
+  const double di = rz[0];
+  const double alpha = sc->alpha, beta = di / sc->d_prev;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    const unsigned it = sc->iter + 1;
+    sc->iter = it;
+    sc->d = di;
+    if (sqrt(fabs(di)) < sc->conv || di == 0.0) sc->done = SFR_DONE_CONV;
+    else if (!isfinite(di)) sc->done = SFR_DONE_BREAK;
+    else if (it >= sc->maxit) sc->done = SFR_DONE_MAXIT;
+  }
   constexpr int RPC = RPT * SFR_BLOCK;
-  const double alpha = sc->alpha, beta = sc->beta;
   for (int64_t c = xcd_chunk(blockIdx.x, gridDim.x); c * RPC < n;
        c += gridDim.x) {
 #pragma unroll
@@ -258,11 +311,11 @@ __global__ __launch_bounds__(SFR_BLOCK) void k_pcg_dir(int64_t n,
   }
 }
 
-// x.Ax, x.x, r.D^-1 r and r.r with r = b - Ax  (src/Cg.c:155-176)
+// partials of x.Ax, x.x, r.D^-1 r and r.r with r = b - Ax (src/Cg.c:155-176)
 template <class Op, int RPT>
 __global__ __launch_bounds__(SFR_BLOCK) void k_final(Op A,
     const double *__restrict__ b, const double *__restrict__ x,
-    const double *__restrict__ dinv, sfr_scalars *sc, double *part)
+    const double *__restrict__ dinv, double *pout)
 {
   constexpr int L = Op::L, RPC = RPT * (SFR_BLOCK / L);
   const bool lead = (threadIdx.x % L) == 0;
@@ -283,19 +336,22 @@ __global__ __launch_bounds__(SFR_BLOCK) void k_final(Op A,
       }
     }
   }
-  if (grid_sum<4>(acc, part, &sc->counter) && threadIdx.x == 0) {
-    for (int k = 0; k < 4; k++) sc->res[k] = acc[k];
-  }
+  store_partials<4>(acc, pout);
 }
 
 // ---- launchers ----------------------------------------------------------------
+
+static inline unsigned blocks1(int64_t n)
+{
+  return (unsigned)((n + SFR_BLOCK - 1) / SFR_BLOCK);
+}
 
 extern "C" int sfr_k_gather(hipStream_t s, int64_t nnz, const int32_t *vmap,
                             const double *raw, double *val)
 {
   if (nnz <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_gather, dim3((unsigned)((nnz + SFR_BLOCK - 1) / SFR_BLOCK)),
-                     dim3(SFR_BLOCK), 0, s, nnz, vmap, raw, val);
+  hipLaunchKernelGGL(k_gather, dim3(blocks1(nnz)), dim3(SFR_BLOCK), 0, s, nnz,
+                     vmap, raw, val);
   return hipGetLastError();
 }
 
@@ -303,16 +359,22 @@ extern "C" int sfr_k_diag(hipStream_t s, int64_t n, const int32_t *dmap,
                           const double *raw, const double *raw_ad, double *ad,
                           double *dinv)
 {
-  hipLaunchKernelGGL(k_diag, dim3((unsigned)((n + SFR_BLOCK - 1) / SFR_BLOCK)),
-                     dim3(SFR_BLOCK), 0, s, n, dmap, raw, raw_ad, ad, dinv);
+  hipLaunchKernelGGL(k_diag, dim3(blocks1(n)), dim3(SFR_BLOCK), 0, s, n, dmap,
+                     raw, raw_ad, ad, dinv);
   return hipGetLastError();
 }
 
 extern "C" int sfr_k_fill(hipStream_t s, int64_t n, double v, double *x)
 {
   if (n <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_fill, dim3((unsigned)((n + SFR_BLOCK - 1) / SFR_BLOCK)),
-                     dim3(SFR_BLOCK), 0, s, n, v, x);
+  hipLaunchKernelGGL(k_fill, dim3(blocks1(n)), dim3(SFR_BLOCK), 0, s, n, v, x);
+  return hipGetLastError();
+}
+
+extern "C" int sfr_k_fill32(hipStream_t s, int64_t n, float v, float *x)
+{
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_fill32, dim3(blocks1(n)), dim3(SFR_BLOCK), 0, s, n, v, x);
   return hipGetLastError();
 }
 
@@ -320,19 +382,17 @@ extern "C" int sfr_k_scale32(hipStream_t s, const sfr_mat_view *A,
                              double *dscale, float *val32)
 {
   const int64_t n = A->n;
-  const unsigned gn = (unsigned)((n + SFR_BLOCK - 1) / SFR_BLOCK);
-  hipLaunchKernelGGL(k_dscale, dim3(gn), dim3(SFR_BLOCK), 0, s, n, A->ad,
-                     dscale);
+  hipLaunchKernelGGL(k_dscale, dim3(blocks1(n)), dim3(SFR_BLOCK), 0, s, n,
+                     A->ad, dscale);
   if (A->fmt == SFR_FMT_ELL) {
     const int64_t nnz = n * A->slots;
     if (nnz > 0)
-      hipLaunchKernelGGL(k_scale32_ell,
-                         dim3((unsigned)((nnz + SFR_BLOCK - 1) / SFR_BLOCK)),
-                         dim3(SFR_BLOCK), 0, s, n, nnz, A->col, A->val,
-                         (const double *)dscale, val32);
+      hipLaunchKernelGGL(k_scale32_ell, dim3(blocks1(nnz)), dim3(SFR_BLOCK), 0,
+                         s, n, nnz, A->col, A->val, (const double *)dscale,
+                         val32);
   } else {
-    hipLaunchKernelGGL(k_scale32_csr, dim3(gn), dim3(SFR_BLOCK), 0, s, n,
-                       A->rowptr, A->col, A->val, (const double *)dscale,
+    hipLaunchKernelGGL(k_scale32_csr, dim3(blocks1(n)), dim3(SFR_BLOCK), 0, s,
+                       n, A->rowptr, A->col, A->val, (const double *)dscale,
                        val32);
   }
   return hipGetLastError();
@@ -349,6 +409,17 @@ extern "C" int sfr_k_spmv(hipStream_t s, const sfr_mat_view *A,
   return hipGetLastError();
 }
 
+// partials of b.D^-1 b, b.b into region 2; x = 0 when newX
+extern "C" int sfr_k_pcg_norm(hipStream_t s, const sfr_mat_view *A,
+                              sfr_geom g0, const double *b, double *x,
+                              int newX, double *part, int *nparts)
+{
+  const sfr_geom gv = {g0.rpt, grid_for(A->n, g0.rpt, 1)};
+  SFR_LAUNCH_V(k_pcg_s1, gv, s, A->n, b, A->dinv, x, newX, REGION(part, 2));
+  *nparts = gv.grid;
+  return hipGetLastError();
+}
+
 extern "C" int sfr_k_pcg_setup(hipStream_t s, const sfr_mat_view *A,
                                sfr_geom g0, const double *b, double *x,
                                double *r, double *p, int newX, double tol,
@@ -356,19 +427,15 @@ extern "C" int sfr_k_pcg_setup(hipStream_t s, const sfr_mat_view *A,
 {
   const sfr_geom gv = {g0.rpt, grid_for(A->n, g0.rpt, 1)};
   const sfr_geom gm = {g0.rpt, grid_for(A->n, g0.rpt, op_lanes(A))};
-  switch (gv.rpt) {
-  case 1: hipLaunchKernelGGL(k_pcg_setup1<1>, dim3(gv.grid), dim3(SFR_BLOCK), 0, s,
-                             A->n, b, A->dinv, x, newX, tol, maxit, sc, part); break;
-  case 2: hipLaunchKernelGGL(k_pcg_setup1<2>, dim3(gv.grid), dim3(SFR_BLOCK), 0, s,
-                             A->n, b, A->dinv, x, newX, tol, maxit, sc, part); break;
-  default: hipLaunchKernelGGL(k_pcg_setup1<4>, dim3(gv.grid), dim3(SFR_BLOCK), 0, s,
-                              A->n, b, A->dinv, x, newX, tol, maxit, sc, part); break;
-  }
+  double *rc = REGION(part, 2), *rb = REGION(part, 1);
+  SFR_LAUNCH_V(k_pcg_s1, gv, s, A->n, b, A->dinv, x, newX, rc);
   with_op64(A, [&](auto op) {
     using Op = decltype(op);
-    SFR_LAUNCH_RPT(k_pcg_setup2, Op, gm, s, op, b, A->dinv, x, r, p, newX, sc,
-                   part);
+    SFR_LAUNCH_RPT(k_pcg_s2, Op, gm, s, op, b, A->dinv, x, r, p, newX, tol,
+                   maxit, (const double *)rc, gv.grid, rb, sc);
   });
+  hipLaunchKernelGGL(k_pcg_start, dim3(1), dim3(SFR_BLOCK), 0, s,
+                     (const double *)rb, gm.grid, sc);
   return hipGetLastError();
 }
 
@@ -378,24 +445,15 @@ extern "C" int sfr_k_pcg_iter(hipStream_t s, const sfr_mat_view *A,
 {
   const sfr_geom gv = {g0.rpt, grid_for(A->n, g0.rpt, 1)};
   const sfr_geom gm = {g0.rpt, grid_for(A->n, g0.rpt, op_lanes(A))};
+  double *ra = REGION(part, 0), *rb = REGION(part, 1);
   with_op64(A, [&](auto op) {
     using Op = decltype(op);
-    SFR_LAUNCH_RPT(k_pcg_spmv, Op, gm, s, op, p, q, sc, part);
+    SFR_LAUNCH_RPT(k_pcg_spmv, Op, gm, s, op, p, q, ra, sc);
   });
-  switch (gv.rpt) {
-  case 1:
-    hipLaunchKernelGGL(k_pcg_resid<1>, dim3(gv.grid), dim3(SFR_BLOCK), 0, s, A->n, q, A->dinv, r, sc, part);
-    hipLaunchKernelGGL(k_pcg_dir<1>, dim3(gv.grid), dim3(SFR_BLOCK), 0, s, A->n, r, A->dinv, x, p, sc);
-    break;
-  case 2:
-    hipLaunchKernelGGL(k_pcg_resid<2>, dim3(gv.grid), dim3(SFR_BLOCK), 0, s, A->n, q, A->dinv, r, sc, part);
-    hipLaunchKernelGGL(k_pcg_dir<2>, dim3(gv.grid), dim3(SFR_BLOCK), 0, s, A->n, r, A->dinv, x, p, sc);
-    break;
-  default:
-    hipLaunchKernelGGL(k_pcg_resid<4>, dim3(gv.grid), dim3(SFR_BLOCK), 0, s, A->n, q, A->dinv, r, sc, part);
-    hipLaunchKernelGGL(k_pcg_dir<4>, dim3(gv.grid), dim3(SFR_BLOCK), 0, s, A->n, r, A->dinv, x, p, sc);
-    break;
-  }
+  SFR_LAUNCH_V(k_pcg_resid, gv, s, A->n, q, A->dinv, r, (const double *)ra,
+               gm.grid, rb, sc);
+  SFR_LAUNCH_V(k_pcg_dir, gv, s, A->n, r, A->dinv, x, p, (const double *)rb,
+               gv.grid, sc);
   return hipGetLastError();
 }
 
@@ -404,43 +462,13 @@ extern "C" int sfr_k_final(hipStream_t s, const sfr_mat_view *A, sfr_geom g0,
                            double *part)
 {
   const sfr_geom gm = {g0.rpt, grid_for(A->n, g0.rpt, op_lanes(A))};
+  double *rd = REGION(part, 3);
   with_op64(A, [&](auto op) {
     using Op = decltype(op);
-    SFR_LAUNCH_RPT(k_final, Op, gm, s, op, b, x, A->dinv, sc, part);
+    SFR_LAUNCH_RPT(k_final, Op, gm, s, op, b, x, A->dinv, rd);
   });
-  return hipGetLastError();
-}
-
-// b.D^-1 b, b.b, tolerance and x = 0 when newX (first half of the PCG setup)
-extern "C" int sfr_k_pcg_norm(hipStream_t s, const sfr_mat_view *A,
-                              sfr_geom g0, const double *b, double *x,
-                              int newX, double tol, unsigned maxit,
-                              sfr_scalars *sc, double *part)
-{
-  const sfr_geom gv = {g0.rpt, grid_for(A->n, g0.rpt, 1)};
-  switch (gv.rpt) {
-  case 1: hipLaunchKernelGGL(k_pcg_setup1<1>, dim3(gv.grid), dim3(SFR_BLOCK), 0, s,
-                             A->n, b, A->dinv, x, newX, tol, maxit, sc, part); break;
-  case 2: hipLaunchKernelGGL(k_pcg_setup1<2>, dim3(gv.grid), dim3(SFR_BLOCK), 0, s,
-                             A->n, b, A->dinv, x, newX, tol, maxit, sc, part); break;
-  default: hipLaunchKernelGGL(k_pcg_setup1<4>, dim3(gv.grid), dim3(SFR_BLOCK), 0, s,
-                              A->n, b, A->dinv, x, newX, tol, maxit, sc, part); break;
-  }
-  return hipGetLastError();
-}
-
-__global__ __launch_bounds__(SFR_BLOCK) void k_fill32(int64_t n, float v,
-                                                      float *__restrict__ x)
-{
-  const int64_t i = (int64_t)blockIdx.x * SFR_BLOCK + threadIdx.x;
-  if (i < n) x[i] = v;
-}
-
-extern "C" int sfr_k_fill32(hipStream_t s, int64_t n, float v, float *x)
-{
-  if (n <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_fill32, dim3((unsigned)((n + SFR_BLOCK - 1) / SFR_BLOCK)),
-                     dim3(SFR_BLOCK), 0, s, n, v, x);
+  hipLaunchKernelGGL(k_final_sum, dim3(1), dim3(SFR_BLOCK), 0, s,
+                     (const double *)rd, gm.grid, sc);
   return hipGetLastError();
 }
 
@@ -452,7 +480,7 @@ extern "C" int sfr_k_pcg_spmv_only(hipStream_t s, const sfr_mat_view *A,
   const sfr_geom gm = {g0.rpt, grid_for(A->n, g0.rpt, op_lanes(A))};
   with_op64(A, [&](auto op) {
     using Op = decltype(op);
-    SFR_LAUNCH_RPT(k_pcg_spmv, Op, gm, s, op, p, q, sc, part);
+    SFR_LAUNCH_RPT(k_pcg_spmv, Op, gm, s, op, p, q, REGION(part, 0), sc);
   });
   return hipGetLastError();
 }

@@ -71,7 +71,7 @@ int sfr_init(sfr_context **out, int device)
   if (!ctx) return SFR_BAD_ARG;
   ctx->device = device;
   ctx->chunk = 16;
-  ctx->inner_tol = 1e-3;
+  ctx->ru_delta = 0.1;
   if (hipSetDevice(device) != hipSuccess) { free(ctx); return SFR_NO_DEVICE; }
   int rc = SFR_OK;
 #define INIT_TRY(call) do { if ((call) != hipSuccess) { rc = SFR_HIP_ERROR; goto fail; } } while (0)
@@ -340,7 +340,11 @@ int sfr_matrix_create(sfr_context *ctx, const sfr_system *sys, int format,
   A->n = n; A->fmt = fmt; A->storage = sys->storage; A->unsym = sys->unsym;
   A->nad = sys->nad; A->nraw = nraw; A->diag_in_al = sys->storage == SFR_CSR;
   A->h_ia = sys->ia; A->h_ja = sys->ja;
-  A->rpt = 2;
+  /* rows per thread: the smallest that keeps a streaming kernel at or below
+   * 1024 workgroups, so a consumer reduces at most 1024 partials */
+  A->rpt = 1;
+  while (A->rpt < 8 && (n + (int64_t)SFR_BLOCK * A->rpt - 1) / ((int64_t)SFR_BLOCK * A->rpt) > 1024)
+    A->rpt *= 2;
   A->lanes = 4;
   int32_t *hcol = NULL, *hmap = NULL, *hptr = NULL;
   int rc = SFR_OK;
@@ -430,6 +434,18 @@ int sfr_matrix_prepare_mixed(sfr_context *ctx, sfr_matrix *A)
   sfr_mat_view v = sfr_view(A);
   TRY(sfr_k_scale32(ctx->stream, &v, A->dscale, A->val32));
   A->have32 = 1;
+  return SFR_OK;
+}
+
+int sfr_matrix_tune(sfr_context *ctx, sfr_matrix *A, int rpt, int lanes)
+{
+  if (!ctx || !A) return SFR_BAD_ARG;
+  if (rpt != 1 && rpt != 2 && rpt != 4 && rpt != 8)
+    return sfr_set_error(ctx, SFR_BAD_ARG, "rows per thread %d", rpt);
+  if (lanes != 1 && lanes != 2 && lanes != 4 && lanes != 8)
+    return sfr_set_error(ctx, SFR_BAD_ARG, "lanes per row %d", lanes);
+  A->rpt = rpt;
+  if (A->fmt == SFR_FMT_CSR) A->lanes = lanes;
   return SFR_OK;
 }
 

@@ -87,8 +87,9 @@ static int get_graph(sfr_context *ctx, int kind, const void *k0,
   return SFR_OK;
 }
 
-/* Replay the loop graph until the device reports the watched flag set.
- * which = 0: sc->done; which = 1: sc->inner_done.  Two chunks in flight. */
+/* Replay the loop graph until the device reports the loop finished:
+ * which = 0: sc->done set; which = 1: sc->done or sc->ru set.  Two chunks
+ * are kept in flight. */
 static int run_loop(sfr_context *ctx, hipGraphExec_t ex, int which,
                     unsigned max_chunks)
 {
@@ -107,9 +108,9 @@ static int run_loop(sfr_context *ctx, hipGraphExec_t ex, int which,
     const int slot = (int)(seen & 1);
     TRY(hipEventSynchronize(ctx->ev_poll[slot]));
     seen++;
-    /* layout from &iter: iter, done, maxit, seal, counter, inner_iter, inner_done */
+    /* layout from &iter: iter, done, maxit, outer, ru, ... */
     const unsigned *p = ctx->h_poll + 8 * slot;
-    const unsigned flag = which ? p[6] : p[1];
+    const unsigned flag = which ? (p[1] | p[4]) : p[1];
     if (flag) break;
   }
   return SFR_OK;
@@ -206,19 +207,24 @@ static int solve_pcg_mixed(sfr_context *ctx, sfr_matrix *A, double tol,
   float *const *w32 = ctx->w32;
   hipStream_t s = ctx->stream;
   TRY(hipEventRecord(ctx->ev_start, s));
-  /* norms of b, tolerance (k_pcg_setup's first kernel), x = 0 if newX */
-  TRY(sfr_k_pcg_norm(s, &v, g, d_b, d_x, newX, tol, maxit, ctx->d_sc, ctx->part));
-  int first = 1;
-  for (unsigned outer = 0;; outer++) {
-    TRY(sfr_k_mx_outer(s, &v, g, d_b, d_x, w32[0], first, newX, w32[1], w32[2], w32[0],
-                       ctx->inner_tol, maxit, ctx->d_sc, ctx->part));
-    first = 0;
+  int nparts = 0;
+  TRY(sfr_k_pcg_norm(s, &v, g, d_b, d_x, newX, ctx->part, &nparts));
+  TRY(sfr_k_mx_start(s, ctx->part, nparts, tol, maxit, ctx->ru_delta, ctx->d_sc));
+  TRY(sfr_k_mx_fold(s, &v, g, d_b, d_x, w32[0], w32[1], w32[2], 1, newX, ctx->d_sc,
+                    ctx->part));
+  /* every fold is triggered by a residual drop of delta (or the stop test):
+   * bound the number of folds generously */
+  for (int folds = 0; folds < 100000; folds++) {
+    CHECK(run_loop(ctx, ex, 1, chunk_budget(ctx, maxit)));
+    const unsigned *pl = NULL;
     TRY(hipMemcpyAsync(ctx->h_poll, &ctx->d_sc->iter, 8 * sizeof(unsigned),
                        hipMemcpyDeviceToHost, s));
     TRY(hipStreamSynchronize(s));
-    if (ctx->h_poll[1] != SFR_DONE_RUN) break;
-    CHECK(run_loop(ctx, ex, 1, chunk_budget(ctx, maxit)));
-    if (outer > maxit) break;
+    pl = ctx->h_poll;
+    if (pl[1] != SFR_DONE_RUN) break;
+    if (!pl[4]) break;                    /* loop stopped with no request */
+    TRY(sfr_k_mx_fold(s, &v, g, d_b, d_x, w32[0], w32[1], w32[2], 0, 0, ctx->d_sc,
+                      ctx->part));
   }
   return finish(ctx, &v, g, d_b, d_x, st);
 }
@@ -380,9 +386,9 @@ int sfr_set_option(sfr_context *ctx, const char *name, double value)
     ctx->chunk = (int)value;
     return SFR_OK;
   }
-  if (!strcmp(name, "inner_tol")) {
-    if (!(value > 0 && value < 1)) return sfr_set_error(ctx, SFR_BAD_ARG, "inner_tol %g", value);
-    ctx->inner_tol = value;
+  if (!strcmp(name, "ru_delta")) {
+    if (!(value > 0 && value < 1)) return sfr_set_error(ctx, SFR_BAD_ARG, "ru_delta %g", value);
+    ctx->ru_delta = value;
     return SFR_OK;
   }
   return sfr_set_error(ctx, SFR_BAD_ARG, "unknown option %s", name);

@@ -17,29 +17,32 @@ extern "C" {
 #endif
 
 #define SFR_BLOCK   256          /* threads per workgroup (4 wave64)        */
-#define SFR_MAXG    16384        /* max workgroups of a reducing kernel     */
+#define SFR_MAXG    16384        /* partial slots per reduction region      */
+#define SFR_GRIDCAP 2048         /* max workgroups of a streaming kernel    */
 #define SFR_MAXDOT  8            /* partial sums one kernel may reduce      */
 #define SFR_MAXSYS  8            /* systems of one shared-pattern batch     */
 
 /* Scalars of one Krylov solve, resident on the device.  The last workgroup
  * of each reducing kernel writes them; later kernels read them. */
 typedef struct sfr_scalars {
-  double   d;          /* PCG: r.D^-1 r   BiCGSTAB: r.r0   inner CG: r.r   */
+  double   d;          /* PCG: r.D^-1 r   BiCGSTAB: r.r0   inner CG: r.r     */
+  double   d_prev;     /* value of d the previous kernel started from      */
   double   alpha, beta, omega;
   double   norm_b;     /* reference norm of b                               */
   double   conv;       /* tol * norm_b                                      */
   double   bb;         /* b.b                                               */
   double   tol;
-  double   rho0;       /* inner: initial r.r                                */
-  double   inner_tol2; /* inner: stop when r.r <= inner_tol2 * rho0         */
+  double   rho_max;    /* mixed: largest inner r.r since the last update    */
+  double   delta2;     /* mixed: reliable-update threshold, squared         */
   double   res[6];     /* final / outer dots                                */
+  /* polled by the host (8 words from iter) */
   unsigned iter;       /* completed iterations                              */
   unsigned done;       /* SFR_DONE_*                                        */
   unsigned maxit;
-  unsigned seal;       /* set once the loop has been seen finished          */
-  unsigned counter;    /* last-workgroup arrival counter                    */
-  unsigned inner_iter, inner_done, inner_maxit;
-  unsigned outer;      /* mixed: FP64 refinement steps taken                */
+  unsigned outer;      /* mixed: reliable (FP64) updates taken              */
+  unsigned ru;         /* mixed: reliable update requested                  */
+  unsigned run2, run3, run4;  /* run flags handed from kernel to kernel     */
+  unsigned run5;
   unsigned pad[7];
 } sfr_scalars;
 
@@ -100,7 +103,7 @@ struct sfr_context {
   int          ngraph;
   sfr_matrix  *cached;          /* pattern cache of the host-pointer path  */
   int          chunk;           /* iterations per captured chunk           */
-  double       inner_tol;       /* MIXED inner relative reduction          */
+  double       ru_delta;        /* MIXED: reliable-update residual drop    */
 };
 
 int sfr_set_error(sfr_context *ctx, int code, const char *fmt, ...);
@@ -136,10 +139,10 @@ int sfr_k_pcg_setup(hipStream_t s, const sfr_mat_view *A, sfr_geom g,
 int sfr_k_pcg_iter(hipStream_t s, const sfr_mat_view *A, sfr_geom g,
                    double *x, double *r, double *p, double *q,
                    sfr_scalars *sc, double *part);
-/* b.D^-1 b, b.b, conv = tol*sqrt|b.D^-1 b|, x = 0 when newX */
+/* partials of b.D^-1 b and b.b (region 2), x = 0 when newX */
 int sfr_k_pcg_norm(hipStream_t s, const sfr_mat_view *A, sfr_geom g,
-                   const double *b, double *x, int newX, double tol,
-                   unsigned maxit, sfr_scalars *sc, double *part);
+                   const double *b, double *x, int newX, double *part,
+                   int *nparts);
 /* final quantities: x.Ax, x.x, r.D^-1 r, r.r with r = b - Ax -> sc->res */
 int sfr_k_final(hipStream_t s, const sfr_mat_view *A, sfr_geom g,
                 const double *b, const double *x, sfr_scalars *sc,
@@ -155,16 +158,16 @@ int sfr_k_bicg_iter(hipStream_t s, const sfr_mat_view *A, sfr_geom g,
                     double *z, double *v, double *t, sfr_scalars *sc,
                     double *part);
 
-/* mixed precision: FP64 outer residual + FP32 inner CG on D^-1/2 A D^-1/2 */
-/* outer step: x += D^-1/2 y (skipped when first), r = b - Ax in FP64, stop
- * test, next inner right-hand side rh = ph = D^-1/2 r, y = 0 */
-int sfr_k_mx_outer(hipStream_t s, const sfr_mat_view *A, sfr_geom g,
-                   const double *b, double *x, const float *y, int first,
-                   int newX, float *rh, float *ph, float *yz,
-                   double inner_tol, unsigned inner_maxit, sfr_scalars *sc,
-                   double *part);
+/* mixed precision: FP32 CG on D^-1/2 A D^-1/2 with FP64 reliable updates */
+int sfr_k_mx_start(hipStream_t s, const double *part, int nparts, double tol,
+                   unsigned maxit, double delta, sfr_scalars *sc);
+/* fold y into x, true FP64 residual, stop test, new direction; init = first
+ * call (p = r, y = 0), zero_x = x is known to be 0 */
+int sfr_k_mx_fold(hipStream_t s, const sfr_mat_view *A, sfr_geom g,
+                  const double *b, double *x, float *y, float *r, float *p,
+                  int init, int zero_x, sfr_scalars *sc, double *part);
 int sfr_k_mx_inner_iter(hipStream_t s, const sfr_mat_view *A, sfr_geom g,
-                        float *y, float *rh, float *ph, float *qh,
+                        float *y, float *r, float *p, float *q,
                         sfr_scalars *sc, double *part);
 
 #ifdef __cplusplus

@@ -85,6 +85,9 @@ class DeviceMatrix:
         self.ctx._check(lib.sfr_matrix_set_values(self.ctx.h, self.h, ptr(al),
                                                   ptr(ad), ptr(au)))
 
+    def tune(self, rpt, lanes=4):
+        self.ctx._check(lib.sfr_matrix_tune(self.ctx.h, self.h, rpt, lanes))
+
     def info(self):
         n, f, s, nnz = C.c_int32(), C.c_int(), C.c_int(), C.c_int64()
         lib.sfr_matrix_info(self.h, C.byref(n), C.byref(f), C.byref(s),
