@@ -1,0 +1,109 @@
+"""Writes the golden fixtures under tests/golden/ (run where the reference
+tree and oracle/_ref exist: `python tests/golden/make_golden.py`).
+
+  retangulo_cells.txt        cell-node lists of the reference's 20x10 quad
+                             test mesh (test/input/coo.zip, coo/retangulo.dat)
+  retangulo_bin.mtx,         the reference's own MatrixMarket pattern goldens
+  retangulo_rcm_bin.mtx      (test/output/coo/), reord false / true
+  solver_config/*.dat        the reference example's solver config files
+  ref_vectors.json           outputs of the reference's compiled C
+                             (oracle/_ref): GENRCM orderings (digest + first
+                             entries) and solverC results on small systems
+  ref_solutions.npz          the solution vectors of those solves
+"""
+import hashlib
+import io
+import json
+import os
+import shutil
+import sys
+import zipfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+REF = "/root/reference"
+sys.path.insert(0, ROOT)
+
+
+def sha(a):
+    return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
+
+
+def mesh_fixture():
+    with zipfile.ZipFile(os.path.join(REF, "test/input/coo.zip")) as z:
+        txt = z.read("coo/retangulo.dat").decode()
+    lines = txt.splitlines()
+    i0 = next(i for i, l in enumerate(lines) if l.strip() == "cells") + 1
+    i1 = next(i for i, l in enumerate(lines) if l.strip().startswith("endCell"))
+    with open(os.path.join(HERE, "retangulo_cells.txt"), "w") as f:
+        f.write("# cells of coo/retangulo.dat (reference test/input/coo.zip):"
+                " id mat type nen ndm? n1 n2 n3 n4\n")
+        for l in lines[i0:i1]:
+            f.write(" ".join(l.split()) + "\n")
+    for name in ("retangulo_bin.mtx", "retangulo_rcm_bin.mtx"):
+        shutil.copy(os.path.join(REF, "test/output/coo", name),
+                    os.path.join(HERE, name))
+    os.makedirs(os.path.join(HERE, "solver_config"), exist_ok=True)
+    for name in ("solverPres.dat", "solverVel.dat"):
+        shutil.copy(os.path.join(REF, "exemplos/lid_driven_cavity/config", name),
+                    os.path.join(HERE, "solver_config", name))
+
+
+def ref_vectors():
+    import oracle
+    from sfr3d_amd import structure as S, systems as Y
+    R = oracle.ref()
+    R.ref_init(1 << 32)
+    out = {"rcm": {}, "solves": []}
+    meshes = [("hex", (5, 4, 3)), ("hex", (9, 8, 7)), ("hex", (32, 32, 32)),
+              ("hex", (96, 96, 96)), ("hex", (128, 128, 128)),
+              ("quad", (128, 128)), ("quad", (20, 10))]
+    for kind, dims in meshes:
+        nelcon, nviz = (S.hex_adjacency(*dims) if kind == "hex"
+                        else S.quad_adjacency(*dims))
+        numel, mv = nelcon.shape
+        num = np.zeros(numel, np.int32)
+        R.ref_reord(num, np.ascontiguousarray(nelcon.ravel()), nviz, mv,
+                    numel, 1)
+        out["rcm"][f"{kind}_{'x'.join(map(str, dims))}"] = {
+            "sha256": sha(num), "head": num[:16].tolist(),
+            "tail": num[-16:].tolist()}
+        print("rcm", kind, dims, flush=True)
+    sols = {}
+    cases = [
+        ("pcg_poisson3d_12_csrd", lambda: Y.poisson3d(12, seed=3, storage=2), 1, None),
+        ("pcg_poisson3d_12_csrc", lambda: Y.poisson3d(12, seed=3, storage=3), 1, None),
+        ("pcg_poisson3d_12_ell", lambda: Y.poisson3d(12, seed=3, storage=4), 1, None),
+        ("pcg_poisson2d_20x16_pinned", lambda: Y.poisson2d(20, 16, seed=5, storage=2), 1, None),
+        ("bicg_momentum_10_csrd", lambda: Y.momentum3d(10), 2, 0),
+        ("bicg_momentum_10_csrd_w", lambda: Y.momentum3d(10), 2, 2),
+        ("bicg_momentum_10_csrc", lambda: Y.momentum3d(10, storage=3), 2, 1),
+        ("bicg_poisson3d_10_ell", lambda: Y.poisson3d(10, seed=4, storage=4), 2, None),
+    ]
+    for name, make, solver, comp in cases:
+        B = make()
+        s = B.system
+        b = B.b if comp is None else B.extra["b_all"][comp]
+        x = np.zeros(s.neq)
+        res = np.zeros(4)
+        ad = s.ad if s.ad is not None else s.al
+        rc = R.ref_solve(s.neq, s.nad, s.ia, s.ja, np.ascontiguousarray(s.al),
+                         np.ascontiguousarray(ad), None,
+                         np.ascontiguousarray(b), x, 1e-8, 5000, s.storage,
+                         solver, 1, int(s.unsym), res, None, 0)
+        assert rc == 0
+        out["solves"].append({"name": name, "iters": int(res[0]),
+                              "xKx": res[1], "norm_x": res[2],
+                              "x_sha256": sha(x)})
+        sols[name] = x
+        print("solve", name, int(res[0]), flush=True)
+    with open(os.path.join(HERE, "ref_vectors.json"), "w") as f:
+        json.dump(out, f, indent=1)
+    np.savez_compressed(os.path.join(HERE, "ref_solutions.npz"), **sols)
+
+
+if __name__ == "__main__":
+    mesh_fixture()
+    ref_vectors()
