@@ -40,6 +40,9 @@ def main():
     ap.add_argument("--n", type=int, default=128)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--profile-kernels", action="store_true",
+                    help="only launch the two SpMV kernels and a plain copy "
+                         "of known size (for rocprofv3 counter runs)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is "
@@ -64,6 +67,10 @@ def main():
     info = A.info()
     db = ctx.array(B.b)
     dx = ctx.empty(neq)
+
+    if args.profile_kernels:
+        profile_kernels(ctx, A, lib)
+        return
 
     def run(precision, steps, warmup):
         st = None
@@ -168,6 +175,27 @@ def main():
     print(json.dumps(out), flush=True)
     A.destroy()
     ctx.close()
+
+
+def profile_kernels(ctx, A, lib):
+    """A few launches of each SpMV kernel (FP32 inner, FP64) and one
+    device-to-device copy of 256 MiB, so byte counters can be scaled against
+    a kernel whose traffic is known (256 MiB read + 256 MiB written)."""
+    ms = C.c_double()
+    lib.sfr_kernel_time(ctx.h, A.h, 1, 5, C.byref(ms))
+    log(f"[profile] k_mx_spmv {ms.value * 1e3:.1f} us")
+    lib.sfr_kernel_time(ctx.h, A.h, 0, 5, C.byref(ms))
+    log(f"[profile] k_pcg_spmv {ms.value * 1e3:.1f} us")
+    nbytes = 256 << 20
+    src = ctx.empty(nbytes // 4, np.float32)
+    dst = ctx.empty(nbytes // 4, np.float32)
+    hip = C.CDLL("libamdhip64.so")
+    hip.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+    rc = hip.hipMemcpy(C.c_void_p(dst.ptr), C.c_void_p(src.ptr), nbytes, 3)
+    lib.sfr_synchronize(ctx.h)
+    log(f"[profile] copy 256 MiB done (rc={rc})")
+    src.free()
+    dst.free()
 
 
 def cpu_baseline(sysm, b, threads, x_gpu64):
