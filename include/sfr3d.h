@@ -132,6 +132,24 @@ int sfr_solve_device(sfr_context *ctx, sfr_matrix *A, int solver,
                      const double *d_b, double *d_x, int newX,
                      sfr_stats *stats);
 
+/* Shared-pattern batch (SFR3D's u/v/w momentum solves, reference
+ * src/Simple.c:2738-2750, 2980-3036): nsys (1..4) systems with the same
+ * ia/ja and off-diagonal values, each with its own diagonal, right-hand side
+ * and solution, solved by one set of batched launches in which every SpMV
+ * reads the shared pattern once.  Each system keeps its own scalars and stops
+ * on its own test; stats[c] holds system c's results.  Solver: PBICGSTAB.
+ * Host form: ad[c] (NULL: sys->ad for every system), b[c], x[c] are arrays of
+ * neq doubles.  Device form: d_ad (or NULL), d_b, d_x hold nsys*neq doubles,
+ * system c at offset c*neq (the reference's b + c*neq layout). */
+int sfr_solve_shared(sfr_context *ctx, const sfr_system *sys, int nsys,
+                     const double *const *ad, const double *const *b,
+                     double *const *x, int solver, double tol, unsigned maxit,
+                     int newX, sfr_stats *stats);
+int sfr_solve_shared_device(sfr_context *ctx, sfr_matrix *A, int nsys,
+                            const double *d_ad, const double *d_b,
+                            double *d_x, int solver, double tol,
+                            unsigned maxit, int newX, sfr_stats *stats);
+
 /* Measurement hook: average device time of one launch of a loop kernel,
  * HIP events on the context stream.  which = 0: FP64 PCG SpMV + p.q;
  * which = 1: FP32 inner SpMV + p.q. */

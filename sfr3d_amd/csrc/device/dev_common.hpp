@@ -121,6 +121,37 @@ struct EllOp {
   }
 };
 
+// NB systems sharing the pattern and the off-diagonal values, each with its
+// own diagonal: x and ad interleaved per row (x[i*NB + c]).  One pass over
+// the row's columns and values serves all NB products.
+template <typename T, int S, bool UNIT, int NB>
+__device__ __forceinline__ void ell_row_multi(const EllOp<T, S, UNIT> &A,
+    int64_t i, bool valid, const T *__restrict__ ad, const T *__restrict__ x,
+    T (&acc)[NB])
+{
+#pragma unroll
+  for (int c = 0; c < NB; c++) acc[c] = T(0);
+  if (!valid) return;
+#pragma unroll
+  for (int c = 0; c < NB; c++) acc[c] = ad[i * NB + c] * x[i * NB + c];
+  if constexpr (S > 0) {
+#pragma unroll
+    for (int f = 0; f < S; f++) {
+      const T v = A.val[f * A.n + i];
+      const int64_t j = A.col[f * A.n + i];
+#pragma unroll
+      for (int c = 0; c < NB; c++) acc[c] += v * x[j * NB + c];
+    }
+  } else {
+    for (int f = 0; f < A.s; f++) {
+      const T v = A.val[f * A.n + i];
+      const int64_t j = A.col[f * A.n + i];
+#pragma unroll
+      for (int c = 0; c < NB; c++) acc[c] += v * x[j * NB + c];
+    }
+  }
+}
+
 // Full-row CSR (off-diagonals only, diagonal apart); L consecutive lanes per
 // row stride through the row and are combined with xor shuffles.
 template <typename T, int L_, bool UNIT>
@@ -146,6 +177,47 @@ struct CsrOp {
     return acc;
   }
 };
+
+template <typename T, int L, bool UNIT, int NB>
+__device__ __forceinline__ void csr_row_multi(const CsrOp<T, L, UNIT> &A,
+    int64_t i, bool valid, const T *__restrict__ ad, const T *__restrict__ x,
+    T (&acc)[NB])
+{
+#pragma unroll
+  for (int c = 0; c < NB; c++) acc[c] = T(0);
+  if (valid) {
+    const int sub = threadIdx.x & (L - 1);
+    const int e = A.ptr[i + 1];
+    for (int k = A.ptr[i] + sub; k < e; k += L) {
+      const T v = A.val[k];
+      const int64_t j = A.col[k];
+#pragma unroll
+      for (int c = 0; c < NB; c++) acc[c] += v * x[j * NB + c];
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < NB; c++)
+#pragma unroll
+    for (int off = L / 2; off > 0; off >>= 1) acc[c] += __shfl_xor(acc[c], off, L);
+  if (valid) {
+#pragma unroll
+    for (int c = 0; c < NB; c++) acc[c] += ad[i * NB + c] * x[i * NB + c];
+  }
+}
+
+template <typename T, int S, bool UNIT, int NB>
+__device__ __forceinline__ void row_multi(const EllOp<T, S, UNIT> &A, int64_t i,
+    bool valid, const T *__restrict__ ad, const T *__restrict__ x, T (&acc)[NB])
+{
+  ell_row_multi<T, S, UNIT, NB>(A, i, valid, ad, x, acc);
+}
+
+template <typename T, int L, bool UNIT, int NB>
+__device__ __forceinline__ void row_multi(const CsrOp<T, L, UNIT> &A, int64_t i,
+    bool valid, const T *__restrict__ ad, const T *__restrict__ x, T (&acc)[NB])
+{
+  csr_row_multi<T, L, UNIT, NB>(A, i, valid, ad, x, acc);
+}
 
 // Call f(op) with the FP64 operator matching the view.
 template <typename F>

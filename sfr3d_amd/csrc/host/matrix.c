@@ -80,7 +80,7 @@ int sfr_init(sfr_context **out, int device)
   INIT_TRY(hipMalloc((void **)&ctx->d_sc, sizeof(sfr_scalars) * SFR_MAXSYS));
   INIT_TRY(hipMemset(ctx->d_sc, 0, sizeof(sfr_scalars) * SFR_MAXSYS));
   INIT_TRY(hipHostMalloc((void **)&ctx->h_sc, sizeof(sfr_scalars) * SFR_MAXSYS, 0));
-  INIT_TRY(hipHostMalloc((void **)&ctx->h_poll, sizeof(unsigned) * 2 * 8 * SFR_MAXSYS, 0));
+  INIT_TRY(hipHostMalloc((void **)&ctx->h_poll, sizeof(sfr_scalars) * 2 * SFR_MAXSYS, 0));
   INIT_TRY(hipMalloc((void **)&ctx->part, sizeof(double) * SFR_MAXDOT * SFR_MAXG * SFR_MAXSYS));
   INIT_TRY(hipEventCreate(&ctx->ev_start));
   INIT_TRY(hipEventCreate(&ctx->ev_stop));
@@ -110,6 +110,7 @@ void sfr_finalize(sfr_context *ctx)
   if (ctx->cached) sfr_matrix_destroy(ctx, ctx->cached);
   for (int i = 0; i < 10; i++) hipFree(ctx->w64[i]);
   for (int i = 0; i < 6; i++) hipFree(ctx->w32[i]);
+  hipFree(ctx->stage);
   hipFree(ctx->d_sc);
   hipFree(ctx->part);
   if (ctx->h_sc) hipHostFree(ctx->h_sc);

@@ -29,7 +29,7 @@
 int sfr_workspace(sfr_context *ctx, int64_t n);
 int sfr_matrix_prepare_mixed(sfr_context *ctx, sfr_matrix *A);
 
-enum { G_PCG = 1, G_BICG = 2, G_MX = 3 };
+enum { G_PCG = 1, G_BICG = 2, G_MX = 3, G_BB = 4 };
 
 /* Capture `chunk` iterations of one loop body, or reuse a captured one. */
 static int get_graph(sfr_context *ctx, int kind, const void *k0,
@@ -62,8 +62,11 @@ static int get_graph(sfr_context *ctx, int kind, const void *k0,
     else if (kind == G_BICG)
       lrc = sfr_k_bicg_iter(s, v, g, (double *)k1, w[0], w[1], w[2], w[3], w[4], w[5],
                             ctx->d_sc, ctx->part);
-    else
+    else if (kind == G_MX)
       lrc = sfr_k_mx_inner_iter(s, v, g, w32[0], w32[1], w32[2], w32[3], ctx->d_sc, ctx->part);
+    else                    /* G_BB: k2 carries the batch size */
+      lrc = sfr_k_bb_iter(s, v, g, (int)(intptr_t)k2, w[8], w[9], w[0], w[1], w[2], w[3],
+                          w[4], w[5], w[6], ctx->d_sc, ctx->part);
   }
   hipGraph_t graph = NULL;
   const hipError_t erc = hipStreamEndCapture(s, &graph);
@@ -87,11 +90,12 @@ static int get_graph(sfr_context *ctx, int kind, const void *k0,
   return SFR_OK;
 }
 
-/* Replay the loop graph until the device reports the loop finished:
- * which = 0: sc->done set; which = 1: sc->done or sc->ru set.  Two chunks
- * are kept in flight. */
+/* Replay the loop graph until the device reports the loop finished for
+ * every one of the nsys systems: which = 0 waits for sc->done, which = 1 for
+ * sc->done or sc->ru (a mixed-precision fold request).  Two chunks are kept
+ * in flight; after each one the scalar blocks are copied to pinned memory. */
 static int run_loop(sfr_context *ctx, hipGraphExec_t ex, int which,
-                    unsigned max_chunks)
+                    unsigned max_chunks, int nsys)
 {
   hipStream_t s = ctx->stream;
   unsigned launched = 0, seen = 0;
@@ -99,8 +103,8 @@ static int run_loop(sfr_context *ctx, hipGraphExec_t ex, int which,
     while (launched - seen < 2 && launched < max_chunks) {
       const int slot = (int)(launched & 1);
       TRY(hipGraphLaunch(ex, s));
-      TRY(hipMemcpyAsync(ctx->h_poll + 8 * slot, &ctx->d_sc->iter, 8 * sizeof(unsigned),
-                         hipMemcpyDeviceToHost, s));
+      TRY(hipMemcpyAsync(ctx->h_poll + SFR_MAXSYS * slot, ctx->d_sc,
+                         sizeof(sfr_scalars) * (size_t)nsys, hipMemcpyDeviceToHost, s));
       TRY(hipEventRecord(ctx->ev_poll[slot], s));
       launched++;
     }
@@ -108,10 +112,11 @@ static int run_loop(sfr_context *ctx, hipGraphExec_t ex, int which,
     const int slot = (int)(seen & 1);
     TRY(hipEventSynchronize(ctx->ev_poll[slot]));
     seen++;
-    /* layout from &iter: iter, done, maxit, outer, ru, ... */
-    const unsigned *p = ctx->h_poll + 8 * slot;
-    const unsigned flag = which ? (p[1] | p[4]) : p[1];
-    if (flag) break;
+    const sfr_scalars *p = ctx->h_poll + SFR_MAXSYS * slot;
+    int finished = 1;
+    for (int c = 0; c < nsys; c++)
+      if (!(p[c].done || (which && p[c].ru))) finished = 0;
+    if (finished) break;
   }
   return SFR_OK;
 }
@@ -173,7 +178,7 @@ static int solve_pcg64(sfr_context *ctx, sfr_matrix *A, double tol,
   TRY(hipEventRecord(ctx->ev_start, ctx->stream));
   TRY(sfr_k_pcg_setup(ctx->stream, &v, g, d_b, d_x, ctx->w64[0], ctx->w64[1], newX, tol,
                       maxit, ctx->d_sc, ctx->part));
-  CHECK(run_loop(ctx, ex, 0, chunk_budget(ctx, maxit)));
+  CHECK(run_loop(ctx, ex, 0, chunk_budget(ctx, maxit), 1));
   return finish(ctx, &v, g, d_b, d_x, st);
 }
 
@@ -190,7 +195,7 @@ static int solve_bicg64(sfr_context *ctx, sfr_matrix *A, double tol,
   TRY(hipEventRecord(ctx->ev_start, ctx->stream));
   TRY(sfr_k_bicg_setup(ctx->stream, &v, g, d_b, d_x, w[0], w[1], w[2], w[3], newX, tol,
                        maxit, ctx->d_sc, ctx->part));
-  CHECK(run_loop(ctx, ex, 0, chunk_budget(ctx, maxit)));
+  CHECK(run_loop(ctx, ex, 0, chunk_budget(ctx, maxit), 1));
   return finish(ctx, &v, g, d_b, d_x, st);
 }
 
@@ -215,18 +220,123 @@ static int solve_pcg_mixed(sfr_context *ctx, sfr_matrix *A, double tol,
   /* every fold is triggered by a residual drop of delta (or the stop test):
    * bound the number of folds generously */
   for (int folds = 0; folds < 100000; folds++) {
-    CHECK(run_loop(ctx, ex, 1, chunk_budget(ctx, maxit)));
-    const unsigned *pl = NULL;
-    TRY(hipMemcpyAsync(ctx->h_poll, &ctx->d_sc->iter, 8 * sizeof(unsigned),
-                       hipMemcpyDeviceToHost, s));
+    CHECK(run_loop(ctx, ex, 1, chunk_budget(ctx, maxit), 1));
+    TRY(hipMemcpyAsync(ctx->h_poll, ctx->d_sc, sizeof(sfr_scalars), hipMemcpyDeviceToHost, s));
     TRY(hipStreamSynchronize(s));
-    pl = ctx->h_poll;
-    if (pl[1] != SFR_DONE_RUN) break;
-    if (!pl[4]) break;                    /* loop stopped with no request */
+    if (ctx->h_poll[0].done != SFR_DONE_RUN) break;
+    if (!ctx->h_poll[0].ru) break;        /* loop stopped with no request */
     TRY(sfr_k_mx_fold(s, &v, g, d_b, d_x, w32[0], w32[1], w32[2], 0, 0, ctx->d_sc,
                       ctx->part));
   }
   return finish(ctx, &v, g, d_b, d_x, st);
+}
+
+/* Shared-pattern batch: interleave the system-major inputs, run the batched
+ * BiCGSTAB, de-interleave x.  Workspace (interleaved, nsys*n each):
+ * w0 x, w1 r, w2 r0, w3 p, w4 z, w5 v, w6 t, w7 b, w8 ad, w9 dinv. */
+int sfr_solve_shared_device(sfr_context *ctx, sfr_matrix *A, int nsys,
+                            const double *d_ad, const double *d_b, double *d_x,
+                            int solver, double tol, unsigned maxit, int newX,
+                            sfr_stats *stats)
+{
+  if (!ctx || !A || !d_b || !d_x || nsys < 1 || nsys > 4) return SFR_BAD_ARG;
+  if (nsys == 1) {
+    /* one system: the single-system path, with its own diagonal */
+    if (d_ad) return sfr_set_error(ctx, SFR_NOT_SUPPORTED, "nsys = 1 with a separate diagonal");
+    return sfr_solve_device(ctx, A, solver, SFR_PREC_FP64, tol, maxit, d_b, d_x, newX, stats);
+  }
+  if (solver != SFR_PBICGSTAB)
+    return sfr_set_error(ctx, SFR_NOT_SUPPORTED, "shared-pattern batch: solver %d", solver);
+  if (maxit == 0) return sfr_set_error(ctx, SFR_BAD_ARG, "maxit = 0");
+  TRY(hipSetDevice(ctx->device));
+  const int64_t n = A->n, tot = n * nsys;
+  CHECK(sfr_workspace(ctx, tot));
+  sfr_mat_view v = sfr_view(A);
+  sfr_geom g = {A->rpt, 0};
+  double *const *w = ctx->w64;
+  hipStream_t s = ctx->stream;
+  hipGraphExec_t ex;
+  CHECK(get_graph(ctx, G_BB, A, w[0], (const void *)(intptr_t)nsys, &v, g, &ex));
+  TRY(hipEventRecord(ctx->ev_start, s));
+  if (d_ad) TRY(sfr_k_interleave(s, n, nsys, d_ad, w[8], w[9]));
+  else for (int c = 0; c < nsys; c++) {
+    /* the matrix's own diagonal for every system */
+    TRY(hipMemcpyAsync(ctx->w64[1] + c * n, A->ad, sizeof(double) * (size_t)n,
+                       hipMemcpyDeviceToDevice, s));
+  }
+  if (!d_ad) TRY(sfr_k_interleave(s, n, nsys, w[1], w[8], w[9]));
+  TRY(sfr_k_interleave(s, n, nsys, d_b, w[7], NULL));
+  if (!newX) TRY(sfr_k_interleave(s, n, nsys, d_x, w[0], NULL));
+  TRY(sfr_k_bb_setup(s, &v, g, nsys, w[8], w[9], w[7], w[0], w[1], w[2], w[3], w[4], newX,
+                     tol, maxit, ctx->d_sc, ctx->part));
+  CHECK(run_loop(ctx, ex, 0, chunk_budget(ctx, maxit), nsys));
+  TRY(sfr_k_bb_final(s, &v, g, nsys, w[8], w[9], w[7], w[0], ctx->d_sc, ctx->part));
+  TRY(sfr_k_deinterleave(s, n, nsys, w[0], d_x));
+  TRY(hipEventRecord(ctx->ev_stop, s));
+  TRY(hipMemcpyAsync(ctx->h_sc, ctx->d_sc, sizeof(sfr_scalars) * (size_t)nsys,
+                     hipMemcpyDeviceToHost, s));
+  TRY(hipStreamSynchronize(s));
+  float ms = 0.0f;
+  TRY(hipEventElapsedTime(&ms, ctx->ev_start, ctx->ev_stop));
+  int worst = SFR_OK;
+  for (int c = 0; c < nsys; c++) {
+    sfr_stats st;
+    fill_stats(&ctx->h_sc[c], &st, ms);
+    if (stats) stats[c] = st;
+    if (st.status != SFR_OK && worst == SFR_OK) worst = st.status;
+  }
+  return worst;
+}
+
+/* Host arrays: the reference's u/v/w call pattern (src/Simple.c:2980-3036) */
+int sfr_solve_shared(sfr_context *ctx, const sfr_system *sys, int nsys,
+                     const double *const *ad, const double *const *b,
+                     double *const *x, int solver, double tol, unsigned maxit,
+                     int newX, sfr_stats *stats)
+{
+  if (!ctx || !sys || !b || !x || nsys < 1 || nsys > 4) return SFR_BAD_ARG;
+  TRY(hipSetDevice(ctx->device));
+  sfr_matrix *A = ctx->cached;
+  if (A && (A->h_ia != sys->ia || A->h_ja != sys->ja || A->n != sys->neq ||
+            A->nad != sys->nad || A->storage != sys->storage || A->unsym != sys->unsym)) {
+    sfr_matrix_destroy(ctx, A);
+    A = NULL;
+  }
+  if (A) {
+    CHECK(sfr_matrix_set_values(ctx, A, sys->al, ad ? ad[0] : sys->ad, sys->au));
+  } else {
+    sfr_system s0 = *sys;
+    if (ad) s0.ad = ad[0];
+    CHECK(sfr_matrix_create(ctx, &s0, SFR_FMT_AUTO, &A));
+    ctx->cached = A;
+  }
+  const int64_t n = A->n, tot = n * nsys;
+  if (ctx->stage_n < 3 * tot) {
+    TRY(hipStreamSynchronize(ctx->stream));
+    hipFree(ctx->stage);
+    ctx->stage = NULL;
+    ctx->stage_n = 0;
+    TRY(hipMalloc((void **)&ctx->stage, sizeof(double) * (size_t)(3 * tot)));
+    ctx->stage_n = 3 * tot;
+  }
+  double *d_ad = ctx->stage, *d_b = ctx->stage + tot, *d_x = ctx->stage + 2 * tot;
+  const size_t bytes = sizeof(double) * (size_t)n;
+  for (int c = 0; c < nsys; c++) {
+    if (ad) TRY(hipMemcpyAsync(d_ad + c * n, ad[c], bytes, hipMemcpyHostToDevice, ctx->stream));
+    TRY(hipMemcpyAsync(d_b + c * n, b[c], bytes, hipMemcpyHostToDevice, ctx->stream));
+    if (!newX) TRY(hipMemcpyAsync(d_x + c * n, x[c], bytes, hipMemcpyHostToDevice, ctx->stream));
+  }
+  int rc;
+  if (nsys == 1)
+    rc = sfr_solve_device(ctx, A, solver, SFR_PREC_FP64, tol, maxit, d_b, d_x, newX, stats);
+  else
+    rc = sfr_solve_shared_device(ctx, A, nsys, ad ? d_ad : NULL, d_b, d_x, solver, tol, maxit,
+                                 newX, stats);
+  if (rc == SFR_HIP_ERROR || rc == SFR_BAD_ARG || rc == SFR_NOT_SUPPORTED) return rc;
+  for (int c = 0; c < nsys; c++)
+    TRY(hipMemcpyAsync(x[c], d_x + c * n, bytes, hipMemcpyDeviceToHost, ctx->stream));
+  TRY(hipStreamSynchronize(ctx->stream));
+  return rc;
 }
 
 int sfr_solve_device(sfr_context *ctx, sfr_matrix *A, int solver,

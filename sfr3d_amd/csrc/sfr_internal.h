@@ -95,7 +95,9 @@ struct sfr_context {
   sfr_scalars *d_sc, *h_sc;     /* SFR_MAXSYS each; h_sc is pinned        */
   double      *part;            /* SFR_MAXDOT * SFR_MAXG * SFR_MAXSYS      */
   hipEvent_t   ev_start, ev_stop, ev_poll[2];
-  unsigned    *h_poll;          /* pinned [2][4]: done, iter, inner_done   */
+  sfr_scalars *h_poll;          /* pinned [2][SFR_MAXSYS] status copies    */
+  double      *stage;           /* system-major staging of batched inputs  */
+  int64_t      stage_n;
   int64_t      ws_n;
   double      *w64[10];
   float       *w32[6];
@@ -157,6 +159,27 @@ int sfr_k_bicg_iter(hipStream_t s, const sfr_mat_view *A, sfr_geom g,
                     double *x, double *r, const double *r0, double *p,
                     double *z, double *v, double *t, sfr_scalars *sc,
                     double *part);
+
+/* shared-pattern batch of nb (2..4) BiCGSTAB systems, vectors interleaved
+ * per row (x[i*nb + c]), per-system diagonal ad/dinv interleaved alike,
+ * scalars sc[0..nb) */
+int sfr_k_bb_setup(hipStream_t s, const sfr_mat_view *A, sfr_geom g, int nb,
+                   const double *ad, const double *dinv, const double *b,
+                   double *x, double *r, double *r0, double *p, double *z,
+                   int newX, double tol, unsigned maxit, sfr_scalars *sc,
+                   double *part);
+int sfr_k_bb_iter(hipStream_t s, const sfr_mat_view *A, sfr_geom g, int nb,
+                  const double *ad, const double *dinv, double *x, double *r,
+                  const double *r0, double *p, double *z, double *v,
+                  double *t, sfr_scalars *sc, double *part);
+int sfr_k_bb_final(hipStream_t s, const sfr_mat_view *A, sfr_geom g, int nb,
+                   const double *ad, const double *dinv, const double *b,
+                   const double *x, sfr_scalars *sc, double *part);
+/* system-major src[c*n+i] -> interleaved dst[i*nb+c] (inv: 1/value, or NULL) */
+int sfr_k_interleave(hipStream_t s, int64_t n, int nb, const double *src,
+                     double *dst, double *inv);
+int sfr_k_deinterleave(hipStream_t s, int64_t n, int nb, const double *src,
+                       double *dst);
 
 /* mixed precision: FP32 CG on D^-1/2 A D^-1/2 with FP64 reliable updates */
 int sfr_k_mx_start(hipStream_t s, const double *part, int nparts, double tol,

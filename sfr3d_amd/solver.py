@@ -147,6 +147,30 @@ class Context:
         self._check(rc, allow=(L.MAXIT, L.BREAKDOWN))
         return x, st.as_dict()
 
+    def solve_shared(self, system, solver, tol, maxit, bs, ads=None, xs=None,
+                     newX=True):
+        """Shared-pattern batch (u/v/w): one system's pattern and
+        off-diagonals, per-system diagonals ads[c], right-hand sides bs[c].
+        Returns (list of x, list of stats dicts)."""
+        nsys = len(bs)
+        bs = [np.ascontiguousarray(b, np.float64) for b in bs]
+        xs = ([np.zeros(system.neq) for _ in range(nsys)] if xs is None
+              else [np.array(x, np.float64) for x in xs])
+        if ads is not None:
+            ads = [np.ascontiguousarray(a, np.float64) for a in ads]
+        P = C.c_void_p * nsys
+        pb = P(*[ptr(b) for b in bs])
+        px = P(*[ptr(x) for x in xs])
+        pad = P(*[ptr(a) for a in ads]) if ads is not None else None
+        st = (SfrStats * nsys)()
+        s = system.c()
+        rc = lib.sfr_solve_shared(self.h, C.byref(s), nsys,
+                                  C.cast(pad, C.c_void_p) if pad else None,
+                                  C.cast(pb, C.c_void_p), C.cast(px, C.c_void_p),
+                                  solver, tol, maxit, 1 if newX else 0, st)
+        self._check(rc, allow=(L.MAXIT, L.BREAKDOWN))
+        return xs, [st[c].as_dict() for c in range(nsys)]
+
     def solve_device(self, A, solver, tol, maxit, d_b, d_x, newX=True,
                      precision=L.PREC_FP64):
         st = SfrStats()
