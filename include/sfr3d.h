@@ -150,6 +150,22 @@ int sfr_solve_shared_device(sfr_context *ctx, sfr_matrix *A, int nsys,
                             double *d_x, int solver, double tol,
                             unsigned maxit, int newX, sfr_stats *stats);
 
+/* Adjoint of a solve x = A^-1 b, for the gradient of a loss L(x): solves
+ * A^T lambda = g with g = dL/dx (solver SFR_PCG when A is symmetric,
+ * A^T = A; SFR_PBICGSTAB runs on the transposed operator, built on first
+ * use), then
+ *   dL/db = lambda                          (d_lambda, neq)
+ *   dL/dal[k] = -sum lambda_i x_j over the entries (i, j) raw value k stands
+ *     for, in the caller's storage order ([al | au] for CSRC unsym, the
+ *     row-major ELLPACK array, ...); a value kept once for (i,j) and (j,i)
+ *     gets both terms                        (d_grad_al, NULL to skip)
+ *   dL/dad_i = -lambda_i x_i                 (d_grad_ad, neq, NULL to skip)
+ * d_x is the forward solution.  No counterpart in the reference. */
+int sfr_solve_adjoint(sfr_context *ctx, sfr_matrix *A, int solver, double tol,
+                      unsigned maxit, const double *d_x, const double *d_g,
+                      double *d_lambda, double *d_grad_al, double *d_grad_ad,
+                      sfr_stats *stats);
+
 /* Measurement hook: average device time of one launch of a loop kernel,
  * HIP events on the context stream.  which = 0: FP64 PCG SpMV + p.q;
  * which = 1: FP32 inner SpMV + p.q. */

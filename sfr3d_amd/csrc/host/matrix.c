@@ -296,10 +296,83 @@ void sfr_matrix_destroy(sfr_context *ctx, sfr_matrix *A)
     }
     ctx->ngraph = w;
   }
-  hipFree(A->col); hipFree(A->rowptr); hipFree(A->vmap); hipFree(A->dmap);
-  hipFree(A->raw); hipFree(A->raw_ad); hipFree(A->val); hipFree(A->ad);
-  hipFree(A->dinv); hipFree(A->dscale); hipFree(A->val32);
+  if (A->T) sfr_matrix_destroy(ctx, A->T);
+  hipFree(A->col); hipFree(A->rowptr); hipFree(A->vmap); hipFree(A->val);
+  hipFree(A->dscale); hipFree(A->val32); hipFree(A->grad_rc);
+  if (!A->borrowed) {
+    hipFree(A->dmap); hipFree(A->raw); hipFree(A->raw_ad); hipFree(A->ad);
+    hipFree(A->dinv);
+  }
   free(A);
+}
+
+/* Build the device format of A (n rows, entries rp/ent) and upload the
+ * index arrays.  fmt = SFR_FMT_ELL / SFR_FMT_CSR / SFR_FMT_AUTO. */
+static int upload_format(sfr_context *ctx, sfr_matrix *A, int32_t n,
+                         const int64_t *rp, const entry *ent, int fmt)
+{
+  int maxrow = 0;
+  for (int32_t i = 0; i < n; i++) {
+    const int64_t l = rp[i + 1] - rp[i];
+    if (l > maxrow) maxrow = (int)l;
+  }
+  if (fmt == SFR_FMT_AUTO)
+    fmt = (maxrow <= 16 && (int64_t)maxrow * n <= 2 * rp[n] + 16 * (int64_t)n)
+              ? SFR_FMT_ELL : SFR_FMT_CSR;
+  if (fmt != SFR_FMT_ELL && fmt != SFR_FMT_CSR)
+    return sfr_set_error(ctx, SFR_BAD_ARG, "format %d", fmt);
+  A->n = n;
+  A->fmt = fmt;
+  /* rows per thread: the smallest that keeps a streaming kernel at or below
+   * 1024 workgroups, so a consumer reduces at most 1024 partials */
+  A->rpt = 1;
+  while (A->rpt < 8 && (n + (int64_t)SFR_BLOCK * A->rpt - 1) / ((int64_t)SFR_BLOCK * A->rpt) > 1024)
+    A->rpt *= 2;
+  A->lanes = 4;
+  int32_t *hcol = NULL, *hmap = NULL, *hptr = NULL;
+  int rc = SFR_OK;
+  if (fmt == SFR_FMT_ELL) {
+    const int s = maxrow > 0 ? maxrow : 1;
+    A->slots = s;
+    A->nnz = (int64_t)s * n;
+    hcol = (int32_t *)malloc(sizeof(int32_t) * (size_t)A->nnz);
+    hmap = (int32_t *)malloc(sizeof(int32_t) * (size_t)A->nnz);
+    if (!hcol || !hmap) { rc = sfr_set_error(ctx, SFR_BAD_ARG, "host allocation"); goto done; }
+    for (int32_t i = 0; i < n; i++) {
+      const int64_t b = rp[i], l = rp[i + 1] - rp[i];
+      for (int f = 0; f < s; f++) {
+        const int64_t k = (int64_t)f * n + i;
+        if (f < l) { hcol[k] = ent[b + f].col; hmap[k] = ent[b + f].raw; }
+        else { hcol[k] = i; hmap[k] = -1; }
+      }
+    }
+  } else {
+    A->slots = maxrow;
+    A->nnz = rp[n];
+    hcol = (int32_t *)malloc(sizeof(int32_t) * (size_t)(A->nnz + 1));
+    hmap = (int32_t *)malloc(sizeof(int32_t) * (size_t)(A->nnz + 1));
+    hptr = (int32_t *)malloc(sizeof(int32_t) * ((size_t)n + 1));
+    if (!hcol || !hmap || !hptr) { rc = sfr_set_error(ctx, SFR_BAD_ARG, "host allocation"); goto done; }
+    for (int32_t i = 0; i <= n; i++) hptr[i] = (int32_t)rp[i];
+    for (int64_t k = 0; k < A->nnz; k++) { hcol[k] = ent[k].col; hmap[k] = ent[k].raw; }
+    A->lanes = maxrow <= 4 ? 2 : (maxrow <= 12 ? 4 : 8);
+  }
+#define MTRY(call) do { hipError_t e_ = (call); if (e_ != hipSuccess) { \
+    rc = sfr_set_error(ctx, SFR_HIP_ERROR, "%s: %s", #call, hipGetErrorString(e_)); goto done; } } while (0)
+  MTRY(hipMalloc((void **)&A->col, sizeof(int32_t) * (size_t)(A->nnz + 1)));
+  MTRY(hipMalloc((void **)&A->vmap, sizeof(int32_t) * (size_t)(A->nnz + 1)));
+  MTRY(hipMalloc((void **)&A->val, sizeof(double) * (size_t)(A->nnz + 1)));
+  MTRY(hipMemcpyAsync(A->col, hcol, sizeof(int32_t) * (size_t)A->nnz, hipMemcpyHostToDevice, ctx->stream));
+  MTRY(hipMemcpyAsync(A->vmap, hmap, sizeof(int32_t) * (size_t)A->nnz, hipMemcpyHostToDevice, ctx->stream));
+  if (hptr) {
+    MTRY(hipMalloc((void **)&A->rowptr, sizeof(int32_t) * ((size_t)n + 1)));
+    MTRY(hipMemcpyAsync(A->rowptr, hptr, sizeof(int32_t) * ((size_t)n + 1), hipMemcpyHostToDevice, ctx->stream));
+  }
+  MTRY(hipStreamSynchronize(ctx->stream));
+#undef MTRY
+done:
+  free(hcol); free(hmap); free(hptr);
+  return rc;
 }
 
 int sfr_matrix_create(sfr_context *ctx, const sfr_system *sys, int format,
@@ -315,6 +388,8 @@ int sfr_matrix_create(sfr_context *ctx, const sfr_system *sys, int format,
                          sys->storage);
   if (sys->storage == SFR_ELLPACK && (sys->ia[0] < 0 || sys->ia[0] > 64))
     return sfr_set_error(ctx, SFR_BAD_ARG, "ELLPACK width %d", sys->ia[0]);
+  if (format != SFR_FMT_AUTO && format != SFR_FMT_ELL && format != SFR_FMT_CSR)
+    return sfr_set_error(ctx, SFR_BAD_ARG, "format %d", format);
   int64_t *rp = NULL; entry *ent = NULL; int32_t *dmap = NULL; int64_t nraw = 0;
   const int frc = full_rows(sys, &rp, &ent, &dmap, &nraw);
   if (frc == -2)
@@ -323,85 +398,121 @@ int sfr_matrix_create(sfr_context *ctx, const sfr_system *sys, int format,
   if (frc == -3) return sfr_set_error(ctx, SFR_BAD_ARG, "CSR row without diagonal");
   if (frc) return sfr_set_error(ctx, SFR_BAD_ARG, "host allocation failed");
   const int32_t n = sys->neq;
-  int maxrow = 0;
-  for (int32_t i = 0; i < n; i++) {
-    const int64_t l = rp[i + 1] - rp[i];
-    if (l > maxrow) maxrow = (int)l;
-  }
-  int fmt = format;
-  if (fmt == SFR_FMT_AUTO)
-    fmt = (maxrow <= 16 && (int64_t)maxrow * n <= 2 * rp[n] + 16 * (int64_t)n)
-              ? SFR_FMT_ELL : SFR_FMT_CSR;
-  if (fmt != SFR_FMT_ELL && fmt != SFR_FMT_CSR) {
-    free(rp); free(ent); free(dmap);
-    return sfr_set_error(ctx, SFR_BAD_ARG, "format %d", format);
-  }
   sfr_matrix *A = (sfr_matrix *)calloc(1, sizeof *A);
   if (!A) { free(rp); free(ent); free(dmap); return SFR_BAD_ARG; }
-  A->n = n; A->fmt = fmt; A->storage = sys->storage; A->unsym = sys->unsym;
+  A->storage = sys->storage; A->unsym = sys->unsym;
   A->nad = sys->nad; A->nraw = nraw; A->diag_in_al = sys->storage == SFR_CSR;
   A->h_ia = sys->ia; A->h_ja = sys->ja;
-  /* rows per thread: the smallest that keeps a streaming kernel at or below
-   * 1024 workgroups, so a consumer reduces at most 1024 partials */
-  A->rpt = 1;
-  while (A->rpt < 8 && (n + (int64_t)SFR_BLOCK * A->rpt - 1) / ((int64_t)SFR_BLOCK * A->rpt) > 1024)
-    A->rpt *= 2;
-  A->lanes = 4;
-  int32_t *hcol = NULL, *hmap = NULL, *hptr = NULL;
-  int rc = SFR_OK;
-  if (fmt == SFR_FMT_ELL) {
-    const int s = maxrow > 0 ? maxrow : 1;
-    A->slots = s;
-    A->nnz = (int64_t)s * n;
-    hcol = (int32_t *)malloc(sizeof(int32_t) * (size_t)A->nnz);
-    hmap = (int32_t *)malloc(sizeof(int32_t) * (size_t)A->nnz);
-    if (!hcol || !hmap) { rc = SFR_BAD_ARG; goto done; }
-    for (int32_t i = 0; i < n; i++) {
-      const int64_t b = rp[i], l = rp[i + 1] - rp[i];
-      for (int f = 0; f < s; f++) {
-        const int64_t k = (int64_t)f * n + i;
-        if (f < l) { hcol[k] = ent[b + f].col; hmap[k] = ent[b + f].raw; }
-        else { hcol[k] = i; hmap[k] = -1; }
-      }
-    }
-  } else {
-    A->slots = maxrow;
-    A->nnz = rp[n];
-    hcol = (int32_t *)malloc(sizeof(int32_t) * (size_t)(A->nnz + 1));
-    hmap = (int32_t *)malloc(sizeof(int32_t) * (size_t)(A->nnz + 1));
-    hptr = (int32_t *)malloc(sizeof(int32_t) * ((size_t)n + 1));
-    if (!hcol || !hmap || !hptr) { rc = SFR_BAD_ARG; goto done; }
-    for (int32_t i = 0; i <= n; i++) hptr[i] = (int32_t)rp[i];
-    for (int64_t k = 0; k < A->nnz; k++) { hcol[k] = ent[k].col; hmap[k] = ent[k].raw; }
-    A->lanes = maxrow <= 4 ? 2 : (maxrow <= 12 ? 4 : 8);
-  }
+  int rc = upload_format(ctx, A, n, rp, ent, format);
+  if (rc == SFR_OK) {
 #define MTRY(call) do { hipError_t e_ = (call); if (e_ != hipSuccess) { \
     rc = sfr_set_error(ctx, SFR_HIP_ERROR, "%s: %s", #call, hipGetErrorString(e_)); goto done; } } while (0)
-  MTRY(hipMalloc((void **)&A->col, sizeof(int32_t) * (size_t)(A->nnz + 1)));
-  MTRY(hipMalloc((void **)&A->vmap, sizeof(int32_t) * (size_t)(A->nnz + 1)));
-  MTRY(hipMalloc((void **)&A->val, sizeof(double) * (size_t)(A->nnz + 1)));
-  MTRY(hipMalloc((void **)&A->ad, sizeof(double) * (size_t)n));
-  MTRY(hipMalloc((void **)&A->dinv, sizeof(double) * (size_t)n));
-  MTRY(hipMalloc((void **)&A->raw, sizeof(double) * (size_t)(nraw + 1)));
-  MTRY(hipMalloc((void **)&A->raw_ad, sizeof(double) * (size_t)n));
-  MTRY(hipMemcpyAsync(A->col, hcol, sizeof(int32_t) * (size_t)A->nnz, hipMemcpyHostToDevice, ctx->stream));
-  MTRY(hipMemcpyAsync(A->vmap, hmap, sizeof(int32_t) * (size_t)A->nnz, hipMemcpyHostToDevice, ctx->stream));
-  if (hptr) {
-    MTRY(hipMalloc((void **)&A->rowptr, sizeof(int32_t) * ((size_t)n + 1)));
-    MTRY(hipMemcpyAsync(A->rowptr, hptr, sizeof(int32_t) * ((size_t)n + 1), hipMemcpyHostToDevice, ctx->stream));
-  }
-  if (dmap) {
-    MTRY(hipMalloc((void **)&A->dmap, sizeof(int32_t) * (size_t)n));
-    MTRY(hipMemcpyAsync(A->dmap, dmap, sizeof(int32_t) * (size_t)n, hipMemcpyHostToDevice, ctx->stream));
-  }
-  MTRY(hipStreamSynchronize(ctx->stream));
+    MTRY(hipMalloc((void **)&A->ad, sizeof(double) * (size_t)n));
+    MTRY(hipMalloc((void **)&A->dinv, sizeof(double) * (size_t)n));
+    MTRY(hipMalloc((void **)&A->raw, sizeof(double) * (size_t)(nraw + 1)));
+    MTRY(hipMalloc((void **)&A->raw_ad, sizeof(double) * (size_t)n));
+    if (dmap) {
+      MTRY(hipMalloc((void **)&A->dmap, sizeof(int32_t) * (size_t)n));
+      MTRY(hipMemcpyAsync(A->dmap, dmap, sizeof(int32_t) * (size_t)n, hipMemcpyHostToDevice, ctx->stream));
+      MTRY(hipStreamSynchronize(ctx->stream));
+    }
 #undef MTRY
-  rc = sfr_matrix_set_values(ctx, A, sys->al, sys->ad, sys->au);
+    rc = sfr_matrix_set_values(ctx, A, sys->al, sys->ad, sys->au);
+  }
 done:
-  free(rp); free(ent); free(dmap); free(hcol); free(hmap); free(hptr);
+  free(rp); free(ent); free(dmap);
   if (rc != SFR_OK) { sfr_matrix_destroy(ctx, A); return rc; }
   *out = A;
   return SFR_OK;
+}
+
+/* Adjoint support, built on first use: the transposed operator (same raw
+ * values, entries (j, i)) and, for each raw value, the matrix entries it
+ * feeds.  The device index arrays are read back, so nothing of the host
+ * system has to be kept. */
+int sfr_matrix_prepare_adjoint(sfr_context *ctx, sfr_matrix *A)
+{
+  if (A->have_grad) return SFR_OK;
+  const int32_t n = A->n;
+  const int64_t nnz = A->nnz;
+  int32_t *col = (int32_t *)malloc(sizeof(int32_t) * (size_t)(nnz + 1));
+  int32_t *map = (int32_t *)malloc(sizeof(int32_t) * (size_t)(nnz + 1));
+  int32_t *ptr = A->rowptr ? (int32_t *)malloc(sizeof(int32_t) * ((size_t)n + 1)) : NULL;
+  int32_t *dmap = A->dmap ? (int32_t *)malloc(sizeof(int32_t) * (size_t)n) : NULL;
+  int32_t *rc4 = (int32_t *)malloc(sizeof(int32_t) * 4 * (size_t)(A->nraw + 1));
+  int64_t *cnt = (int64_t *)calloc((size_t)n + 1, sizeof(int64_t));
+  entry *tent = NULL;
+  int rc = SFR_OK;
+  if (!col || !map || !rc4 || !cnt || (A->rowptr && !ptr) || (A->dmap && !dmap)) {
+    rc = sfr_set_error(ctx, SFR_BAD_ARG, "host allocation");
+    goto out;
+  }
+#define ATRY(call) do { hipError_t e_ = (call); if (e_ != hipSuccess) { \
+    rc = sfr_set_error(ctx, SFR_HIP_ERROR, "%s: %s", #call, hipGetErrorString(e_)); goto out; } } while (0)
+  ATRY(hipMemcpy(col, A->col, sizeof(int32_t) * (size_t)nnz, hipMemcpyDeviceToHost));
+  ATRY(hipMemcpy(map, A->vmap, sizeof(int32_t) * (size_t)nnz, hipMemcpyDeviceToHost));
+  if (ptr) ATRY(hipMemcpy(ptr, A->rowptr, sizeof(int32_t) * ((size_t)n + 1), hipMemcpyDeviceToHost));
+  if (dmap) ATRY(hipMemcpy(dmap, A->dmap, sizeof(int32_t) * (size_t)n, hipMemcpyDeviceToHost));
+  for (int64_t k = 0; k < 4 * (A->nraw + 1); k++) rc4[k] = -1;
+  /* entries (i, j, raw) of the device format */
+  for (int32_t i = 0; i < n; i++) {
+    const int64_t b = ptr ? ptr[i] : 0, e = ptr ? ptr[i + 1] : A->slots;
+    for (int64_t q = b; q < e; q++) {
+      const int64_t k = ptr ? q : q * n + i;
+      const int32_t raw = map[k];
+      if (raw < 0) continue;
+      int32_t *slot = rc4 + 4 * (int64_t)raw;
+      if (slot[0] < 0) { slot[0] = i; slot[1] = col[k]; }
+      else if (slot[2] < 0) { slot[2] = i; slot[3] = col[k]; }
+      else { rc = sfr_set_error(ctx, SFR_NOT_SUPPORTED, "raw value feeds > 2 entries"); goto out; }
+      cnt[col[k] + 1]++;
+    }
+  }
+  if (dmap)                             /* storage CSR: diagonal inside al */
+    for (int32_t i = 0; i < n; i++) {
+      int32_t *slot = rc4 + 4 * (int64_t)dmap[i];
+      slot[0] = i; slot[1] = i;
+    }
+  for (int32_t i = 0; i < n; i++) cnt[i + 1] += cnt[i];
+  tent = (entry *)malloc(sizeof(entry) * (size_t)(cnt[n] + 1));
+  if (!tent) { rc = sfr_set_error(ctx, SFR_BAD_ARG, "host allocation"); goto out; }
+  {
+    int64_t *pos = (int64_t *)malloc(sizeof(int64_t) * (size_t)n);
+    if (!pos) { rc = sfr_set_error(ctx, SFR_BAD_ARG, "host allocation"); goto out; }
+    for (int32_t i = 0; i < n; i++) pos[i] = cnt[i];
+    for (int32_t i = 0; i < n; i++) {
+      const int64_t b = ptr ? ptr[i] : 0, e = ptr ? ptr[i + 1] : A->slots;
+      for (int64_t q = b; q < e; q++) {
+        const int64_t k = ptr ? q : q * n + i;
+        if (map[k] < 0) continue;
+        tent[pos[col[k]]++] = (entry){i, map[k]};   /* row col[k] of A^T */
+      }
+    }
+    free(pos);
+  }
+  for (int32_t i = 0; i < n; i++)
+    qsort(tent + cnt[i], (size_t)(cnt[i + 1] - cnt[i]), sizeof(entry), cmp_entry);
+  {
+    sfr_matrix *T = (sfr_matrix *)calloc(1, sizeof *T);
+    if (!T) { rc = sfr_set_error(ctx, SFR_BAD_ARG, "host allocation"); goto out; }
+    T->storage = A->storage; T->unsym = A->unsym; T->nad = A->nad;
+    T->nraw = A->nraw; T->diag_in_al = A->diag_in_al; T->borrowed = 1;
+    rc = upload_format(ctx, T, n, cnt, tent, A->fmt);
+    if (rc != SFR_OK) { sfr_matrix_destroy(ctx, T); goto out; }
+    T->raw = A->raw; T->raw_ad = A->raw_ad; T->ad = A->ad; T->dinv = A->dinv;
+    T->dmap = A->dmap;
+    ATRY(sfr_k_gather(ctx->stream, T->nnz, T->vmap, T->raw, T->val));
+    A->T = T;
+  }
+  ATRY(hipMalloc((void **)&A->grad_rc, sizeof(int32_t) * 4 * (size_t)(A->nraw + 1)));
+  ATRY(hipMemcpyAsync(A->grad_rc, rc4, sizeof(int32_t) * 4 * (size_t)(A->nraw + 1),
+                      hipMemcpyHostToDevice, ctx->stream));
+  ATRY(hipStreamSynchronize(ctx->stream));
+#undef ATRY
+  A->have_grad = 1;
+out:
+  free(col); free(map); free(ptr); free(dmap); free(rc4); free(cnt); free(tent);
+  return rc;
 }
 
 int sfr_matrix_set_values(sfr_context *ctx, sfr_matrix *A, const double *al,
@@ -421,6 +532,7 @@ int sfr_matrix_set_values(sfr_context *ctx, sfr_matrix *A, const double *al,
   if (!A->diag_in_al)
     TRY(hipMemcpyAsync(A->raw_ad, ad, sizeof(double) * (size_t)A->n, hipMemcpyHostToDevice, ctx->stream));
   TRY(sfr_k_gather(ctx->stream, A->nnz, A->vmap, A->raw, A->val));
+  if (A->T) TRY(sfr_k_gather(ctx->stream, A->T->nnz, A->T->vmap, A->raw, A->T->val));
   TRY(sfr_k_diag(ctx->stream, A->n, A->diag_in_al ? A->dmap : NULL, A->raw, A->raw_ad, A->ad, A->dinv));
   TRY(hipStreamSynchronize(ctx->stream));
   A->have32 = 0;

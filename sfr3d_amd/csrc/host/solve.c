@@ -28,6 +28,7 @@
 
 int sfr_workspace(sfr_context *ctx, int64_t n);
 int sfr_matrix_prepare_mixed(sfr_context *ctx, sfr_matrix *A);
+int sfr_matrix_prepare_adjoint(sfr_context *ctx, sfr_matrix *A);
 
 enum { G_PCG = 1, G_BICG = 2, G_MX = 3, G_BB = 4 };
 
@@ -388,6 +389,29 @@ int sfr_kernel_time(sfr_context *ctx, sfr_matrix *A, int which, int reps,
   TRY(hipEventElapsedTime(&ms, ctx->ev_start, ctx->ev_stop));
   *ms_per_launch = ms / reps;
   return SFR_OK;
+}
+
+/* Adjoint of x = A^-1 b: A^T lambda = g, then the gradients in the caller's
+ * raw storage order.  solver PCG: A is symmetric (A^T = A); PBICGSTAB:
+ * BiCGSTAB on the transposed operator. */
+int sfr_solve_adjoint(sfr_context *ctx, sfr_matrix *A, int solver, double tol,
+                      unsigned maxit, const double *d_x, const double *d_g,
+                      double *d_lambda, double *d_grad_al, double *d_grad_ad,
+                      sfr_stats *st)
+{
+  if (!ctx || !A || !d_x || !d_g || !d_lambda) return SFR_BAD_ARG;
+  if (solver != SFR_PCG && solver != SFR_PBICGSTAB)
+    return sfr_set_error(ctx, SFR_BAD_ARG, "solver id %d", solver);
+  TRY(hipSetDevice(ctx->device));
+  CHECK(sfr_matrix_prepare_adjoint(ctx, A));
+  const int rc = solver == SFR_PCG
+      ? sfr_solve_device(ctx, A, SFR_PCG, SFR_PREC_FP64, tol, maxit, d_g, d_lambda, 1, st)
+      : sfr_solve_device(ctx, A->T, SFR_PBICGSTAB, SFR_PREC_FP64, tol, maxit, d_g, d_lambda, 1, st);
+  if (rc == SFR_HIP_ERROR || rc == SFR_BAD_ARG || rc == SFR_NOT_SUPPORTED) return rc;
+  TRY(sfr_k_adj_grad(ctx->stream, d_grad_al ? A->nraw : 0, A->grad_rc, d_lambda, d_x,
+                     d_grad_al, d_grad_ad ? A->n : 0, d_grad_ad));
+  TRY(hipStreamSynchronize(ctx->stream));
+  return rc;
 }
 
 /* Host-pointer solve: the pattern is uploaded once and recognised again by

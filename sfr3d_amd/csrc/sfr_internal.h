@@ -80,6 +80,11 @@ struct sfr_matrix {
   double  *raw, *raw_ad, *val, *ad, *dinv, *dscale;
   float   *val32;
   const int32_t *h_ia, *h_ja;   /* host pattern identity (value-only uploads) */
+  /* adjoint support, built on first use */
+  struct sfr_matrix *T;  /* transposed operator (shares raw, ad, dinv)     */
+  int      borrowed;     /* raw/ad/dinv belong to another matrix           */
+  int32_t *grad_rc;      /* nraw x 4: (row, col) of up to two entries fed  */
+  int      have_grad;    /*   by each raw value, -1 = none                 */
 };
 
 typedef struct sfr_graph {
@@ -180,6 +185,12 @@ int sfr_k_interleave(hipStream_t s, int64_t n, int nb, const double *src,
                      double *dst, double *inv);
 int sfr_k_deinterleave(hipStream_t s, int64_t n, int nb, const double *src,
                        double *dst);
+
+/* adjoint: dL/d raw[k] = -(lam[r0] x[c0] + lam[r1] x[c1]) over the entries
+ * raw value k feeds; dL/d ad_i = -lam_i x_i */
+int sfr_k_adj_grad(hipStream_t s, int64_t nraw, const int32_t *rc,
+                   const double *lam, const double *x, double *grad_raw,
+                   int64_t n, double *grad_ad);
 
 /* mixed precision: FP32 CG on D^-1/2 A D^-1/2 with FP64 reliable updates */
 int sfr_k_mx_start(hipStream_t s, const double *part, int nparts, double tol,

@@ -171,6 +171,18 @@ class Context:
         self._check(rc, allow=(L.MAXIT, L.BREAKDOWN))
         return xs, [st[c].as_dict() for c in range(nsys)]
 
+    def solve_adjoint(self, A, solver, tol, maxit, d_x, d_g, d_lam,
+                      d_grad_al=None, d_grad_ad=None):
+        """A^T lambda = g and the value gradients (sfr_solve_adjoint)."""
+        st = SfrStats()
+        rc = lib.sfr_solve_adjoint(self.h, A.h, solver, tol, maxit, d_x.ptr,
+                                   d_g.ptr, d_lam.ptr,
+                                   d_grad_al.ptr if d_grad_al else None,
+                                   d_grad_ad.ptr if d_grad_ad else None,
+                                   C.byref(st))
+        self._check(rc, allow=(L.MAXIT, L.BREAKDOWN))
+        return st.as_dict()
+
     def solve_device(self, A, solver, tol, maxit, d_b, d_x, newX=True,
                      precision=L.PREC_FP64):
         st = SfrStats()
