@@ -166,6 +166,55 @@ int sfr_solve_adjoint(sfr_context *ctx, sfr_matrix *A, int solver, double tol,
                       double *d_lambda, double *d_grad_al, double *d_grad_ad,
                       sfr_stats *stats);
 
+/* ---- uneven batch ----------------------------------------------------------
+ * Many independent SPD systems of any sizes, patterns and storages (up to
+ * 1024), solved together by mixed-precision Jacobi-PCG (FP32 inner CG with
+ * FP64 reliable updates, each system stopping on the reference PCG's own
+ * test, src/Cg.c:73-78,141-142): one fixed set of launches per iteration
+ * serves every system still running, replayed from one captured graph.  The
+ * reference solves such systems one solverC call at a time; this entry has
+ * no counterpart there.
+ *
+ * sfr_batch_create uploads the systems.  tune = 1 runs the autotuner: the
+ * systems are grouped in size buckets (2^k <= neq < 2^(k+1)), and for each
+ * bucket the device format (ELL or CSR) and the work-item size (rows per
+ * thread) are chosen by timing the batch's own iteration kernels on up to
+ * four of the bucket's systems.  tune = 0 takes the format heuristic of
+ * sfr_matrix_create and a size rule.
+ * stats[s] (nsys entries) holds system s's results; its time_s is the time
+ * from the start of the batch solve to the moment system s converged,
+ * measured on the device clock. */
+typedef struct sfr_batch sfr_batch;
+
+typedef struct sfr_bucket_choice {
+  int    bucket;       /* k: systems with 2^k <= neq < 2^(k+1)              */
+  int    nsys;         /* systems in the bucket                             */
+  int    format;       /* SFR_FMT_ELL / SFR_FMT_CSR                         */
+  int    rpt;          /* rows per thread (work item = rpt * 256 rows)      */
+  double ns_per_row;   /* measured FP32 iteration time per row (0: untuned) */
+} sfr_bucket_choice;
+
+int  sfr_batch_create(sfr_context *ctx, int nsys, const sfr_system *sys,
+                      int tune, sfr_batch **B);
+int  sfr_batch_set_values(sfr_context *ctx, sfr_batch *B, int s,
+                          const double *al, const double *ad,
+                          const double *au);
+/* host form: b[s], x[s] are arrays of system s's neq doubles */
+int  sfr_batch_solve(sfr_context *ctx, sfr_batch *B, double tol,
+                     unsigned maxit, const double *const *b, double *const *x,
+                     int newX, sfr_stats *stats);
+/* device form: d_b, d_x hold every system's vector, system s at the row
+ * offset sfr_batch_info reports (total rows: sfr_batch_rows) */
+int  sfr_batch_solve_device(sfr_context *ctx, sfr_batch *B, double tol,
+                            unsigned maxit, const double *d_b, double *d_x,
+                            int newX, sfr_stats *stats);
+int64_t sfr_batch_rows(const sfr_batch *B);
+int  sfr_batch_info(const sfr_batch *B, int s, int64_t *row0, int32_t *neq,
+                    int *format, int *rpt);
+/* tuner choices, one per non-empty bucket; returns how many were written */
+int  sfr_batch_buckets(const sfr_batch *B, sfr_bucket_choice *out, int max);
+void sfr_batch_destroy(sfr_context *ctx, sfr_batch *B);
+
 /* Measurement hook: average device time of one launch of a loop kernel,
  * HIP events on the context stream.  which = 0: FP64 PCG SpMV + p.q;
  * which = 1: FP32 inner SpMV + p.q. */

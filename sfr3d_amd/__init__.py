@@ -7,10 +7,12 @@ include/sfr3d_setup.h) that keeps the reference's solverC contract.
 from . import _lib
 from ._lib import (PCG, PBICGSTAB, CSR, CSRD, CSRC, ELLPACK, CSRDCOO, CSRCCOO,
                    PREC_FP64, PREC_MIXED, FMT_AUTO, FMT_ELL, FMT_CSR, LIB_PATH)
-from .solver import Context, System, DeviceMatrix, DeviceArray, SolverError
+from .solver import (Context, System, DeviceMatrix, DeviceArray, Batch,
+                     SolverError)
 from . import structure, systems
 
-__all__ = ["Context", "System", "DeviceMatrix", "DeviceArray", "SolverError",
+__all__ = ["Context", "System", "DeviceMatrix", "DeviceArray", "Batch",
+           "SolverError",
            "structure", "systems", "PCG", "PBICGSTAB", "CSR", "CSRD", "CSRC",
            "ELLPACK", "CSRDCOO", "CSRCCOO", "PREC_FP64", "PREC_MIXED",
            "FMT_AUTO", "FMT_ELL", "FMT_CSR", "LIB_PATH"]

@@ -52,6 +52,14 @@ class SfrStats(C.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class SfrBucketChoice(C.Structure):
+    _fields_ = [("bucket", C.c_int), ("nsys", C.c_int), ("format", C.c_int),
+                ("rpt", C.c_int), ("ns_per_row", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 class SfrPattern(C.Structure):
     _fields_ = [("neq", C.c_int32), ("nad", C.c_int32), ("nia", C.c_int32),
                 ("nja", C.c_int64), ("nval", C.c_int64),
@@ -99,6 +107,19 @@ _sig("sfr_solve_adjoint", C.c_int, vp, vp, C.c_int, C.c_double, C.c_uint, vp,
 _sig("sfr_kernel_time", C.c_int, vp, vp, C.c_int, C.c_int,
      C.POINTER(C.c_double))
 _sig("sfr_set_option", C.c_int, vp, C.c_char_p, C.c_double)
+# uneven batch
+_sig("sfr_batch_create", C.c_int, vp, C.c_int, C.POINTER(SfrSystem), C.c_int,
+     C.POINTER(vp))
+_sig("sfr_batch_set_values", C.c_int, vp, vp, C.c_int, vp, vp, vp)
+_sig("sfr_batch_solve", C.c_int, vp, vp, C.c_double, C.c_uint, vp, vp,
+     C.c_int, C.POINTER(SfrStats))
+_sig("sfr_batch_solve_device", C.c_int, vp, vp, C.c_double, C.c_uint, vp, vp,
+     C.c_int, C.POINTER(SfrStats))
+_sig("sfr_batch_rows", C.c_int64, vp)
+_sig("sfr_batch_info", C.c_int, vp, C.c_int, C.POINTER(C.c_int64),
+     C.POINTER(C.c_int32), C.POINTER(C.c_int), C.POINTER(C.c_int))
+_sig("sfr_batch_buckets", C.c_int, vp, C.POINTER(SfrBucketChoice), C.c_int)
+_sig("sfr_batch_destroy", None, vp, vp)
 _sig("sfr_solverC", None, vp, C.c_int, C.c_int, C.c_int, C.c_int, vp, vp, vp,
      vp, vp, vp, vp, vp, vp, C.c_double, C.c_uint, C.c_short, C.c_short, vp,
      C.c_bool, C.c_bool, C.c_bool, C.c_char_p)

@@ -122,7 +122,7 @@ static int run_loop(sfr_context *ctx, hipGraphExec_t ex, int which,
   return SFR_OK;
 }
 
-static void fill_stats(const sfr_scalars *sc, sfr_stats *st, float ms)
+void sfr_fill_stats(const sfr_scalars *sc, sfr_stats *st, float ms)
 {
   if (!st) return;
   memset(st, 0, sizeof *st);
@@ -156,7 +156,7 @@ static int finish(sfr_context *ctx, const sfr_mat_view *v, sfr_geom g,
   float ms = 0.0f;
   TRY(hipEventElapsedTime(&ms, ctx->ev_start, ctx->ev_stop));
   sfr_stats tmp;
-  fill_stats(ctx->h_sc, &tmp, ms);
+  sfr_fill_stats(ctx->h_sc, &tmp, ms);
   if (st) *st = tmp;
   return tmp.status;
 }
@@ -282,7 +282,7 @@ int sfr_solve_shared_device(sfr_context *ctx, sfr_matrix *A, int nsys,
   int worst = SFR_OK;
   for (int c = 0; c < nsys; c++) {
     sfr_stats st;
-    fill_stats(&ctx->h_sc[c], &st, ms);
+    sfr_fill_stats(&ctx->h_sc[c], &st, ms);
     if (stats) stats[c] = st;
     if (st.status != SFR_OK && worst == SFR_OK) worst = st.status;
   }

@@ -114,6 +114,9 @@ struct sfr_context {
 };
 
 int sfr_set_error(sfr_context *ctx, int code, const char *fmt, ...);
+/* solve statistics from a scalar block (ms = device time of the solve) */
+void sfr_fill_stats(const sfr_scalars *sc, sfr_stats *st, float ms);
+int  sfr_matrix_prepare_mixed(sfr_context *ctx, sfr_matrix *A);
 sfr_mat_view sfr_view(const sfr_matrix *A);
 sfr_geom     sfr_geom_for(const sfr_context *ctx, int64_t n, int rpt);
 
@@ -203,6 +206,42 @@ int sfr_k_mx_fold(hipStream_t s, const sfr_mat_view *A, sfr_geom g,
 int sfr_k_mx_inner_iter(hipStream_t s, const sfr_mat_view *A, sfr_geom g,
                         float *y, float *r, float *p, float *q,
                         sfr_scalars *sc, double *part);
+
+/* ---- uneven batch: independent systems, mixed-precision PCG ------------- */
+#define SFR_UB_MAXSYS 1024       /* systems of one uneven batch             */
+
+/* One system of an uneven batch as the kernels see it: its device matrix
+ * (either format) and where its rows sit in the batch vectors. */
+typedef struct sfr_ub_sys {
+  const int32_t *col, *rowptr;
+  const double  *val, *ad, *dinv, *dscale;
+  const float   *val32;
+  int64_t        row0;      /* first row in the batch vectors               */
+  int32_t        n, slots, fmt, rpt;
+  int32_t        g;         /* work items of rpt*SFR_BLOCK rows             */
+  int32_t        gfirst;    /* first partial slot (4 per work item)         */
+} sfr_ub_sys;
+
+typedef struct sfr_ub_view {
+  const sfr_ub_sys *sys;
+  int           nsys, grid;
+  int          *act;        /* [0] active systems, [1] their work items,
+                               [2, 2+nsys) ids, [2+nsys, 2+2nsys) prefix     */
+  const double *b;
+  double       *x;
+  float        *y, *r, *p, *q;
+  sfr_scalars  *sc;         /* nsys                                         */
+  unsigned long long *ts;   /* [s] completion stamp, [nsys] start stamp     */
+  double       *partA, *partB;
+} sfr_ub_view;
+
+/* all = 1: list every system and stamp the start; 2: list every system;
+ * 0: list the systems still running */
+int sfr_k_ub_compact(hipStream_t s, const sfr_ub_view *v, int all);
+int sfr_k_ub_init(hipStream_t s, const sfr_ub_view *v, int newX, double tol,
+                  unsigned maxit, double delta);
+int sfr_k_ub_iter(hipStream_t s, const sfr_ub_view *v);
+int sfr_k_ub_final(hipStream_t s, const sfr_ub_view *v);
 
 #ifdef __cplusplus
 }

@@ -101,6 +101,72 @@ class DeviceMatrix:
             self.h = None
 
 
+class Batch:
+    """Uneven batch of independent systems (sfr_batch_*): mixed-precision
+    PCG over all of them in one stream of launches."""
+
+    def __init__(self, ctx, systems, tune=True):
+        self.ctx = ctx
+        self.systems = list(systems)        # keeps host arrays alive
+        n = len(self.systems)
+        arr = (SfrSystem * n)(*[s.c() for s in self.systems])
+        h = C.c_void_p()
+        ctx._check(lib.sfr_batch_create(ctx.h, n, arr, 1 if tune else 0,
+                                        C.byref(h)))
+        self.h = h.value
+        self.nsys = n
+        self.rows = int(lib.sfr_batch_rows(self.h))
+
+    def info(self, s):
+        r0, n, f, rpt = C.c_int64(), C.c_int32(), C.c_int(), C.c_int()
+        self.ctx._check(lib.sfr_batch_info(self.h, s, C.byref(r0), C.byref(n),
+                                           C.byref(f), C.byref(rpt)))
+        return {"row0": r0.value, "neq": n.value, "format": f.value,
+                "rpt": rpt.value}
+
+    def buckets(self):
+        out = (L.SfrBucketChoice * 32)()
+        k = lib.sfr_batch_buckets(self.h, out, 32)
+        return [out[i].as_dict() for i in range(k)]
+
+    def set_values(self, s, al, ad, au=None):
+        al = np.ascontiguousarray(al, np.float64)
+        ad = None if ad is None else np.ascontiguousarray(ad, np.float64)
+        au = None if au is None else np.ascontiguousarray(au, np.float64)
+        self.ctx._check(lib.sfr_batch_set_values(self.ctx.h, self.h, s,
+                                                 ptr(al), ptr(ad), ptr(au)))
+
+    def solve(self, bs, tol, maxit, xs=None, newX=True):
+        """Host arrays: bs[s] (and xs[s]) of system s's neq doubles.
+        Returns (list of x, list of stats dicts)."""
+        bs = [np.ascontiguousarray(b, np.float64) for b in bs]
+        assert len(bs) == self.nsys
+        xs = ([np.zeros(s.neq) for s in self.systems] if xs is None
+              else [np.array(x, np.float64) for x in xs])
+        P = C.c_void_p * self.nsys
+        pb = P(*[ptr(b) for b in bs])
+        px = P(*[ptr(x) for x in xs])
+        st = (SfrStats * self.nsys)()
+        rc = lib.sfr_batch_solve(self.ctx.h, self.h, tol, maxit,
+                                 C.cast(pb, C.c_void_p), C.cast(px, C.c_void_p),
+                                 1 if newX else 0, st)
+        self.ctx._check(rc, allow=(L.MAXIT, L.BREAKDOWN))
+        return xs, [st[c].as_dict() for c in range(self.nsys)]
+
+    def solve_device(self, d_b, d_x, tol, maxit, newX=True):
+        """Device vectors of sfr_batch_rows doubles, system s at info(s)['row0']."""
+        st = (SfrStats * self.nsys)()
+        rc = lib.sfr_batch_solve_device(self.ctx.h, self.h, tol, maxit,
+                                        d_b.ptr, d_x.ptr, 1 if newX else 0, st)
+        self.ctx._check(rc, allow=(L.MAXIT, L.BREAKDOWN))
+        return [st[c].as_dict() for c in range(self.nsys)]
+
+    def destroy(self):
+        if self.h:
+            lib.sfr_batch_destroy(self.ctx.h, self.h)
+            self.h = None
+
+
 class Context:
     def __init__(self, device=0):
         h = C.c_void_p()
@@ -135,6 +201,9 @@ class Context:
 
     def matrix(self, system, fmt=L.FMT_AUTO):
         return DeviceMatrix(self, system, fmt)
+
+    def batch(self, systems, tune=True):
+        return Batch(self, systems, tune)
 
     def solve(self, system, solver, tol, maxit, b, x=None, newX=True):
         """Host-array solve (sfr_solve).  Returns (x, stats dict)."""

@@ -251,3 +251,32 @@ def momentum3d(n, seed=1, storage=CSRD, reorder=False):
         bs.append(bc[:neq].copy())
     return Built(sysm, bs[0], id_, num, nelcon, nviz, pat,
                  {"b_all": np.stack(bs), "lA": lA})
+
+
+# ---------------------------------------------------------------------------
+# Uneven batch: nsys independent all-Neumann 3-D Poisson systems, cube side
+# n_s = round(smin * (smax/smin)^u), u ~ U[0,1) (log-uniform), conductances
+# exp(U[ln kmin, 0]), RHS U[-1, 1], one seeded cell pinned (numEqV2 drops it,
+# as the reference does for the pressure reference cell).  System s draws
+# its field from seed (seed << 16) + s; the batch-level draws (sides, pinned
+# cells) use stream 5 of `seed`.
+
+def uneven_sides(nsys, seed=1, smin=16, smax=96):
+    u = u01(seed, 5, np.arange(nsys))
+    return np.rint(smin * (smax / smin) ** u).astype(np.int64)
+
+
+def uneven_batch(nsys=256, seed=1, smin=16, smax=96, kmin=1e-3,
+                 storage=ELLPACK, sides=None):
+    sides = uneven_sides(nsys, seed, smin, smax) if sides is None else sides
+    pin = u01(seed, 6, np.arange(len(sides)))
+    out = []
+    for s, n in enumerate(sides):
+        n = int(n)
+        cell = int(pin[s] * n ** 3)
+        B = poisson3d(n, seed=(seed << 16) + s, kmin=kmin, reorder=False,
+                      storage=storage, pinned_cell=cell)
+        B.extra["side"] = n
+        B.extra["pinned"] = cell
+        out.append(B)
+    return out

@@ -70,3 +70,13 @@ def test_reference_solver_config_files(tmp_path):
     f.write_text("// comment line\nname cg\ntol 0\nstorage EllPack\nend\n")
     c = structure.read_solver_config(str(f))
     assert c["tol"] == 2.0 ** -53 and c["storage"] == 4 and c["maxit"] == 50000
+
+
+def test_batch_entry_points_refuse_bad_arguments():
+    lib = _lib.lib
+    h = C.c_void_p()
+    assert lib.sfr_batch_create(None, 1, None, 0, C.byref(h)) == _lib.BAD_ARG
+    assert lib.sfr_batch_rows(None) == 0
+    assert lib.sfr_batch_info(None, 0, None, None, None, None) == _lib.BAD_ARG
+    assert lib.sfr_batch_buckets(None, None, 0) == 0
+    lib.sfr_batch_destroy(None, None)
