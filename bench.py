@@ -40,6 +40,9 @@ def main():
     ap.add_argument("--n", type=int, default=128)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--no-extra", action="store_true",
+                    help="skip the other workloads (u/v/w batch, uneven "
+                         "batch, adjoint) and the copy ceiling")
     ap.add_argument("--profile-kernels", action="store_true",
                     help="only launch the two SpMV kernels and a plain copy "
                          "of known size (for rocprofv3 counter runs)")
@@ -126,9 +129,18 @@ def main():
 
     relerr = float(np.linalg.norm(xmx - x64) / np.linalg.norm(x64))
 
+    copy_gbs = None if args.no_extra else hbm_copy_ceiling(ctx)
+
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(sysm, B.b, args.cpu_threads, x64)
+
+    extra = {}
+    if not args.no_extra:
+        db.free(); dx.free(); A.destroy()
+        extra["momentum_uvw"] = bench_uvw(ctx, S, systems, lib)
+        extra["uneven_batch"] = bench_uneven(ctx, S, systems)
+        extra["adjoint"] = bench_adjoint(ctx, S, systems)
 
     out = {
         "metric": "Krylov solves/s + HBM GB/s, 128³ Poisson, mixed vs FP64 @ relres 1e-8",
@@ -171,9 +183,12 @@ def main():
                                    "launch_us": round(t_k64 * 1e6, 2),
                                    "bytes_per_launch": b64}},
         "cpu_baseline": cpu,
+        "hbm_copy_GBs": copy_gbs,
     }
+    out.update(extra)
     print(json.dumps(out), flush=True)
-    A.destroy()
+    if args.no_extra:
+        A.destroy()
     ctx.close()
 
 
@@ -196,6 +211,188 @@ def profile_kernels(ctx, A, lib):
     log(f"[profile] copy 256 MiB done (rc={rc})")
     src.free()
     dst.free()
+
+
+def _hip():
+    hip = C.CDLL("libamdhip64.so")
+    vp = C.c_void_p
+    hip.hipMemcpyAsync.argtypes = [vp, vp, C.c_size_t, C.c_int, vp]
+    hip.hipEventCreate.argtypes = [C.POINTER(vp)]
+    hip.hipEventRecord.argtypes = [vp, vp]
+    hip.hipEventSynchronize.argtypes = [vp]
+    hip.hipEventElapsedTime.argtypes = [C.POINTER(C.c_float), vp, vp]
+    hip.hipEventDestroy.argtypes = [vp]
+    return hip
+
+
+def hbm_copy_ceiling(ctx, gib=2, reps=5):
+    """Device-to-device copy of `gib` GiB (far above the 256 MB last-level
+    cache), timed with HIP events: the streaming rate this HBM sustains,
+    counted as read + write bytes."""
+    hip = _hip()
+    n = gib << 30
+    src = ctx.empty(n // 8)
+    dst = ctx.empty(n // 8)
+    e0, e1 = C.c_void_p(), C.c_void_p()
+    hip.hipEventCreate(C.byref(e0))
+    hip.hipEventCreate(C.byref(e1))
+    hip.hipMemcpyAsync(C.c_void_p(dst.ptr), C.c_void_p(src.ptr), n, 3, None)
+    hip.hipEventRecord(e0, None)
+    for _ in range(reps):
+        hip.hipMemcpyAsync(C.c_void_p(dst.ptr), C.c_void_p(src.ptr), n, 3, None)
+    hip.hipEventRecord(e1, None)
+    hip.hipEventSynchronize(e1)
+    ms = C.c_float()
+    hip.hipEventElapsedTime(C.byref(ms), e0, e1)
+    hip.hipEventDestroy(e0)
+    hip.hipEventDestroy(e1)
+    src.free()
+    dst.free()
+    gbs = 2.0 * n * reps / (ms.value * 1e-3) / 1e9
+    log(f"[bench] HBM copy ceiling ({gib} GiB D2D): {gbs:.0f} GB/s")
+    return round(gbs, 1)
+
+
+def bench_uvw(ctx, S, systems, lib, reps=3):
+    """u/v/w momentum (96^3 Taylor-Green upwind, shared CSRD pattern):
+    one batched BiCGSTAB launch stream vs three sequential device solves."""
+    from sfr3d_amd._lib import SfrStats
+    M = systems.momentum3d(96)
+    n = M.system.neq
+    A = ctx.matrix(M.system)
+    bs = M.extra["b_all"]
+    db = ctx.array(np.ascontiguousarray(bs.reshape(-1)))
+    dx = ctx.empty(3 * n)
+    db1 = [ctx.array(bs[c]) for c in range(3)]
+    dx1 = ctx.empty(n)
+    st = (SfrStats * 3)()
+
+    def batched():
+        rc = lib.sfr_solve_shared_device(ctx.h, A.h, 3, None, db.ptr, dx.ptr,
+                                         S.PBICGSTAB, TOL, 5000, 1, st)
+        assert rc == 0, rc
+
+    def sequential():
+        out = []
+        for c in range(3):
+            s1 = ctx.solve_device(A, S.PBICGSTAB, TOL, 5000, db1[c], dx1)
+            out.append((s1, dx1.get()))
+        return out
+
+    batched()
+    sequential()
+    t = time.perf_counter()
+    for _ in range(reps):
+        batched()
+    lib.sfr_synchronize(ctx.h)
+    tb = (time.perf_counter() - t) / reps
+    xb = dx.get().reshape(3, n)
+    t = time.perf_counter()
+    for _ in range(reps):
+        seq = sequential()
+    ts = (time.perf_counter() - t) / reps
+    rel = max(float(np.linalg.norm(xb[c] - seq[c][1]) / np.linalg.norm(seq[c][1]))
+              for c in range(3))
+    its = [st[c].iters for c in range(3)]
+    log(f"[bench] u/v/w 96^3 BiCGSTAB: batched {tb * 1e3:.2f} ms, 3 sequential "
+        f"{ts * 1e3:.2f} ms ({ts / tb:.2f}x), iters {its}, max relerr {rel:.1e}")
+    A.destroy(); db.free(); dx.free(); dx1.free()
+    for d in db1:
+        d.free()
+    return {"workload": "96^3 Taylor-Green upwind momentum, u/v/w, shared "
+                        "CSRD pattern, Jacobi-BiCGSTAB to tol 1e-8",
+            "neq": n, "batched_ms": round(tb * 1e3, 3),
+            "sequential_ms": round(ts * 1e3, 3),
+            "speedup": round(ts / tb, 3), "iters": its,
+            "relres": [st[c].relres for c in range(3)],
+            "max_relerr_batched_vs_sequential": rel}
+
+
+def bench_uneven(ctx, S, systems, reps=3):
+    """256 independent all-Neumann Poisson systems (sides log-uniform in
+    [16, 96], conductances exp(U[ln 1e-3, 0]), one pinned cell each), mixed
+    PCG to tol 1e-8 in one stream of launches, tuned per size bucket."""
+    t0 = time.time()
+    built = systems.uneven_batch(256, seed=1)
+    t_build = time.time() - t0
+    t0 = time.time()
+    Bt = ctx.batch([b.system for b in built], tune=True)
+    t_tune = time.time() - t0
+    hb = np.zeros(Bt.rows)
+    for k, b in enumerate(built):
+        r0 = Bt.info(k)["row0"]
+        hb[r0:r0 + b.system.neq] = b.b
+    db, dx = ctx.array(hb), ctx.empty(Bt.rows)
+    sts = Bt.solve_device(db, dx, TOL, 100000)
+    walls = []
+    for _ in range(reps):
+        t = time.perf_counter()
+        sts = Bt.solve_device(db, dx, TOL, 100000)
+        walls.append(time.perf_counter() - t)
+    wall = float(np.median(walls))
+    n = np.array([b.system.neq for b in built], float)
+    it = np.array([s["iters"] for s in sts], float)
+    outer = np.array([s["outer"] for s in sts], float)
+    ts = np.array([s["time_s"] for s in sts])
+    # algorithmic bytes: 88 per row per FP32 iteration, 168 per fold, set-up
+    # and final passes 104
+    byt = float((n * (it * 88.0 + (outer + 1) * 168.0 + 104.0)).sum())
+    ok = sum(1 for s in sts if s["status"] == 0)
+    log(f"[bench] uneven 256: {256 / wall:.1f} solves/s ({wall * 1e3:.1f} ms), "
+        f"p50 {np.percentile(ts, 50) * 1e3:.1f} ms p99 {np.percentile(ts, 99) * 1e3:.1f} ms, "
+        f"{byt / wall / 1e9:.0f} GB/s algorithmic, ok {ok}/256")
+    out = {"workload": "256 all-Neumann 3-D Poisson systems, sides log-uniform "
+                       "[16, 96], k = exp(U[ln 1e-3, 0]), one pinned cell each, "
+                       "mixed PCG to tol 1e-8, one stream of launches",
+           "rows": int(n.sum()), "solves_per_s": round(256 / wall, 2),
+           "ms_per_batch": round(wall * 1e3, 2),
+           "p50_ms": round(float(np.percentile(ts, 50)) * 1e3, 2),
+           "p99_ms": round(float(np.percentile(ts, 99)) * 1e3, 2),
+           "converged": ok, "max_relres": max(s["relres"] for s in sts),
+           "iters_min_med_max": [int(it.min()), int(np.median(it)), int(it.max())],
+           "solve_GBs_algorithmic": round(byt / wall / 1e9, 1),
+           "buckets": [{k: c[k] for k in ("bucket", "nsys", "format", "rpt")}
+                       for c in Bt.buckets()],
+           "build_s": round(t_build, 2), "create_and_tune_s": round(t_tune, 2)}
+    Bt.destroy(); db.free(); dx.free()
+    return out
+
+
+def bench_adjoint(ctx, S, systems, reps=5):
+    """64^3 variable-coefficient diffusion: forward FP64 PCG plus the adjoint
+    solve A^T lambda = dL/dx with the value gradients, device times."""
+    B = systems.poisson3d(64, seed=11)
+    n = B.system.neq
+    A = ctx.matrix(B.system)
+    db, dx, dg, dl = ctx.array(B.b), ctx.empty(n), ctx.empty(n), ctx.empty(n)
+    dgal, dgad = ctx.empty(len(B.system.al)), ctx.empty(n)
+    rng = np.random.default_rng(5)
+    probes = rng.choice(n, 512, replace=False)
+    st = ctx.solve_device(A, S.PCG, TOL, 20000, db, dx)
+    g = np.zeros(n)
+    g[probes] = 2.0 * dx.get()[probes]
+    dg.set(g)
+    ctx.solve_adjoint(A, S.PCG, TOL, 20000, dx, dg, dl, dgal, dgad)
+    tf, ta = [], []
+    for _ in range(reps):
+        t = time.perf_counter()
+        st = ctx.solve_device(A, S.PCG, TOL, 20000, db, dx)
+        tf.append(time.perf_counter() - t)
+        t = time.perf_counter()
+        sa = ctx.solve_adjoint(A, S.PCG, TOL, 20000, dx, dg, dl, dgal, dgad)
+        ta.append(time.perf_counter() - t)
+    out = {"workload": "64^3 variable-coefficient diffusion, loss over 512 probe "
+                       "cells; forward FP64 PCG + adjoint solve with value "
+                       "gradients, tol 1e-8",
+           "neq": n, "forward_ms": round(float(np.median(tf)) * 1e3, 3),
+           "adjoint_ms": round(float(np.median(ta)) * 1e3, 3),
+           "forward_iters": st["iters"], "adjoint_iters": sa["iters"]}
+    log(f"[bench] adjoint 64^3: forward {out['forward_ms']} ms, adjoint "
+        f"{out['adjoint_ms']} ms")
+    A.destroy()
+    for d in (db, dx, dg, dl, dgal, dgad):
+        d.free()
+    return out
 
 
 def cpu_baseline(sysm, b, threads, x_gpu64):

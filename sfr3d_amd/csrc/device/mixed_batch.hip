@@ -55,7 +55,11 @@ __device__ __forceinline__ void walk(const sfr_ub_view &v, F &&body)
       if (s_pre[mid] <= e) lo = mid;
       else hi = mid - 1;
     }
-    body(s_sys[lo], e - s_pre[lo]);
+    // the same for every lane: keep it in scalar registers, so the system's
+    // descriptor and scalars are read with scalar loads
+    const int s = __builtin_amdgcn_readfirstlane(s_sys[lo]);
+    const int lb = __builtin_amdgcn_readfirstlane(e - s_pre[lo]);
+    body(s, lb);
   }
 }
 
@@ -94,6 +98,62 @@ __device__ __forceinline__ float row32(const UbSys &S, int64_t i,
     for (int32_t k = S.rowptr[i]; k < e; k++) acc += S.val32[k] * x[S.col[k]];
   }
   return acc;
+}
+
+// Rows a thread keeps in flight: the slot kernels handle UB_U rows of a
+// work item at once, all loads first, then the arithmetic and the stores,
+// so a thread has UB_U rows' memory traffic outstanding instead of one.
+#define UB_U 4
+
+// rows r0 .. r0+UB_U-1 of this thread in the work item starting at `base`;
+// rows outside the item or the system are clamped to the last row (loads
+// stay in bounds) and flagged !ok (no store, no contribution)
+struct Rows {
+  int64_t i[UB_U];
+  bool ok[UB_U];
+};
+
+__device__ __forceinline__ Rows rows_at(const UbSys &S, int64_t base, int r0)
+{
+  Rows R;
+#pragma unroll
+  for (int u = 0; u < UB_U; u++) {
+    const int64_t i = base + (int64_t)(r0 + u) * SFR_BLOCK + threadIdx.x;
+    R.ok[u] = r0 + u < S.rpt && i < S.n;
+    R.i[u] = R.ok[u] ? i : S.n - 1;
+  }
+  return R;
+}
+
+// (A^ x)_i for the UB_U rows of R
+__device__ __forceinline__ void rows32(const UbSys &S, const Rows &R,
+                                       const float *__restrict__ x,
+                                       float (&out)[UB_U])
+{
+  if (S.fmt == SFR_FMT_ELL && S.slots == 6) {
+    const int64_t n = S.n;
+    const float *__restrict__ val = S.val32;
+    const int32_t *__restrict__ col = S.col;
+    int32_t cj[UB_U][6];
+    float a[UB_U][6];
+#pragma unroll
+    for (int u = 0; u < UB_U; u++)
+#pragma unroll
+      for (int f = 0; f < 6; f++) {
+        cj[u][f] = col[f * n + R.i[u]];
+        a[u][f] = val[f * n + R.i[u]];
+      }
+#pragma unroll
+    for (int u = 0; u < UB_U; u++) {
+      float acc = x[R.i[u]];
+#pragma unroll
+      for (int f = 0; f < 6; f++) acc += a[u][f] * x[cj[u][f]];
+      out[u] = acc;
+    }
+  } else {
+#pragma unroll
+    for (int u = 0; u < UB_U; u++) out[u] = row32(S, R.i[u], x);
+  }
 }
 
 // value of the not yet folded solution x + D^-1/2 y at row j
@@ -253,15 +313,21 @@ __global__ __launch_bounds__(SFR_BLOCK) void k_ub_s1(sfr_ub_view v)
     const int64_t base = (int64_t)lb * S.rpt * SFR_BLOCK;
     const float *p = v.p + S.row0;
     if (mode == UB_CG) {                       // q = A^ p; p.q
-      float *q = v.q + S.row0;
+      const float *__restrict__ pp = p;
+      float *__restrict__ q = v.q + S.row0;
       double acc[1] = {0.0};
-      for (int r = 0; r < S.rpt; r++) {
-        const int64_t i = base + r * SFR_BLOCK + threadIdx.x;
-        if (i < S.n) {
-          const float qi = row32(S, i, p);
-          q[i] = qi;
-          acc[0] += (double)qi * (double)p[i];
-        }
+      for (int r0 = 0; r0 < S.rpt; r0 += UB_U) {
+        const Rows R = rows_at(S, base, r0);
+        float qv[UB_U], pv[UB_U];
+        rows32(S, R, pp, qv);
+#pragma unroll
+        for (int u = 0; u < UB_U; u++) pv[u] = pp[R.i[u]];
+#pragma unroll
+        for (int u = 0; u < UB_U; u++)
+          if (R.ok[u]) {
+            q[R.i[u]] = qv[u];
+            acc[0] += (double)qv[u] * (double)pv[u];
+          }
       }
       store_at<1>(acc, slots_of(v.partA, S), S.g, lb);
       return;
@@ -315,16 +381,24 @@ __global__ __launch_bounds__(SFR_BLOCK) void k_ub_s2(sfr_ub_view v)
         c->run3 = UB_CG;
       }
       const float fa = (float)alpha;
-      const float *q = v.q + S.row0;
-      float *r = v.r + S.row0;
+      const float *__restrict__ q = v.q + S.row0;
+      float *__restrict__ r = v.r + S.row0;
       double acc[1] = {0.0};
-      for (int k = 0; k < S.rpt; k++) {
-        const int64_t i = base + k * SFR_BLOCK + threadIdx.x;
-        if (i < S.n) {
-          const float ri = r[i] - fa * q[i];
-          r[i] = ri;
-          acc[0] += (double)ri * (double)ri;
+      for (int r0 = 0; r0 < S.rpt; r0 += UB_U) {
+        const Rows R = rows_at(S, base, r0);
+        float rv[UB_U], qv[UB_U];
+#pragma unroll
+        for (int u = 0; u < UB_U; u++) {
+          rv[u] = r[R.i[u]];
+          qv[u] = q[R.i[u]];
         }
+#pragma unroll
+        for (int u = 0; u < UB_U; u++)
+          if (R.ok[u]) {
+            const float ri = rv[u] - fa * qv[u];
+            r[R.i[u]] = ri;
+            acc[0] += (double)ri * (double)ri;
+          }
       }
       store_at<1>(acc, slots_of(v.partB, S), S.g, lb);
       return;
@@ -372,8 +446,8 @@ __global__ __launch_bounds__(SFR_BLOCK) void k_ub_s3(sfr_ub_view v)
     if (!run) return;
     const bool owner = lb == 0 && threadIdx.x == 0;
     const int64_t base = (int64_t)lb * S.rpt * SFR_BLOCK;
-    float *p = v.p + S.row0;
-    const float *r = v.r + S.row0;
+    float *__restrict__ p = v.p + S.row0;
+    const float *__restrict__ r = v.r + S.row0;
     if (run == UB_CG) {
       double t[1];
       reduce_partials<1>(slots_of(v.partB, S), S.g, t);
@@ -382,14 +456,22 @@ __global__ __launch_bounds__(SFR_BLOCK) void k_ub_s3(sfr_ub_view v)
       const bool fold = rr < c->delta2 * c->rho_max || rr <= conv * conv ||
                         !isfinite(rr);
       const float fa = (float)c->alpha, fb = (float)(rr / c->d_prev);
-      float *y = v.y + S.row0;
-      for (int k = 0; k < S.rpt; k++) {
-        const int64_t i = base + k * SFR_BLOCK + threadIdx.x;
-        if (i < S.n) {
-          const float pi = p[i];
-          y[i] += fa * pi;
-          if (!fold) p[i] = r[i] + fb * pi;
+      float *__restrict__ y = v.y + S.row0;
+      for (int r0 = 0; r0 < S.rpt; r0 += UB_U) {
+        const Rows R = rows_at(S, base, r0);
+        float pv[UB_U], rv[UB_U], yv[UB_U];
+#pragma unroll
+        for (int u = 0; u < UB_U; u++) {
+          pv[u] = p[R.i[u]];
+          rv[u] = r[R.i[u]];
+          yv[u] = y[R.i[u]];
         }
+#pragma unroll
+        for (int u = 0; u < UB_U; u++)
+          if (R.ok[u]) {
+            y[R.i[u]] = yv[u] + fa * pv[u];
+            if (!fold) p[R.i[u]] = rv[u] + fb * pv[u];
+          }
       }
       if (owner) {
         const unsigned it = c->iter + 1;

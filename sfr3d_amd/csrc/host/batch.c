@@ -32,6 +32,10 @@
 #define ROWALIGN 64
 #define TUNE_SAMPLES 4
 #define TUNE_REPS 8
+#ifndef TUNE_ROWS
+#define TUNE_ROWS ((int64_t)4 << 20)   /* rows a tuning trial fills          */
+#endif
+#define TUNE_MAXRPT 16
 
 struct sfr_batch {
   int          nsys;
@@ -164,32 +168,33 @@ static sfr_ub_view view_of(const ub_bufs *u, int nsys, int grid)
 /* ---- autotuner ------------------------------------------------------------ */
 
 /* Time of one FP32 CG iteration (the three slot kernels, every system in CG
- * state) per row, for the sample matrices M[0..m) at rows per thread rpt. */
-static int time_iteration(sfr_context *ctx, sfr_matrix **M, int m, int rpt,
-                          double *ns_per_row)
+ * state) per row, for `copies` systems cycling through the sample matrices
+ * M[0..m), at rows per thread rpt.  u holds buffers for the copies at rows
+ * per thread 1 (the most work items). */
+static int time_iteration(sfr_context *ctx, const ub_bufs *u, sfr_matrix **M,
+                          int m, int copies, int rpt, double *ns_per_row)
 {
-  sfr_ub_sys h[TUNE_SAMPLES];
+  sfr_ub_sys *h = (sfr_ub_sys *)malloc(sizeof(sfr_ub_sys) * (size_t)copies);
+  if (!h) return sfr_set_error(ctx, SFR_BAD_ARG, "host allocation");
   int64_t rows = 0, items = 0, real = 0;
-  for (int k = 0; k < m; k++) {
-    describe(&h[k], M[k], rows, rpt, items);
-    rows += align_rows(M[k]->n);
-    items += h[k].g;
-    real += M[k]->n;
+  for (int c = 0; c < copies; c++) {
+    describe(&h[c], M[c % m], rows, rpt, items);
+    rows += align_rows(M[c % m]->n);
+    items += h[c].g;
+    real += M[c % m]->n;
   }
-  ub_bufs u;
-  CHECK(alloc_bufs(ctx, &u, m, rows, items));
   const int grid = grid_of(ctx, items);
-  sfr_ub_view v = view_of(&u, m, grid);
+  sfr_ub_view v = view_of(u, copies, grid);
   hipStream_t s = ctx->stream;
   int rc = SFR_OK;
   float ms = 0.0f;
 #define UTRY(call) do { hipError_t e_ = (hipError_t)(call); if (e_ != hipSuccess) { \
     rc = sfr_set_error(ctx, SFR_HIP_ERROR, "%s: %s", #call, hipGetErrorString(e_)); goto out; } } while (0)
-  UTRY(hipMemcpyAsync(u.d_sys, h, sizeof(sfr_ub_sys) * (size_t)m, hipMemcpyHostToDevice, s));
-  UTRY(sfr_k_fill(s, rows, 1.0, u.b));
+  UTRY(hipMemcpyAsync(u->d_sys, h, sizeof(sfr_ub_sys) * (size_t)copies, hipMemcpyHostToDevice, s));
+  UTRY(sfr_k_fill(s, rows, 1.0, u->b));
   UTRY(sfr_k_ub_compact(s, &v, 1));
   /* tol 0, delta 0: no fold after the first, so every timed iteration is
-   * a CG iteration of every sample */
+   * a CG iteration of every copy */
   UTRY(sfr_k_ub_init(s, &v, 1, 0.0, 0x7fffffffu, 0.0));
   for (int it = 0; it < 2; it++) UTRY(sfr_k_ub_iter(s, &v));
   UTRY(hipEventRecord(ctx->ev_start, s));
@@ -201,7 +206,7 @@ static int time_iteration(sfr_context *ctx, sfr_matrix **M, int m, int rpt,
 out:
 #undef UTRY
   hipStreamSynchronize(s);
-  free_bufs(&u);
+  free(h);
   return rc;
 }
 
@@ -214,6 +219,19 @@ static int tune_bucket(sfr_context *ctx, const sfr_system *sys, const int *mem,
   const int ns = m < TUNE_SAMPLES ? m : TUNE_SAMPLES;
   int pick[TUNE_SAMPLES];
   for (int k = 0; k < ns; k++) pick[k] = mem[(int64_t)k * m / ns];
+  /* replicas of the samples up to TUNE_ROWS rows: the bulk of a batch solve
+   * runs with the GPU full, and that is the regime the choice is made for */
+  int64_t round_rows = 0, round_items = 0;
+  for (int k = 0; k < ns; k++) {
+    round_rows += align_rows(sys[pick[k]].neq);
+    round_items += items_for(sys[pick[k]].neq, 1);
+  }
+  int64_t rounds = (TUNE_ROWS + round_rows - 1) / round_rows;
+  if (rounds * ns > SFR_UB_MAXSYS) rounds = SFR_UB_MAXSYS / ns;
+  if (rounds < 1) rounds = 1;
+  const int copies = (int)(rounds * ns);
+  ub_bufs u;
+  CHECK(alloc_bufs(ctx, &u, copies, rounds * round_rows, rounds * round_items));
   sfr_matrix *M[2][TUNE_SAMPLES];
   memset(M, 0, sizeof M);
   const int fmts[2] = {SFR_FMT_ELL, SFR_FMT_CSR};
@@ -224,12 +242,13 @@ static int tune_bucket(sfr_context *ctx, const sfr_system *sys, const int *mem,
       rc = sfr_matrix_create(ctx, &sys[pick[k]], fmts[f], &M[f][k]);
       if (rc == SFR_OK) rc = sfr_matrix_prepare_mixed(ctx, M[f][k]);
     }
-    for (int rpt = 1; rpt <= 8 && rc == SFR_OK; rpt *= 2) {
+    for (int rpt = 1; rpt <= TUNE_MAXRPT && rc == SFR_OK; rpt *= 2) {
       double t = 0.0;
-      rc = time_iteration(ctx, M[f], ns, rpt, &t);
+      rc = time_iteration(ctx, &u, M[f], ns, copies, rpt, &t);
       if (rc == SFR_OK && (best < 0.0 || t < best)) { best = t; bf = f; brpt = rpt; }
     }
   }
+  free_bufs(&u);
   for (int f = 0; f < 2; f++)
     for (int k = 0; k < ns; k++) {
       if (rc == SFR_OK && f == bf) {
