@@ -280,3 +280,146 @@ def uneven_batch(nsys=256, seed=1, smin=16, smax=96, kmin=1e-3,
         B.extra["pinned"] = cell
         out.append(B)
     return out
+
+
+# ---------------------------------------------------------------------------
+# 2-D lid-driven cavity (unit square, n x n quads numbered row-major from the
+# bottom-left, faces S,E,N,W): SIMPLE on a collocated grid with first-order
+# upwind convection and Rhie-Chow face fluxes, the pressure-correction system
+# of every outer iteration recorded in SFR3D's storages.  The pressure-
+# correction matrix has off-diagonals -rho d_f A_f/|xi| (d_f = V/a_P averaged
+# to the face), its diagonal the sum over interior faces, its right-hand side
+# minus the cell's net mass outflow, and one reference cell pinned (numEqV2
+# drops it); walls carry no correction.
+
+def _momentum_matrix(aS, aE, aN, aW, aP):
+    """CSR (scipy) of the n x n five-point operator aP x_P - sum a_nb x_nb."""
+    import scipy.sparse as sp
+    ny, nx = aP.shape
+    idx = np.arange(nx * ny).reshape(ny, nx)
+    rows, cols, vals = [idx.ravel()], [idx.ravel()], [aP.ravel()]
+    for a, dj, di in ((aS, -1, 0), (aE, 0, 1), (aN, 1, 0), (aW, 0, -1)):
+        jj, ii = np.nonzero(a)
+        rows.append(idx[jj, ii])
+        cols.append(idx[jj + dj, ii + di])
+        vals.append(-a[jj, ii])
+    return sp.csr_matrix((np.concatenate(vals), (np.concatenate(rows),
+                                                 np.concatenate(cols))),
+                         shape=(nx * ny, nx * ny))
+
+
+def cavity2d(n=128, outer=20, nu=0.01, rho=1.0, ulid=1.0, alpha_u=0.7,
+             alpha_p=0.3, pinned_cell=0, storages=(CSRD, ELLPACK),
+             p_solve=None, record=True):
+    """Run `outer` SIMPLE iterations; return (systems, fields) where
+    systems[k][storage] is the Built pressure-correction system of outer
+    iteration k and fields = {u, v, p, mass} (mass = max |net outflow| per
+    iteration).  p_solve(system, b) -> x solves the pressure correction (CSRD
+    storage); by default a direct sparse solve."""
+    import scipy.sparse.linalg as spla
+    h = 1.0 / n
+    A = h                                   # face area (unit depth)
+    V = h * h
+    D = nu * A / h                          # interior face diffusion
+    Dw = nu * A / (0.5 * h)                 # wall face diffusion
+    u = np.zeros((n, n))
+    v = np.zeros((n, n))
+    p = np.zeros((n, n))
+    Fx = np.zeros((n, n + 1))               # outward +x mass flux, x-faces
+    Fy = np.zeros((n + 1, n))               # +y mass flux, y-faces
+    nelcon, nviz = S.quad_adjacency(n, n)
+    num = S.reord(nelcon, nviz, False)
+    rt = np.zeros((n * n, 5), np.int16)
+    rt[pinned_cell, 4] = 1
+    id_, neq = S.numeq_v2(num, rt, nviz, 4)
+    pats = {st: S.datastruct(st, False, id_, num, nelcon, nviz, neq)
+            for st in storages}
+    inE = np.zeros((n, n), bool); inE[:, :-1] = True
+    inW = np.zeros((n, n), bool); inW[:, 1:] = True
+    inN = np.zeros((n, n), bool); inN[:-1, :] = True
+    inS = np.zeros((n, n), bool); inS[1:, :] = True
+
+    def grad(f):
+        """cell-centre gradient, face values averaged, boundary = cell."""
+        fe = np.where(inE, 0.5 * (f + np.roll(f, -1, 1)), f)
+        fw = np.where(inW, 0.5 * (f + np.roll(f, 1, 1)), f)
+        fn = np.where(inN, 0.5 * (f + np.roll(f, -1, 0)), f)
+        fs = np.where(inS, 0.5 * (f + np.roll(f, 1, 0)), f)
+        return (fe - fw) / h, (fn - fs) / h
+
+    systems_out, mass = [], []
+    for it in range(outer):
+        Fe, Fw = Fx[:, 1:], Fx[:, :-1]
+        Fn, Fs = Fy[1:, :], Fy[:-1, :]
+        aE = np.where(inE, D + np.maximum(-Fe, 0.0), 0.0)
+        aW = np.where(inW, D + np.maximum(Fw, 0.0), 0.0)
+        aN = np.where(inN, D + np.maximum(-Fn, 0.0), 0.0)
+        aS = np.where(inS, D + np.maximum(Fs, 0.0), 0.0)
+        wall = (np.where(inE, 0.0, Dw) + np.where(inW, 0.0, Dw)
+                + np.where(inN, 0.0, Dw) + np.where(inS, 0.0, Dw))
+        aP = aE + aW + aN + aS + wall + (Fe - Fw + Fn - Fs)
+        aP = np.maximum(aP, aE + aW + aN + aS + wall) / alpha_u
+        dpx, dpy = grad(p)
+        lid = np.where(inN, 0.0, Dw * ulid)
+        bu = -V * dpx + lid + (1.0 - alpha_u) * aP * u
+        bv = -V * dpy + (1.0 - alpha_u) * aP * v
+        M = _momentum_matrix(aS, aE, aN, aW, aP)
+        u = spla.spsolve(M.tocsc(), bu.ravel()).reshape(n, n)
+        v = spla.spsolve(M.tocsc(), bv.ravel()).reshape(n, n)
+        # Rhie-Chow face fluxes
+        d = V / aP
+        dpx, dpy = grad(p)
+        de = 0.5 * (d[:, :-1] + d[:, 1:])
+        ue = (0.5 * (u[:, :-1] + u[:, 1:])
+              - de * ((p[:, 1:] - p[:, :-1]) / h - 0.5 * (dpx[:, :-1] + dpx[:, 1:])))
+        dn = 0.5 * (d[:-1, :] + d[1:, :])
+        vn = (0.5 * (v[:-1, :] + v[1:, :])
+              - dn * ((p[1:, :] - p[:-1, :]) / h - 0.5 * (dpy[:-1, :] + dpy[1:, :])))
+        Fx = np.zeros((n, n + 1)); Fx[:, 1:-1] = rho * ue * A
+        Fy = np.zeros((n + 1, n)); Fy[1:-1, :] = rho * vn * A
+        # pressure correction: faces S,E,N,W; coefficient rho d_f A / h
+        cE = np.zeros((n, n)); cE[:, :-1] = rho * de * A / h
+        cW = np.zeros((n, n)); cW[:, 1:] = rho * de * A / h
+        cN = np.zeros((n, n)); cN[:-1, :] = rho * dn * A / h
+        cS = np.zeros((n, n)); cS[1:, :] = rho * dn * A / h
+        net = Fx[:, 1:] - Fx[:, :-1] + Fy[1:, :] - Fy[:-1, :]
+        mass.append(float(np.abs(net).max()))
+        lA = np.empty((n * n, 5))
+        lA[:, 0], lA[:, 1], lA[:, 2], lA[:, 3] = -cS.ravel(), -cE.ravel(), -cN.ravel(), -cW.ravel()
+        lA[:, 4] = (cS + cE + cN + cW).ravel()
+        lB = -net.ravel()
+        rec = {}
+        for st, pat in pats.items():
+            al, ad, b = S.assemble(pat, id_, nelcon, nviz, lA, lB)
+            sysm = System(neq, pat.nad, pat.ia, pat.ja, al, ad, st, False)
+            rec[st] = Built(sysm, b[:neq].copy(), id_, num, nelcon, nviz, pat,
+                            {"outer": it})
+        if record:
+            systems_out.append(rec)
+        first = rec[storages[0]]
+        if p_solve is None:
+            pr = spla.spsolve(_pressure_csr(first.system).tocsc(), first.b)
+        else:
+            pr = p_solve(first.system, first.b)
+        pc = np.zeros(n * n)
+        cells = np.nonzero(id_ > 0)[0]
+        pc[cells] = pr[id_[cells] - 1]
+        pc = pc.reshape(n, n)
+        # corrections
+        p = p + alpha_p * pc
+        gx, gy = grad(pc)
+        u = u - d * gx
+        v = v - d * gy
+        Fx[:, 1:-1] -= rho * de * A * (pc[:, 1:] - pc[:, :-1]) / h
+        Fy[1:-1, :] -= rho * dn * A * (pc[1:, :] - pc[:-1, :]) / h
+    return systems_out, {"u": u, "v": v, "p": p, "mass": np.array(mass)}
+
+
+def _pressure_csr(sysm):
+    """scipy CSR of a CSRD symmetric (lower) system, for the default direct
+    pressure solve of cavity2d."""
+    import scipy.sparse as sp
+    n = sysm.neq
+    rows = np.repeat(np.arange(n), np.diff(sysm.ia))
+    L = sp.csr_matrix((sysm.al, (rows, sysm.ja)), shape=(n, n))
+    return L + L.T + sp.diags(sysm.ad[:n])

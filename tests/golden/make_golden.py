@@ -10,6 +10,11 @@ tree and oracle/_ref exist: `python tests/golden/make_golden.py`).
                              (oracle/_ref): GENRCM orderings (digest + first
                              entries) and solverC results on small systems
   ref_solutions.npz          the solution vectors of those solves
+  cavity_ref.json            solverC (PCG, tol 1e-8) on pressure-correction
+                             systems of the 128x128 lid-driven cavity (SIMPLE
+                             outer iterations 0, 5, 11; CSRD and ELLPACK):
+                             iterations, xKx, ||x||, digests of the inputs
+                             and of x
 """
 import hashlib
 import io
@@ -104,6 +109,48 @@ def ref_vectors():
     np.savez_compressed(os.path.join(HERE, "ref_solutions.npz"), **sols)
 
 
+CAVITY_N, CAVITY_OUTER, CAVITY_PICK = 128, 12, (0, 5, 11)
+
+
+def cavity_inputs_digest(B):
+    s = B.system
+    return sha(np.concatenate([s.ia.view(np.uint8), s.ja.view(np.uint8),
+                               s.al.view(np.uint8), s.ad[:s.neq].view(np.uint8),
+                               B.b.view(np.uint8)]))
+
+
+def cavity_vectors():
+    import oracle
+    from sfr3d_amd import systems as Y
+    R = oracle.ref()
+    R.ref_init(1 << 31)
+    sysl, fields = Y.cavity2d(n=CAVITY_N, outer=CAVITY_OUTER)
+    out = {"n": CAVITY_N, "outer": CAVITY_OUTER, "mass": fields["mass"].tolist(),
+           "solves": []}
+    for k in CAVITY_PICK:
+        for st, B in sysl[k].items():
+            s = B.system
+            x = np.zeros(s.neq)
+            res = np.zeros(4)
+            rc = R.ref_solve(s.neq, s.nad, s.ia, s.ja, np.ascontiguousarray(s.al),
+                             np.ascontiguousarray(s.ad), None,
+                             np.ascontiguousarray(B.b), x, 1e-8, 50000, s.storage,
+                             1, 1, 0, res, None, 0)
+            assert rc == 0
+            out["solves"].append({"outer": k, "storage": st, "iters": int(res[0]),
+                                  "xKx": res[1], "norm_x": res[2],
+                                  "inputs_sha256": cavity_inputs_digest(B),
+                                  "x_sha256": sha(x)})
+            print("cavity", k, st, int(res[0]), flush=True)
+    with open(os.path.join(HERE, "cavity_ref.json"), "w") as f:
+        json.dump(out, f, indent=1)
+
+
 if __name__ == "__main__":
-    mesh_fixture()
-    ref_vectors()
+    what = sys.argv[1:] or ["mesh", "ref", "cavity"]
+    if "mesh" in what:
+        mesh_fixture()
+    if "ref" in what:
+        ref_vectors()
+    if "cavity" in what:
+        cavity_vectors()
