@@ -109,23 +109,28 @@ def main():
     lib.sfr_kernel_time(ctx.h, A.h, 0, reps, C.byref(ms))
     t_k64 = ms.value * 1e-3
     s = info["slots"]
-    # algorithmic bytes per row of one launch (each stream counted once):
-    #   FP32 inner SpMV: s*(4 val + 4 col) + 4 (read p_i) + 4 (write q_i)
-    #   FP64 SpMV      : s*(8 val + 4 col) + 8 (ad_i) + 8 (p_i) + 8 (q_i)
-    b32 = neq * (s * 8 + 8)
-    b64 = neq * (s * 12 + 24)
+    ib = info["index_bytes"]
+    # algorithmic bytes per row of one launch (each stream counted once; ib =
+    # bytes per stored column index, 2 for 16-bit row offsets):
+    #   fused FP32 kernel: s*(4 val + ib) + r_i, p_old_i (4+4) read,
+    #                      p_new_i, q_i (4+4) written, y_i (4+4) read+written
+    #   FP64 SpMV        : s*(8 val + ib) + 8 (ad_i) + 8 (p_i) + 8 (q_i)
+    b32 = neq * (s * (4 + ib) + 24)
+    b64 = neq * (s * (8 + ib) + 24)
     gbs32 = b32 / t_k32 / 1e9
     gbs64 = b64 / t_k64 / 1e9
-    log(f"[bench] FP32 SpMV {t_k32 * 1e6:.1f} us -> {gbs32:.0f} GB/s; "
+    log(f"[bench] fused FP32 kernel {t_k32 * 1e6:.1f} us -> {gbs32:.0f} GB/s; "
         f"FP64 SpMV {t_k64 * 1e6:.1f} us -> {gbs64:.0f} GB/s")
 
-    # whole-solve algorithmic bytes (per row): FP64 PCG iteration 176 B
-    # (spmv 96 + resid 32 + dir 48); MIXED inner iteration 88 B (spmv 56 +
-    # resid 12 + dir 20), outer step 8+12 (update) + 96+4+12 (residual)
+    # whole-solve algorithmic bytes (per row): FP64 PCG iteration = spmv +
+    # resid 32 + dir 48; mixed FP32 iteration = fused kernel + 12 (alpha,
+    # r -= alpha q, r.r); a fold about 8+12 (update) + FP64 residual
+    m64 = s * (8 + ib) + 24
     it64 = st64["iters"]
-    solve_bytes64 = neq * (it64 * 176.0 + 16 + 96 + 32 + 96)
+    solve_bytes64 = neq * (it64 * (m64 + 80.0) + 16 + 2 * m64 + 32)
     itmx, outer = stmx["iters"], stmx["outer"]
-    solve_bytesmx = neq * (itmx * 88.0 + (outer + 1) * 132.0 + 16 + 96)
+    solve_bytesmx = neq * (itmx * (b32 / neq + 12.0)
+                           + (outer + 1) * (20.0 + m64 + 16) + 16 + m64)
 
     relerr = float(np.linalg.norm(xmx - x64) / np.linalg.norm(x64))
 
@@ -172,12 +177,15 @@ def main():
                  "ref_criterion": [st64["norm_r_m"], st64["conv"]],
                  "solve_GBs_algorithmic": round(solve_bytes64 / t64 / 1e9, 1)},
         "speedup_mixed_vs_fp64": round(t64 / tmx, 3),
-        "roofline": {"bound": "hbm", "kernel": "k_mx_spmv (FP32 ELL SpMV + p.q)",
+        "roofline": {"bound": "hbm",
+                     "kernel": "k_mx_a (fused FP32 iteration kernel: ELL SpMV "
+                               "of r + beta p, direction and correction "
+                               "updates, p.q)",
                      "achieved": round(gbs32, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(gbs32 / HBM_PEAK_GBS, 4),
                      "traffic": None,
                      "launch_us": round(t_k32 * 1e6, 2),
-                     "bytes_per_launch": b32,
+                     "bytes_per_launch": b32, "index_bytes": ib,
                      "fp64_spmv": {"achieved": round(gbs64, 1),
                                    "frac": round(gbs64 / HBM_PEAK_GBS, 4),
                                    "launch_us": round(t_k64 * 1e6, 2),
@@ -198,7 +206,7 @@ def profile_kernels(ctx, A, lib):
     a kernel whose traffic is known (256 MiB read + 256 MiB written)."""
     ms = C.c_double()
     lib.sfr_kernel_time(ctx.h, A.h, 1, 5, C.byref(ms))
-    log(f"[profile] k_mx_spmv {ms.value * 1e3:.1f} us")
+    log(f"[profile] k_mx_a {ms.value * 1e3:.1f} us")
     lib.sfr_kernel_time(ctx.h, A.h, 0, 5, C.byref(ms))
     log(f"[profile] k_pcg_spmv {ms.value * 1e3:.1f} us")
     nbytes = 256 << 20

@@ -116,6 +116,10 @@ int  sfr_matrix_tune(sfr_context *ctx, sfr_matrix *A, int rpt, int lanes);
 /* neq, device format, ELL slots, stored nnz of the device format */
 int  sfr_matrix_info(const sfr_matrix *A, int32_t *neq, int *format,
                      int *slots, int64_t *nnz);
+/* bytes per stored column index the solve kernels read: 2 when the device
+ * ELL's columns are kept as 16-bit offsets from the row (every |j - i| <
+ * 32768, e.g. banded after RCM), else 4 */
+int  sfr_matrix_index_bytes(const sfr_matrix *A);
 
 /* ---- solves ---------------------------------------------------------------
  * sfr_solve: host arrays in and out, the drop-in for solverC's PCG and
@@ -217,7 +221,8 @@ void sfr_batch_destroy(sfr_context *ctx, sfr_batch *B);
 
 /* Measurement hook: average device time of one launch of a loop kernel,
  * HIP events on the context stream.  which = 0: FP64 PCG SpMV + p.q;
- * which = 1: FP32 inner SpMV + p.q. */
+ * which = 1: the first kernel of a mixed-precision FP32 iteration (SpMV of
+ * the updated direction, direction and correction updates, p.q). */
 int sfr_kernel_time(sfr_context *ctx, sfr_matrix *A, int which, int reps,
                     double *ms_per_launch);
 /* Tuning knobs of a context: "chunk" = iterations per captured graph chunk

@@ -98,8 +98,10 @@ __device__ __forceinline__ void reduce_partials(const double *__restrict__ part,
 
 // Slot-major ELL: val[f*n + i], col[f*n + i]; one lane per row.  S > 0 fixes
 // the slot count at compile time, S == 0 reads it at run time.  UNIT = the
-// diagonal is 1 (Jacobi-scaled copy) and ad is not read.
-template <typename T, int S, bool UNIT>
+// diagonal is 1 (Jacobi-scaled copy) and ad is not read.  D16 = the columns
+// are read as 16-bit offsets from the row, dcol[f*n + i] = col - i (half the
+// index traffic; used when every offset fits).
+template <typename T, int S, bool UNIT, bool D16 = false>
 struct EllOp {
   static constexpr int L = 1;
   const T *val;
@@ -107,15 +109,43 @@ struct EllOp {
   const T *ad;
   int64_t n;
   int s;
+  const int16_t *dcol;
+  __device__ __forceinline__ int64_t cj(int f, int64_t i) const
+  {
+    if constexpr (D16) return i + (int64_t)dcol[f * n + i];
+    else return col[f * n + i];
+  }
   __device__ __forceinline__ T row(int64_t i, bool valid, const T *__restrict__ x) const
   {
     if (!valid) return T(0);
     T acc = UNIT ? x[i] : ad[i] * x[i];
     if constexpr (S > 0) {
 #pragma unroll
-      for (int f = 0; f < S; f++) acc += val[f * n + i] * x[col[f * n + i]];
+      for (int f = 0; f < S; f++) acc += val[f * n + i] * x[cj(f, i)];
     } else {
-      for (int f = 0; f < s; f++) acc += val[f * n + i] * x[col[f * n + i]];
+      for (int f = 0; f < s; f++) acc += val[f * n + i] * x[cj(f, i)];
+    }
+    return acc;
+  }
+  // (A (r + beta p))_i with a unit diagonal: the combination of each
+  // neighbour is formed exactly as that neighbour's own row forms it, so the
+  // result is the product with the updated vector, bit for bit
+  __device__ __forceinline__ T row_comb(int64_t i, bool valid,
+      const T *__restrict__ r, const T *__restrict__ p, T beta) const
+  {
+    if (!valid) return T(0);
+    T acc = r[i] + beta * p[i];
+    if constexpr (S > 0) {
+#pragma unroll
+      for (int f = 0; f < S; f++) {
+        const int64_t j = cj(f, i);
+        acc += val[f * n + i] * (r[j] + beta * p[j]);
+      }
+    } else {
+      for (int f = 0; f < s; f++) {
+        const int64_t j = cj(f, i);
+        acc += val[f * n + i] * (r[j] + beta * p[j]);
+      }
     }
     return acc;
   }
@@ -124,8 +154,8 @@ struct EllOp {
 // NB systems sharing the pattern and the off-diagonal values, each with its
 // own diagonal: x and ad interleaved per row (x[i*NB + c]).  One pass over
 // the row's columns and values serves all NB products.
-template <typename T, int S, bool UNIT, int NB>
-__device__ __forceinline__ void ell_row_multi(const EllOp<T, S, UNIT> &A,
+template <typename T, int S, bool UNIT, bool D16, int NB>
+__device__ __forceinline__ void ell_row_multi(const EllOp<T, S, UNIT, D16> &A,
     int64_t i, bool valid, const T *__restrict__ ad, const T *__restrict__ x,
     T (&acc)[NB])
 {
@@ -138,14 +168,14 @@ __device__ __forceinline__ void ell_row_multi(const EllOp<T, S, UNIT> &A,
 #pragma unroll
     for (int f = 0; f < S; f++) {
       const T v = A.val[f * A.n + i];
-      const int64_t j = A.col[f * A.n + i];
+      const int64_t j = A.cj(f, i);
 #pragma unroll
       for (int c = 0; c < NB; c++) acc[c] += v * x[j * NB + c];
     }
   } else {
     for (int f = 0; f < A.s; f++) {
       const T v = A.val[f * A.n + i];
-      const int64_t j = A.col[f * A.n + i];
+      const int64_t j = A.cj(f, i);
 #pragma unroll
       for (int c = 0; c < NB; c++) acc[c] += v * x[j * NB + c];
     }
@@ -174,6 +204,24 @@ struct CsrOp {
 #pragma unroll
     for (int off = L / 2; off > 0; off >>= 1) acc += __shfl_xor(acc, off, L);
     if (valid) acc += UNIT ? x[i] : ad[i] * x[i];
+    return acc;
+  }
+  // (A (r + beta p))_i with a unit diagonal (see EllOp::row_comb)
+  __device__ __forceinline__ T row_comb(int64_t i, bool valid,
+      const T *__restrict__ r, const T *__restrict__ p, T beta) const
+  {
+    T acc = T(0);
+    if (valid) {
+      const int sub = threadIdx.x & (L - 1);
+      const int e = ptr[i + 1];
+      for (int k = ptr[i] + sub; k < e; k += L) {
+        const int64_t j = col[k];
+        acc += val[k] * (r[j] + beta * p[j]);
+      }
+    }
+#pragma unroll
+    for (int off = L / 2; off > 0; off >>= 1) acc += __shfl_xor(acc, off, L);
+    if (valid) acc += r[i] + beta * p[i];
     return acc;
   }
 };
@@ -205,11 +253,12 @@ __device__ __forceinline__ void csr_row_multi(const CsrOp<T, L, UNIT> &A,
   }
 }
 
-template <typename T, int S, bool UNIT, int NB>
-__device__ __forceinline__ void row_multi(const EllOp<T, S, UNIT> &A, int64_t i,
-    bool valid, const T *__restrict__ ad, const T *__restrict__ x, T (&acc)[NB])
+template <typename T, int S, bool UNIT, bool D16, int NB>
+__device__ __forceinline__ void row_multi(const EllOp<T, S, UNIT, D16> &A,
+    int64_t i, bool valid, const T *__restrict__ ad, const T *__restrict__ x,
+    T (&acc)[NB])
 {
-  ell_row_multi<T, S, UNIT, NB>(A, i, valid, ad, x, acc);
+  ell_row_multi<T, S, UNIT, D16, NB>(A, i, valid, ad, x, acc);
 }
 
 template <typename T, int L, bool UNIT, int NB>
@@ -224,6 +273,14 @@ template <typename F>
 __host__ inline void with_op64(const sfr_mat_view *A, F &&f)
 {
   if (A->fmt == SFR_FMT_ELL) {
+    if (A->col16 && A->slots == 6) {
+      f(EllOp<double, 6, false, true>{A->val, A->col, A->ad, A->n, 6, A->col16});
+      return;
+    }
+    if (A->col16 && A->slots == 4) {
+      f(EllOp<double, 4, false, true>{A->val, A->col, A->ad, A->n, 4, A->col16});
+      return;
+    }
     switch (A->slots) {
     case 6: f(EllOp<double, 6, false>{A->val, A->col, A->ad, A->n, 6}); return;
     case 4: f(EllOp<double, 4, false>{A->val, A->col, A->ad, A->n, 4}); return;
@@ -244,6 +301,14 @@ template <typename F>
 __host__ inline void with_op32(const sfr_mat_view *A, F &&f)
 {
   if (A->fmt == SFR_FMT_ELL) {
+    if (A->col16 && A->slots == 6) {
+      f(EllOp<float, 6, true, true>{A->val32, A->col, nullptr, A->n, 6, A->col16});
+      return;
+    }
+    if (A->col16 && A->slots == 4) {
+      f(EllOp<float, 4, true, true>{A->val32, A->col, nullptr, A->n, 4, A->col16});
+      return;
+    }
     switch (A->slots) {
     case 6: f(EllOp<float, 6, true>{A->val32, A->col, nullptr, A->n, 6}); return;
     case 4: f(EllOp<float, 4, true>{A->val32, A->col, nullptr, A->n, 4}); return;

@@ -18,12 +18,17 @@
 //   beta = r.r / (r.r before the fold)
 // Unlike restarted refinement, the search direction survives the fold, so
 // the FP32 loop keeps CG's convergence rate.
-// Per FP32 iteration (three kernels):
-//   M1 k_mx_spmv   q = A^ p; partials p.q             matrix + 2 words
-//   M2 k_mx_resid  alpha; r -= alpha q; partials r.r   3 words
-//   M3 k_mx_dir    beta; y += alpha p; p = r + beta p  5 words
-// A fold is requested by M3 (flag `ru`); the host sees it at its next poll
-// and enqueues the fold kernels R1-R3.
+// Per FP32 iteration (two kernels; the direction lives in two buffers that
+// alternate, pb[k & 1] -> pb[(k + 1) & 1], and the inner r.r in dd[k & 1]):
+//   MA k_mx_a   beta = r.r / r.r_prev; fold decision; y += alpha p_old;
+//               p_new = r + beta p_old (own rows, stored);
+//               q = A^ p_new, formed as A^ (r + beta p_old) in the gathers;
+//               partials p_new.q                    matrix + 6 words
+//   MB k_mx_b   alpha = r.r / p.q; r -= alpha q; partials r.r    3 words
+// The first MA after a fold (flag `fresh`) finds the direction already
+// built and only forms q = A^ p.  A fold is requested by MA (or a
+// breakdown by MB) through the flag `ru`, which only the host reads; it
+// sees it at its next poll and enqueues the fold kernels R1-R3.
 #include "dev_common.hpp"
 
 #define REGION(part, k) ((part) + (size_t)(k) * 4 * SFR_MAXG)
@@ -45,6 +50,9 @@ __global__ __launch_bounds__(SFR_BLOCK) void k_mx_start(const double *pin,
     sc->iter = 0;
     sc->outer = 0;
     sc->ru = 0;
+    sc->fresh = 0;
+    sc->run5 = 0;
+    sc->pbuf = 1;
     sc->done = norm_b == 0.0 ? SFR_DONE_CONV : SFR_DONE_RUN;
   }
 }
@@ -69,14 +77,14 @@ __global__ __launch_bounds__(SFR_BLOCK) void k_mx_fold(int64_t n,
   }
 }
 
-// R2: r64 = b - A x in FP64; r = D^-1/2 r64 in FP32 (and p = r, y = 0 on the
-// first call); partials of r64.D^-1 r64, r64.r64, r.r, r.p
+// R2: r64 = b - A x in FP64; r = D^-1/2 r64 in FP32 (and p_new = r, y = 0 on
+// the first call); partials of r64.D^-1 r64, r64.r64, r.r, r.p_old
 template <class Op, int RPT, bool ZERO_X>
 __global__ __launch_bounds__(SFR_BLOCK) void k_mx_true(Op A,
     const double *__restrict__ b, const double *__restrict__ x,
     const double *__restrict__ dinv, const double *__restrict__ dscale,
-    float *__restrict__ r, float *__restrict__ p, float *__restrict__ y,
-    int init, double *pout, sfr_scalars *sc)
+    float *__restrict__ r, const float *po, float *pn,
+    float *__restrict__ y, int init, double *pout, sfr_scalars *sc)
 {
   if (blockIdx.x == 0 && threadIdx.x == 0)
     sc->run2 = sc->done ? 0u : ((init || sc->ru == 2) ? 2u : 1u);
@@ -99,10 +107,10 @@ __global__ __launch_bounds__(SFR_BLOCK) void k_mx_true(Op A,
         acc[1] += ri * ri;
         acc[2] += (double)ci * (double)ci;
         if (init) {
-          p[i] = ci;
+          pn[i] = ci;
           y[i] = 0.0f;
         } else {
-          acc[3] += (double)ci * (double)p[i];
+          acc[3] += (double)ci * (double)po[i];
         }
       }
     }
@@ -128,10 +136,11 @@ __global__ __launch_bounds__(SFR_BLOCK) void k_mx_copy(int64_t n,
   }
 }
 
-// R3: FP64 stop test; p = r + beta (p - (r.p/r.r) r) after a fold
+// R3: FP64 stop test; p_new = r + beta (p_old - (r.p/r.r) r) after a fold
+// (p_old and p_new may be the same buffer: elementwise, in place)
 template <int RPT>
 __global__ __launch_bounds__(SFR_BLOCK) void k_mx_restart(int64_t n,
-    const float *__restrict__ r, float *__restrict__ p, const double *pin,
+    const float *__restrict__ r, const float *po, float *pn, const double *pin,
     int gin, int init, sfr_scalars *sc)
 {
   const unsigned mode = sc->run2;          // 0 skip, 1 fold, 2 restart (p = r)
@@ -148,9 +157,11 @@ __global__ __launch_bounds__(SFR_BLOCK) void k_mx_restart(int64_t n,
     if (conv) sc->done = SFR_DONE_CONV;
     else if (brk) sc->done = SFR_DONE_BREAK;
     else if (sc->iter >= sc->maxit) sc->done = SFR_DONE_MAXIT;
-    sc->d = rr;
+    sc->dd[1] = rr;                        // read by MB at parity 0
     sc->rho_max = rr;
     sc->ru = 0;
+    sc->fresh = 1;
+    sc->run5 = 1;
   }
   if (conv || brk || mode == 2) return;
   const double beta = rr / sc->d_prev;
@@ -163,48 +174,97 @@ __global__ __launch_bounds__(SFR_BLOCK) void k_mx_restart(int64_t n,
       const int64_t i = c * RPC + r_ * SFR_BLOCK + threadIdx.x;
       if (i < n) {
         const float ri = r[i];
-        p[i] = ri + fb * (p[i] - fg * ri);
+        pn[i] = ri + fb * (po[i] - fg * ri);
       }
     }
   }
 }
 
-// M1: q = A^ p (unit diagonal), partials p.q
+// MA: the fused first kernel of an FP32 iteration (see the file comment).
+// Reads run5 (set by MB), never ru; writes run4, ru, iter, dd[par ^ 1].
 template <class Op, int RPT>
-__global__ __launch_bounds__(SFR_BLOCK) void k_mx_spmv(Op A,
-    const float *__restrict__ p, float *__restrict__ q, double *pout,
-    sfr_scalars *sc)
+__global__ __launch_bounds__(SFR_BLOCK) void k_mx_a(Op A,
+    const float *__restrict__ r, const float *__restrict__ po,
+    float *__restrict__ pn, float *__restrict__ q, float *__restrict__ y,
+    const double *pin, int gin, double *pout, int par, sfr_scalars *sc)
 {
   const bool first = blockIdx.x == 0 && threadIdx.x == 0;
-  if (sc->done || sc->ru) {
+  if (sc->done || !sc->run5) {
     if (first) sc->run4 = 0;
     return;
   }
-  if (first) sc->run4 = 1;
   constexpr int L = Op::L, RPC = RPT * (SFR_BLOCK / L);
   const bool lead = (threadIdx.x % L) == 0;
   double acc[1] = {0.0};
+  if (sc->fresh) {
+    // first iteration after a fold: the direction is already in pn
+    if (first) sc->run4 = 1;
+    for (int64_t c = xcd_chunk(blockIdx.x, gridDim.x); c * RPC < A.n;
+         c += gridDim.x) {
+#pragma unroll
+      for (int r_ = 0; r_ < RPT; r_++) {
+        const int64_t i = c * RPC + r_ * (SFR_BLOCK / L) + threadIdx.x / L;
+        const bool valid = i < A.n;
+        const float qi = A.row(i, valid, pn);
+        if (valid && lead) {
+          q[i] = qi;
+          acc[0] += (double)qi * (double)pn[i];
+        }
+      }
+    }
+    store_partials<1>(acc, pout);
+    return;
+  }
+  double t[1];
+  reduce_partials<1>(pin, gin, t);
+  const double rr = t[0], rr_old = sc->dd[par];
+  const double conv = sc->conv;
+  const bool fold = rr < sc->delta2 * sc->rho_max || rr <= conv * conv ||
+                    !isfinite(rr);
+  const float fa = (float)sc->alpha, fb = (float)(rr / rr_old);
+  if (first) {
+    const unsigned it = sc->iter + 1;
+    sc->iter = it;
+    sc->dd[par ^ 1] = rr;
+    if (fold || it >= sc->maxit) {
+      sc->ru = 1;
+      sc->run4 = 0;
+      sc->pbuf = fold ? (unsigned)par : (unsigned)(par ^ 1);
+      sc->d_prev = rr_old;
+    } else {
+      sc->run4 = 1;
+    }
+  }
   for (int64_t c = xcd_chunk(blockIdx.x, gridDim.x); c * RPC < A.n;
        c += gridDim.x) {
 #pragma unroll
     for (int r_ = 0; r_ < RPT; r_++) {
       const int64_t i = c * RPC + r_ * (SFR_BLOCK / L) + threadIdx.x / L;
       const bool valid = i < A.n;
-      const float qi = A.row(i, valid, p);
+      if (fold) {
+        if (valid && lead) y[i] += fa * po[i];
+        continue;
+      }
+      const float qi = A.row_comb(i, valid, r, po, fb);
       if (valid && lead) {
+        const float pi = po[i];
+        const float pni = r[i] + fb * pi;
+        y[i] += fa * pi;
+        pn[i] = pni;
         q[i] = qi;
-        acc[0] += (double)qi * (double)p[i];
+        acc[0] += (double)qi * (double)pni;
       }
     }
   }
-  store_partials<1>(acc, pout);
+  if (!fold) store_partials<1>(acc, pout);
 }
 
-// M2: alpha = r.r / p.q; r -= alpha q; partials r.r
+// MB: alpha = r.r / p.q; r -= alpha q; partials r.r.  Reads run4 and
+// dd[par ^ 1]; writes run5, alpha, rho_max, fresh (and ru on a breakdown).
 template <int RPT>
-__global__ __launch_bounds__(SFR_BLOCK) void k_mx_resid(int64_t n,
+__global__ __launch_bounds__(SFR_BLOCK) void k_mx_b(int64_t n,
     const float *__restrict__ q, float *__restrict__ r, const double *pin,
-    int gin, double *pout, sfr_scalars *sc)
+    int gin, double *pout, int par, sfr_scalars *sc)
 {
   const bool first = blockIdx.x == 0 && threadIdx.x == 0;
   if (!sc->run4) {
@@ -213,17 +273,21 @@ __global__ __launch_bounds__(SFR_BLOCK) void k_mx_resid(int64_t n,
   }
   double pq[1];
   reduce_partials<1>(pin, gin, pq);
-  const double d = sc->d;
-  const double alpha = d / pq[0];
+  const double rr = sc->dd[par ^ 1];
+  const double alpha = rr / pq[0];
   if (!(pq[0] > 0.0) || !isfinite(alpha)) {
     // FP32 lost positive definiteness: restart from the true residual
-    if (first) { sc->ru = 2; sc->run5 = 0; }
+    if (first) {
+      sc->ru = 2;
+      sc->run5 = 0;
+      sc->pbuf = (unsigned)(par ^ 1);
+    }
     return;
   }
   if (first) {
     sc->alpha = alpha;
-    sc->d_prev = d;
-    if (d > sc->rho_max) sc->rho_max = d;
+    if (rr > sc->rho_max) sc->rho_max = rr;
+    sc->fresh = 0;
     sc->run5 = 1;
   }
   const float fa = (float)alpha;
@@ -244,41 +308,6 @@ __global__ __launch_bounds__(SFR_BLOCK) void k_mx_resid(int64_t n,
   store_partials<1>(acc, pout);
 }
 
-// M3: beta = r.r / d; y += alpha p; p = r + beta p unless a fold is due
-template <int RPT>
-__global__ __launch_bounds__(SFR_BLOCK) void k_mx_dir(int64_t n,
-    const float *__restrict__ r, float *__restrict__ y,
-    float *__restrict__ p, const double *pin, int gin, sfr_scalars *sc)
-{
-  if (!sc->run5) return;
-  double t[1];
-  reduce_partials<1>(pin, gin, t);
-  const double rr = t[0];
-  const double conv = sc->conv;
-  const bool fold = rr < sc->delta2 * sc->rho_max || rr <= conv * conv ||
-                    !isfinite(rr);
-  const float fa = (float)sc->alpha, fb = (float)(rr / sc->d_prev);
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
-    const unsigned it = sc->iter + 1;
-    sc->iter = it;
-    sc->d = rr;
-    if (fold || it >= sc->maxit) sc->ru = 1;
-  }
-  constexpr int RPC = RPT * SFR_BLOCK;
-  for (int64_t c = xcd_chunk(blockIdx.x, gridDim.x); c * RPC < n;
-       c += gridDim.x) {
-#pragma unroll
-    for (int r_ = 0; r_ < RPT; r_++) {
-      const int64_t i = c * RPC + r_ * SFR_BLOCK + threadIdx.x;
-      if (i < n) {
-        const float pi = p[i];
-        y[i] += fa * pi;
-        if (!fold) p[i] = r[i] + fb * pi;
-      }
-    }
-  }
-}
-
 extern "C" int sfr_k_mx_start(hipStream_t s, const double *part, int nparts,
                               double tol, unsigned maxit, double delta,
                               sfr_scalars *sc)
@@ -289,11 +318,12 @@ extern "C" int sfr_k_mx_start(hipStream_t s, const double *part, int nparts,
   return hipGetLastError();
 }
 
-// fold kernels R1-R3 (init: first call -- no fold, p = r, y = 0)
+// fold kernels R1-R3 (init: first call -- no fold, p_new = r, y = 0)
 extern "C" int sfr_k_mx_fold(hipStream_t s, const sfr_mat_view *A,
                              sfr_geom g0, const double *b, double *x,
-                             float *y, float *r, float *p, int init,
-                             int zero_x, sfr_scalars *sc, double *part)
+                             float *y, float *r, const float *p_old,
+                             float *p_new, int init, int zero_x,
+                             sfr_scalars *sc, double *part)
 {
   const sfr_geom gv = {g0.rpt, grid_for(A->n, g0.rpt, 1)};
   const sfr_geom gm = {g0.rpt, grid_for(A->n, g0.rpt, op_lanes(A))};
@@ -307,50 +337,57 @@ extern "C" int sfr_k_mx_fold(hipStream_t s, const sfr_mat_view *A,
       if (zero_x && init)                                                      \
         hipLaunchKernelGGL((k_mx_true<Op, R, true>), dim3(gm.grid),            \
                            dim3(SFR_BLOCK), 0, s, op, b, x, A->dinv,           \
-                           A->dscale, r, p, y, init, rc, sc);                  \
+                           A->dscale, r, p_old, p_new, y, init, rc, sc);       \
       else                                                                     \
         hipLaunchKernelGGL((k_mx_true<Op, R, false>), dim3(gm.grid),           \
                            dim3(SFR_BLOCK), 0, s, op, b, x, A->dinv,           \
-                           A->dscale, r, p, y, init, rc, sc);                  \
+                           A->dscale, r, p_old, p_new, y, init, rc, sc);       \
       break;
     CASE(1) CASE(2) CASE(4)
     default: CASE(8)
 #undef CASE
     }
   });
-  SFR_LAUNCH_V(k_mx_copy, gv, s, A->n, (const float *)r, p, init, sc);
-  SFR_LAUNCH_V(k_mx_restart, gv, s, A->n, (const float *)r, p,
+  SFR_LAUNCH_V(k_mx_copy, gv, s, A->n, (const float *)r, p_new, init, sc);
+  SFR_LAUNCH_V(k_mx_restart, gv, s, A->n, (const float *)r, p_old, p_new,
                (const double *)rc, gm.grid, init, sc);
   return hipGetLastError();
 }
 
 extern "C" int sfr_k_mx_inner_iter(hipStream_t s, const sfr_mat_view *A,
-                                   sfr_geom g0, float *y, float *r, float *p,
-                                   float *q, sfr_scalars *sc, double *part)
+                                   sfr_geom g0, float *y, float *r,
+                                   float *const pb[2], float *q, int par,
+                                   sfr_scalars *sc, double *part)
 {
   const sfr_geom gv = {g0.rpt, grid_for(A->n, g0.rpt, 1)};
   const sfr_geom gm = {g0.rpt, grid_for(A->n, g0.rpt, op_lanes(A))};
   double *ra = REGION(part, 0), *rb = REGION(part, 1);
+  const float *po = pb[par];
+  float *pn = pb[par ^ 1];
   with_op32(A, [&](auto op) {
     using Op = decltype(op);
-    SFR_LAUNCH_RPT(k_mx_spmv, Op, gm, s, op, p, q, ra, sc);
+    SFR_LAUNCH_RPT(k_mx_a, Op, gm, s, op, (const float *)r, po, pn, q, y,
+                   (const double *)rb, gv.grid, ra, par, sc);
   });
-  SFR_LAUNCH_V(k_mx_resid, gv, s, A->n, q, r, (const double *)ra, gm.grid, rb,
-               sc);
-  SFR_LAUNCH_V(k_mx_dir, gv, s, A->n, r, y, p, (const double *)rb, gv.grid,
-               sc);
+  SFR_LAUNCH_V(k_mx_b, gv, s, A->n, (const float *)q, r, (const double *)ra,
+               gm.grid, rb, par, sc);
   return hipGetLastError();
 }
 
-// the FP32 SpMV + p.q kernel of one inner iteration alone (measurement hook)
-extern "C" int sfr_k_mx_spmv_only(hipStream_t s, const sfr_mat_view *A,
-                                  sfr_geom g0, const float *p, float *q,
-                                  sfr_scalars *sc, double *part)
+// MA alone (measurement hook); the caller sets the scalars and the r.r
+// partials so that every launch takes the fused path
+extern "C" int sfr_k_mx_a_only(hipStream_t s, const sfr_mat_view *A,
+                               sfr_geom g0, const float *r, const float *po,
+                               float *pn, float *q, float *y, sfr_scalars *sc,
+                               double *part)
 {
+  const sfr_geom gv = {g0.rpt, grid_for(A->n, g0.rpt, 1)};
   const sfr_geom gm = {g0.rpt, grid_for(A->n, g0.rpt, op_lanes(A))};
   with_op32(A, [&](auto op) {
     using Op = decltype(op);
-    SFR_LAUNCH_RPT(k_mx_spmv, Op, gm, s, op, p, q, REGION(part, 0), sc);
+    SFR_LAUNCH_RPT(k_mx_a, Op, gm, s, op, r, po, pn, q, y,
+                   (const double *)REGION(part, 1), gv.grid, REGION(part, 0),
+                   0, sc);
   });
   return hipGetLastError();
 }

@@ -80,6 +80,13 @@ __device__ __forceinline__ double *slots_of(double *part, const UbSys &S)
   return part + 4 * (size_t)S.gfirst;
 }
 
+// column of slot f of row i in an ELL system (16-bit offsets when present)
+__device__ __forceinline__ int64_t ell_col(const UbSys &S, int f, int64_t i)
+{
+  const int64_t k = f * (int64_t)S.n + i;
+  return S.col16 ? i + (int64_t)S.col16[k] : (int64_t)S.col[k];
+}
+
 // (A^ x)_i of the unit-diagonal FP32 operator of system S
 __device__ __forceinline__ float row32(const UbSys &S, int64_t i,
                                        const float *__restrict__ x)
@@ -89,9 +96,9 @@ __device__ __forceinline__ float row32(const UbSys &S, int64_t i,
     const int64_t n = S.n;
     if (S.slots == 6) {
 #pragma unroll
-      for (int f = 0; f < 6; f++) acc += S.val32[f * n + i] * x[S.col[f * n + i]];
+      for (int f = 0; f < 6; f++) acc += S.val32[f * n + i] * x[ell_col(S, f, i)];
     } else {
-      for (int f = 0; f < S.slots; f++) acc += S.val32[f * n + i] * x[S.col[f * n + i]];
+      for (int f = 0; f < S.slots; f++) acc += S.val32[f * n + i] * x[ell_col(S, f, i)];
     }
   } else {
     const int32_t e = S.rowptr[i + 1];
@@ -133,16 +140,27 @@ __device__ __forceinline__ void rows32(const UbSys &S, const Rows &R,
   if (S.fmt == SFR_FMT_ELL && S.slots == 6) {
     const int64_t n = S.n;
     const float *__restrict__ val = S.val32;
-    const int32_t *__restrict__ col = S.col;
-    int32_t cj[UB_U][6];
+    int64_t cj[UB_U][6];
     float a[UB_U][6];
+    if (S.col16) {
+      const int16_t *__restrict__ d16 = S.col16;
 #pragma unroll
-    for (int u = 0; u < UB_U; u++)
+      for (int u = 0; u < UB_U; u++)
 #pragma unroll
-      for (int f = 0; f < 6; f++) {
-        cj[u][f] = col[f * n + R.i[u]];
-        a[u][f] = val[f * n + R.i[u]];
-      }
+        for (int f = 0; f < 6; f++) {
+          cj[u][f] = R.i[u] + (int64_t)d16[f * n + R.i[u]];
+          a[u][f] = val[f * n + R.i[u]];
+        }
+    } else {
+      const int32_t *__restrict__ col = S.col;
+#pragma unroll
+      for (int u = 0; u < UB_U; u++)
+#pragma unroll
+        for (int f = 0; f < 6; f++) {
+          cj[u][f] = col[f * n + R.i[u]];
+          a[u][f] = val[f * n + R.i[u]];
+        }
+    }
 #pragma unroll
     for (int u = 0; u < UB_U; u++) {
       float acc = x[R.i[u]];
@@ -173,7 +191,7 @@ __device__ __forceinline__ double row64_fold(const UbSys &S, int64_t i,
   if (S.fmt == SFR_FMT_ELL) {
     const int64_t n = S.n;
     for (int f = 0; f < S.slots; f++)
-      acc += S.val[f * n + i] * xval(S, S.col[f * n + i], x, y);
+      acc += S.val[f * n + i] * xval(S, ell_col(S, f, i), x, y);
   } else {
     const int32_t e = S.rowptr[i + 1];
     for (int32_t k = S.rowptr[i]; k < e; k++) acc += S.val[k] * xval(S, S.col[k], x, y);
@@ -188,7 +206,7 @@ __device__ __forceinline__ double row64(const UbSys &S, int64_t i,
   double acc = S.ad[i] * x[i];
   if (S.fmt == SFR_FMT_ELL) {
     const int64_t n = S.n;
-    for (int f = 0; f < S.slots; f++) acc += S.val[f * n + i] * x[S.col[f * n + i]];
+    for (int f = 0; f < S.slots; f++) acc += S.val[f * n + i] * x[ell_col(S, f, i)];
   } else {
     const int32_t e = S.rowptr[i + 1];
     for (int32_t k = S.rowptr[i]; k < e; k++) acc += S.val[k] * x[S.col[k]];

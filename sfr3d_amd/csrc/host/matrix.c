@@ -18,6 +18,7 @@
  */
 #include <stdarg.h>
 #include <stdio.h>
+#include <stdint.h>
 #include <stdlib.h>
 #include <string.h>
 #include "../sfr_internal.h"
@@ -184,6 +185,7 @@ sfr_mat_view sfr_view(const sfr_matrix *A)
   v.n = A->n; v.fmt = A->fmt; v.slots = A->slots; v.lanes = A->lanes;
   v.col = A->col; v.rowptr = A->rowptr; v.val = A->val; v.ad = A->ad;
   v.dinv = A->dinv; v.val32 = A->val32; v.dscale = A->dscale;
+  v.col16 = A->col16;
   return v;
 }
 
@@ -297,7 +299,7 @@ void sfr_matrix_destroy(sfr_context *ctx, sfr_matrix *A)
     ctx->ngraph = w;
   }
   if (A->T) sfr_matrix_destroy(ctx, A->T);
-  hipFree(A->col); hipFree(A->rowptr); hipFree(A->vmap); hipFree(A->val);
+  hipFree(A->col); hipFree(A->col16); hipFree(A->rowptr); hipFree(A->vmap); hipFree(A->val);
   hipFree(A->dscale); hipFree(A->val32); hipFree(A->grad_rc);
   if (!A->borrowed) {
     hipFree(A->dmap); hipFree(A->raw); hipFree(A->raw_ad); hipFree(A->ad);
@@ -363,6 +365,27 @@ static int upload_format(sfr_context *ctx, sfr_matrix *A, int32_t n,
   MTRY(hipMalloc((void **)&A->vmap, sizeof(int32_t) * (size_t)(A->nnz + 1)));
   MTRY(hipMalloc((void **)&A->val, sizeof(double) * (size_t)(A->nnz + 1)));
   MTRY(hipMemcpyAsync(A->col, hcol, sizeof(int32_t) * (size_t)A->nnz, hipMemcpyHostToDevice, ctx->stream));
+  if (fmt == SFR_FMT_ELL) {
+    /* 16-bit offsets from the row when they all fit: half the index stream */
+    int fits = 1;
+    for (int64_t k = 0; k < A->nnz && fits; k++) {
+      const int64_t d = (int64_t)hcol[k] - (int64_t)(k % n);
+      if (d < INT16_MIN || d > INT16_MAX) fits = 0;
+    }
+    if (fits && A->nnz > 0) {
+      int16_t *h16 = (int16_t *)malloc(sizeof(int16_t) * (size_t)A->nnz);
+      if (!h16) { rc = sfr_set_error(ctx, SFR_BAD_ARG, "host allocation"); goto done; }
+      for (int64_t k = 0; k < A->nnz; k++) h16[k] = (int16_t)((int64_t)hcol[k] - (k % n));
+      hipError_t e1 = hipMalloc((void **)&A->col16, sizeof(int16_t) * (size_t)A->nnz);
+      if (e1 == hipSuccess)
+        e1 = hipMemcpy(A->col16, h16, sizeof(int16_t) * (size_t)A->nnz, hipMemcpyHostToDevice);
+      free(h16);
+      if (e1 != hipSuccess) {
+        rc = sfr_set_error(ctx, SFR_HIP_ERROR, "col16 upload: %s", hipGetErrorString(e1));
+        goto done;
+      }
+    }
+  }
   MTRY(hipMemcpyAsync(A->vmap, hmap, sizeof(int32_t) * (size_t)A->nnz, hipMemcpyHostToDevice, ctx->stream));
   if (hptr) {
     MTRY(hipMalloc((void **)&A->rowptr, sizeof(int32_t) * ((size_t)n + 1)));
@@ -560,6 +583,12 @@ int sfr_matrix_tune(sfr_context *ctx, sfr_matrix *A, int rpt, int lanes)
   A->rpt = rpt;
   if (A->fmt == SFR_FMT_CSR) A->lanes = lanes;
   return SFR_OK;
+}
+
+int sfr_matrix_index_bytes(const sfr_matrix *A)
+{
+  if (!A) return 0;
+  return A->col16 ? 2 : 4;
 }
 
 int sfr_matrix_info(const sfr_matrix *A, int32_t *neq, int *format, int *slots,

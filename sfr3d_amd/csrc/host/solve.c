@@ -37,7 +37,9 @@ static int get_graph(sfr_context *ctx, int kind, const void *k0,
                      const void *k1, const void *k2, const sfr_mat_view *v,
                      sfr_geom g, hipGraphExec_t *exec)
 {
-  const int chunk = ctx->chunk;
+  /* the mixed loop alternates its direction buffers: an even chunk ends
+   * where it started */
+  const int chunk = kind == G_MX ? (ctx->chunk + 1) & ~1 : ctx->chunk;
   for (int i = 0; i < ctx->ngraph; i++) {
     sfr_graph *e = &ctx->graphs[i];
     if (e->kind == kind && e->chunk == chunk && e->key[0] == k0 &&
@@ -63,8 +65,11 @@ static int get_graph(sfr_context *ctx, int kind, const void *k0,
     else if (kind == G_BICG)
       lrc = sfr_k_bicg_iter(s, v, g, (double *)k1, w[0], w[1], w[2], w[3], w[4], w[5],
                             ctx->d_sc, ctx->part);
-    else if (kind == G_MX)
-      lrc = sfr_k_mx_inner_iter(s, v, g, w32[0], w32[1], w32[2], w32[3], ctx->d_sc, ctx->part);
+    else if (kind == G_MX) {
+      float *const pb[2] = {w32[2], w32[4]};
+      lrc = sfr_k_mx_inner_iter(s, v, g, w32[0], w32[1], pb, w32[3], it & 1, ctx->d_sc,
+                                ctx->part);
+    }
     else                    /* G_BB: k2 carries the batch size */
       lrc = sfr_k_bb_iter(s, v, g, (int)(intptr_t)k2, w[8], w[9], w[0], w[1], w[2], w[3],
                           w[4], w[5], w[6], ctx->d_sc, ctx->part);
@@ -216,8 +221,9 @@ static int solve_pcg_mixed(sfr_context *ctx, sfr_matrix *A, double tol,
   int nparts = 0;
   TRY(sfr_k_pcg_norm(s, &v, g, d_b, d_x, newX, ctx->part, &nparts));
   TRY(sfr_k_mx_start(s, ctx->part, nparts, tol, maxit, ctx->ru_delta, ctx->d_sc));
-  TRY(sfr_k_mx_fold(s, &v, g, d_b, d_x, w32[0], w32[1], w32[2], 1, newX, ctx->d_sc,
-                    ctx->part));
+  /* the direction for the first iteration of a chunk goes to w32[4] */
+  TRY(sfr_k_mx_fold(s, &v, g, d_b, d_x, w32[0], w32[1], w32[4], w32[4], 1, newX,
+                    ctx->d_sc, ctx->part));
   /* every fold is triggered by a residual drop of delta (or the stop test):
    * bound the number of folds generously */
   for (int folds = 0; folds < 100000; folds++) {
@@ -226,8 +232,9 @@ static int solve_pcg_mixed(sfr_context *ctx, sfr_matrix *A, double tol,
     TRY(hipStreamSynchronize(s));
     if (ctx->h_poll[0].done != SFR_DONE_RUN) break;
     if (!ctx->h_poll[0].ru) break;        /* loop stopped with no request */
-    TRY(sfr_k_mx_fold(s, &v, g, d_b, d_x, w32[0], w32[1], w32[2], 0, 0, ctx->d_sc,
-                      ctx->part));
+    const float *p_old = ctx->h_poll[0].pbuf ? w32[4] : w32[2];
+    TRY(sfr_k_mx_fold(s, &v, g, d_b, d_x, w32[0], w32[1], p_old, w32[4], 0, 0,
+                      ctx->d_sc, ctx->part));
   }
   return finish(ctx, &v, g, d_b, d_x, st);
 }
@@ -371,18 +378,37 @@ int sfr_kernel_time(sfr_context *ctx, sfr_matrix *A, int which, int reps,
   sfr_mat_view v = sfr_view(A);
   sfr_geom g = {A->rpt, 0};
   hipStream_t s = ctx->stream;
-  TRY(hipMemsetAsync(ctx->d_sc, 0, sizeof(sfr_scalars), s));
-  TRY(sfr_k_fill(s, A->n, 1.0, ctx->w64[1]));
-  TRY(sfr_k_fill32(s, A->n, 1.0f, ctx->w32[2]));
-  for (int warm = 0; warm < 2; warm++) {
-    if (which == 0) TRY(sfr_k_pcg_spmv_only(s, &v, g, ctx->w64[1], ctx->w64[2], ctx->d_sc, ctx->part));
-    else TRY(sfr_k_mx_spmv_only(s, &v, g, ctx->w32[2], ctx->w32[3], ctx->d_sc, ctx->part));
+  float *const *w32 = ctx->w32;
+  if (which == 0) {
+    TRY(hipMemsetAsync(ctx->d_sc, 0, sizeof(sfr_scalars), s));
+    TRY(sfr_k_fill(s, A->n, 1.0, ctx->w64[1]));
+  } else {
+    /* the fused FP32 kernel in its steady state: running, not fresh, no
+     * fold due (delta 0, tol 0), alpha 0 (y unchanged), r.r partials 1 */
+    sfr_scalars st;
+    memset(&st, 0, sizeof st);
+    st.run5 = 1;
+    st.dd[0] = st.dd[1] = 1.0;
+    st.maxit = 0x7fffffffu;
+    TRY(hipMemcpyAsync(ctx->d_sc, &st, sizeof st, hipMemcpyHostToDevice, s));
+    TRY(sfr_k_fill(s, 4 * SFR_MAXG, 1.0, ctx->part + 4 * SFR_MAXG));
+    TRY(sfr_k_fill32(s, A->n, 1.0f, w32[1]));
+    TRY(sfr_k_fill32(s, A->n, 1.0f, w32[2]));
+    TRY(sfr_k_fill32(s, A->n, 0.0f, w32[0]));
   }
+#define LAUNCH()                                                               \
+  do {                                                                         \
+    if (which == 0)                                                            \
+      TRY(sfr_k_pcg_spmv_only(s, &v, g, ctx->w64[1], ctx->w64[2], ctx->d_sc,   \
+                              ctx->part));                                     \
+    else                                                                       \
+      TRY(sfr_k_mx_a_only(s, &v, g, w32[1], w32[2], w32[4], w32[3], w32[0],    \
+                          ctx->d_sc, ctx->part));                              \
+  } while (0)
+  for (int warm = 0; warm < 2; warm++) LAUNCH();
   TRY(hipEventRecord(ctx->ev_start, s));
-  for (int r = 0; r < reps; r++) {
-    if (which == 0) TRY(sfr_k_pcg_spmv_only(s, &v, g, ctx->w64[1], ctx->w64[2], ctx->d_sc, ctx->part));
-    else TRY(sfr_k_mx_spmv_only(s, &v, g, ctx->w32[2], ctx->w32[3], ctx->d_sc, ctx->part));
-  }
+  for (int r = 0; r < reps; r++) LAUNCH();
+#undef LAUNCH
   TRY(hipEventRecord(ctx->ev_stop, s));
   TRY(hipEventSynchronize(ctx->ev_stop));
   float ms = 0.0f;

@@ -35,6 +35,7 @@ typedef struct sfr_scalars {
   double   rho_max;    /* mixed: largest inner r.r since the last update    */
   double   delta2;     /* mixed: reliable-update threshold, squared         */
   double   res[6];     /* final / outer dots                                */
+  double   dd[2];      /* mixed: inner r.r, written alternately by parity   */
   /* polled by the host (8 words from iter) */
   unsigned iter;       /* completed iterations                              */
   unsigned done;       /* SFR_DONE_*                                        */
@@ -43,7 +44,9 @@ typedef struct sfr_scalars {
   unsigned ru;         /* mixed: reliable update requested                  */
   unsigned run2, run3, run4;  /* run flags handed from kernel to kernel     */
   unsigned run5;
-  unsigned pad[7];
+  unsigned fresh;      /* mixed: the direction was just rebuilt by a fold   */
+  unsigned pbuf;       /* mixed: direction buffer current at a fold request */
+  unsigned pad[5];
 } sfr_scalars;
 
 enum { SFR_DONE_RUN = 0, SFR_DONE_CONV = 1, SFR_DONE_BREAK = 2,
@@ -62,6 +65,7 @@ typedef struct sfr_mat_view {
   const double  *dinv;
   const float   *val32;     /* D^-1/2 A D^-1/2 off-diagonals (unit diag) */
   const double  *dscale;    /* 1/sqrt(ad)                                */
+  const int16_t *col16;     /* ELL columns as 16-bit row offsets, or NULL */
 } sfr_mat_view;
 
 typedef struct sfr_geom { int rpt; int grid; } sfr_geom;
@@ -77,6 +81,7 @@ struct sfr_matrix {
   int      have32;       /* FP32 scaled copy is current                    */
   int      rpt;          /* tuned rows per thread                          */
   int32_t *col, *rowptr, *vmap, *dmap;
+  int16_t *col16;        /* ELL: col - row when every offset fits 16 bits  */
   double  *raw, *raw_ad, *val, *ad, *dinv, *dscale;
   float   *val32;
   const int32_t *h_ia, *h_ja;   /* host pattern identity (value-only uploads) */
@@ -137,9 +142,6 @@ int sfr_k_fill32(hipStream_t s, int64_t n, float v, float *x);
 int sfr_k_pcg_spmv_only(hipStream_t s, const sfr_mat_view *A, sfr_geom g,
                         const double *p, double *q, sfr_scalars *sc,
                         double *part);
-int sfr_k_mx_spmv_only(hipStream_t s, const sfr_mat_view *A, sfr_geom g,
-                       const float *p, float *q, sfr_scalars *sc,
-                       double *part);
 
 /* PCG, FP64 (reference src/Cg.c:47-209) */
 int sfr_k_pcg_setup(hipStream_t s, const sfr_mat_view *A, sfr_geom g,
@@ -201,11 +203,18 @@ int sfr_k_mx_start(hipStream_t s, const double *part, int nparts, double tol,
 /* fold y into x, true FP64 residual, stop test, new direction; init = first
  * call (p = r, y = 0), zero_x = x is known to be 0 */
 int sfr_k_mx_fold(hipStream_t s, const sfr_mat_view *A, sfr_geom g,
-                  const double *b, double *x, float *y, float *r, float *p,
-                  int init, int zero_x, sfr_scalars *sc, double *part);
+                  const double *b, double *x, float *y, float *r,
+                  const float *p_old, float *p_new, int init, int zero_x,
+                  sfr_scalars *sc, double *part);
+/* one FP32 iteration at position parity par of a captured chunk: the
+ * direction is read from pb[par] and written to pb[par ^ 1] */
 int sfr_k_mx_inner_iter(hipStream_t s, const sfr_mat_view *A, sfr_geom g,
-                        float *y, float *r, float *p, float *q,
-                        sfr_scalars *sc, double *part);
+                        float *y, float *r, float *const pb[2], float *q,
+                        int par, sfr_scalars *sc, double *part);
+/* the fused SpMV kernel of the FP32 iteration alone (measurement hook) */
+int sfr_k_mx_a_only(hipStream_t s, const sfr_mat_view *A, sfr_geom g,
+                    const float *r, const float *po, float *pn, float *q,
+                    float *y, sfr_scalars *sc, double *part);
 
 /* ---- uneven batch: independent systems, mixed-precision PCG ------------- */
 #define SFR_UB_MAXSYS 1024       /* systems of one uneven batch             */
@@ -214,6 +223,7 @@ int sfr_k_mx_inner_iter(hipStream_t s, const sfr_mat_view *A, sfr_geom g,
  * (either format) and where its rows sit in the batch vectors. */
 typedef struct sfr_ub_sys {
   const int32_t *col, *rowptr;
+  const int16_t *col16;     /* ELL: 16-bit row offsets, or NULL             */
   const double  *val, *ad, *dinv, *dscale;
   const float   *val32;
   int64_t        row0;      /* first row in the batch vectors               */

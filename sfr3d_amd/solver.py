@@ -93,7 +93,8 @@ class DeviceMatrix:
         lib.sfr_matrix_info(self.h, C.byref(n), C.byref(f), C.byref(s),
                             C.byref(nnz))
         return {"neq": n.value, "format": f.value, "slots": s.value,
-                "nnz": nnz.value}
+                "nnz": nnz.value,
+                "index_bytes": int(lib.sfr_matrix_index_bytes(self.h))}
 
     def destroy(self):
         if self.h:
