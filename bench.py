@@ -26,6 +26,11 @@ import numpy as np  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
 TOL = 1e-8
+# HBM bytes per launch of k_mx_a on the 128^3 workload, from two rocprofv3
+# counter runs of `bench.py --profile-kernels` (FETCH_SIZE x 2.000 and
+# WRITE_SIZE x 1.000, the factors a 256 MiB copy gives): 115.2 MB read +
+# 25.2 MB written
+K_MX_A_TRAFFIC_128 = 140.4e6
 
 
 def log(*a):
@@ -183,7 +188,8 @@ def main():
                                "updates, p.q)",
                      "achieved": round(gbs32, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(gbs32 / HBM_PEAK_GBS, 4),
-                     "traffic": None,
+                     "traffic": K_MX_A_TRAFFIC_128 if n == 128 else None,
+                     "traffic_unit": "bytes per launch (rocprofv3 PMC)",
                      "launch_us": round(t_k32 * 1e6, 2),
                      "bytes_per_launch": b32, "index_bytes": ib,
                      "fp64_spmv": {"achieved": round(gbs64, 1),
