@@ -100,7 +100,20 @@ __device__ __forceinline__ void reduce_partials(const double *__restrict__ part,
 // the slot count at compile time, S == 0 reads it at run time.  UNIT = the
 // diagonal is 1 (Jacobi-scaled copy) and ad is not read.  D16 = the columns
 // are read as 16-bit offsets from the row, dcol[f*n + i] = col - i (half the
-// index traffic; used when every offset fits).
+// index traffic).  The few entries whose offset does not fit hold
+// SFR_D16_ESC: the gather loop reads the row's own x there (a select, no
+// branch), and a row that has one adds a_k (x[col] - x[i]) afterwards, with
+// the column from a short exception list (k = f*n + i, col).
+__device__ __forceinline__ int64_t d16_col(const int16_t *__restrict__ dcol,
+    const int64_t *xk, const int32_t *xc, int64_t nx, int64_t k, int64_t i)
+{
+  const int d = dcol[k];
+  if (d != SFR_D16_ESC) return i + d;
+  for (int64_t e = 0; e < nx; e++)
+    if (xk[e] == k) return xc[e];
+  return i;                    // not reached: every escape is listed
+}
+
 template <typename T, int S, bool UNIT, bool D16 = false>
 struct EllOp {
   static constexpr int L = 1;
@@ -110,43 +123,78 @@ struct EllOp {
   int64_t n;
   int s;
   const int16_t *dcol;
+  const int64_t *xk;
+  const int32_t *xc;
+  int64_t nx;
+  // column of slot f of row i (exact, escapes resolved; set-up paths)
   __device__ __forceinline__ int64_t cj(int f, int64_t i) const
   {
-    if constexpr (D16) return i + (int64_t)dcol[f * n + i];
+    if constexpr (D16) return d16_col(dcol, xk, xc, nx, f * n + i, i);
     else return col[f * n + i];
+  }
+  // column the gather loop reads: the row itself for an escaped entry
+  __device__ __forceinline__ int64_t cg(int f, int64_t i, bool &esc) const
+  {
+    if constexpr (D16) {
+      const int d = dcol[f * n + i];
+      const bool e = d == SFR_D16_ESC;
+      esc |= e;
+      return e ? i : i + d;
+    } else {
+      return col[f * n + i];
+    }
+  }
+  // sum over the escaped entries of row i of a_k (v(col) - v(i))
+  template <class V>
+  __device__ T esc_fix(int64_t i, V v) const
+  {
+    T acc = T(0);
+    const T vi = v(i);
+    for (int64_t e = 0; e < nx; e++) {
+      const int64_t k = xk[e];
+      if (k % n == i) acc += val[k] * (v((int64_t)xc[e]) - vi);
+    }
+    return acc;
   }
   __device__ __forceinline__ T row(int64_t i, bool valid, const T *__restrict__ x) const
   {
     if (!valid) return T(0);
     T acc = UNIT ? x[i] : ad[i] * x[i];
+    bool esc = false;
     if constexpr (S > 0) {
 #pragma unroll
-      for (int f = 0; f < S; f++) acc += val[f * n + i] * x[cj(f, i)];
+      for (int f = 0; f < S; f++) acc += val[f * n + i] * x[cg(f, i, esc)];
     } else {
-      for (int f = 0; f < s; f++) acc += val[f * n + i] * x[cj(f, i)];
+      for (int f = 0; f < s; f++) acc += val[f * n + i] * x[cg(f, i, esc)];
     }
+    if constexpr (D16)
+      if (esc) acc += esc_fix(i, [&](int64_t j) { return x[j]; });
     return acc;
   }
   // (A (r + beta p))_i with a unit diagonal: the combination of each
   // neighbour is formed exactly as that neighbour's own row forms it, so the
-  // result is the product with the updated vector, bit for bit
+  // result is the product with the updated vector, bit for bit (rows with an
+  // escaped entry: up to the rounding of the fix-up)
   __device__ __forceinline__ T row_comb(int64_t i, bool valid,
       const T *__restrict__ r, const T *__restrict__ p, T beta) const
   {
     if (!valid) return T(0);
     T acc = r[i] + beta * p[i];
+    bool esc = false;
     if constexpr (S > 0) {
 #pragma unroll
       for (int f = 0; f < S; f++) {
-        const int64_t j = cj(f, i);
+        const int64_t j = cg(f, i, esc);
         acc += val[f * n + i] * (r[j] + beta * p[j]);
       }
     } else {
       for (int f = 0; f < s; f++) {
-        const int64_t j = cj(f, i);
+        const int64_t j = cg(f, i, esc);
         acc += val[f * n + i] * (r[j] + beta * p[j]);
       }
     }
+    if constexpr (D16)
+      if (esc) acc += esc_fix(i, [&](int64_t j) { return r[j] + beta * p[j]; });
     return acc;
   }
 };
@@ -164,22 +212,29 @@ __device__ __forceinline__ void ell_row_multi(const EllOp<T, S, UNIT, D16> &A,
   if (!valid) return;
 #pragma unroll
   for (int c = 0; c < NB; c++) acc[c] = ad[i * NB + c] * x[i * NB + c];
+  bool esc = false;
   if constexpr (S > 0) {
 #pragma unroll
     for (int f = 0; f < S; f++) {
       const T v = A.val[f * A.n + i];
-      const int64_t j = A.cj(f, i);
+      const int64_t j = A.cg(f, i, esc);
 #pragma unroll
       for (int c = 0; c < NB; c++) acc[c] += v * x[j * NB + c];
     }
   } else {
     for (int f = 0; f < A.s; f++) {
       const T v = A.val[f * A.n + i];
-      const int64_t j = A.cj(f, i);
+      const int64_t j = A.cg(f, i, esc);
 #pragma unroll
       for (int c = 0; c < NB; c++) acc[c] += v * x[j * NB + c];
     }
   }
+  if constexpr (D16)
+    if (esc) {
+#pragma unroll
+      for (int c = 0; c < NB; c++)
+        acc[c] += A.esc_fix(i, [&](int64_t j) { return x[j * NB + c]; });
+    }
 }
 
 // Full-row CSR (off-diagonals only, diagonal apart); L consecutive lanes per
@@ -274,11 +329,11 @@ __host__ inline void with_op64(const sfr_mat_view *A, F &&f)
 {
   if (A->fmt == SFR_FMT_ELL) {
     if (A->col16 && A->slots == 6) {
-      f(EllOp<double, 6, false, true>{A->val, A->col, A->ad, A->n, 6, A->col16});
+      f(EllOp<double, 6, false, true>{A->val, A->col, A->ad, A->n, 6, A->col16, A->exc_k, A->exc_col, A->nexc});
       return;
     }
     if (A->col16 && A->slots == 4) {
-      f(EllOp<double, 4, false, true>{A->val, A->col, A->ad, A->n, 4, A->col16});
+      f(EllOp<double, 4, false, true>{A->val, A->col, A->ad, A->n, 4, A->col16, A->exc_k, A->exc_col, A->nexc});
       return;
     }
     switch (A->slots) {
@@ -302,11 +357,11 @@ __host__ inline void with_op32(const sfr_mat_view *A, F &&f)
 {
   if (A->fmt == SFR_FMT_ELL) {
     if (A->col16 && A->slots == 6) {
-      f(EllOp<float, 6, true, true>{A->val32, A->col, nullptr, A->n, 6, A->col16});
+      f(EllOp<float, 6, true, true>{A->val32, A->col, nullptr, A->n, 6, A->col16, A->exc_k, A->exc_col, A->nexc});
       return;
     }
     if (A->col16 && A->slots == 4) {
-      f(EllOp<float, 4, true, true>{A->val32, A->col, nullptr, A->n, 4, A->col16});
+      f(EllOp<float, 4, true, true>{A->val32, A->col, nullptr, A->n, 4, A->col16, A->exc_k, A->exc_col, A->nexc});
       return;
     }
     switch (A->slots) {

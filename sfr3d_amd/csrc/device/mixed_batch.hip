@@ -84,7 +84,8 @@ __device__ __forceinline__ double *slots_of(double *part, const UbSys &S)
 __device__ __forceinline__ int64_t ell_col(const UbSys &S, int f, int64_t i)
 {
   const int64_t k = f * (int64_t)S.n + i;
-  return S.col16 ? i + (int64_t)S.col16[k] : (int64_t)S.col[k];
+  return S.col16 ? d16_col(S.col16, S.exc_k, S.exc_col, S.nexc, k, i)
+                 : (int64_t)S.col[k];
 }
 
 // (A^ x)_i of the unit-diagonal FP32 operator of system S
@@ -142,13 +143,18 @@ __device__ __forceinline__ void rows32(const UbSys &S, const Rows &R,
     const float *__restrict__ val = S.val32;
     int64_t cj[UB_U][6];
     float a[UB_U][6];
+    bool esc[UB_U] = {};
     if (S.col16) {
+      // escaped offsets read the row itself here and are fixed up below
       const int16_t *__restrict__ d16 = S.col16;
 #pragma unroll
       for (int u = 0; u < UB_U; u++)
 #pragma unroll
         for (int f = 0; f < 6; f++) {
-          cj[u][f] = R.i[u] + (int64_t)d16[f * n + R.i[u]];
+          const int d = d16[f * n + R.i[u]];
+          const bool e = d == SFR_D16_ESC;
+          esc[u] |= e;
+          cj[u][f] = e ? R.i[u] : R.i[u] + d;
           a[u][f] = val[f * n + R.i[u]];
         }
     } else {
@@ -166,6 +172,13 @@ __device__ __forceinline__ void rows32(const UbSys &S, const Rows &R,
       float acc = x[R.i[u]];
 #pragma unroll
       for (int f = 0; f < 6; f++) acc += a[u][f] * x[cj[u][f]];
+      if (esc[u]) {
+        const float xi = x[R.i[u]];
+        for (int64_t e = 0; e < S.nexc; e++) {
+          const int64_t k = S.exc_k[e];
+          if (k % n == R.i[u]) acc += val[k] * (x[S.exc_col[e]] - xi);
+        }
+      }
       out[u] = acc;
     }
   } else {

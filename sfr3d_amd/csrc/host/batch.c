@@ -92,6 +92,9 @@ static void describe(sfr_ub_sys *d, const sfr_matrix *A, int64_t row0,
   memset(d, 0, sizeof *d);
   d->col = A->col;
   d->col16 = A->fmt == SFR_FMT_ELL ? A->col16 : NULL;
+  d->exc_k = A->exc_k;
+  d->exc_col = A->exc_col;
+  d->nexc = A->nexc;
   d->rowptr = A->fmt == SFR_FMT_CSR ? A->rowptr : NULL;
   d->val = A->val; d->ad = A->ad; d->dinv = A->dinv;
   d->dscale = A->dscale; d->val32 = A->val32;

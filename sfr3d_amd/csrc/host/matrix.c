@@ -186,6 +186,7 @@ sfr_mat_view sfr_view(const sfr_matrix *A)
   v.col = A->col; v.rowptr = A->rowptr; v.val = A->val; v.ad = A->ad;
   v.dinv = A->dinv; v.val32 = A->val32; v.dscale = A->dscale;
   v.col16 = A->col16;
+  v.exc_k = A->exc_k; v.exc_col = A->exc_col; v.nexc = A->nexc;
   return v;
 }
 
@@ -299,7 +300,8 @@ void sfr_matrix_destroy(sfr_context *ctx, sfr_matrix *A)
     ctx->ngraph = w;
   }
   if (A->T) sfr_matrix_destroy(ctx, A->T);
-  hipFree(A->col); hipFree(A->col16); hipFree(A->rowptr); hipFree(A->vmap); hipFree(A->val);
+  hipFree(A->col); hipFree(A->col16); hipFree(A->exc_k); hipFree(A->exc_col);
+  hipFree(A->rowptr); hipFree(A->vmap); hipFree(A->val);
   hipFree(A->dscale); hipFree(A->val32); hipFree(A->grad_rc);
   if (!A->borrowed) {
     hipFree(A->dmap); hipFree(A->raw); hipFree(A->raw_ad); hipFree(A->ad);
@@ -366,24 +368,50 @@ static int upload_format(sfr_context *ctx, sfr_matrix *A, int32_t n,
   MTRY(hipMalloc((void **)&A->val, sizeof(double) * (size_t)(A->nnz + 1)));
   MTRY(hipMemcpyAsync(A->col, hcol, sizeof(int32_t) * (size_t)A->nnz, hipMemcpyHostToDevice, ctx->stream));
   if (fmt == SFR_FMT_ELL) {
-    /* 16-bit offsets from the row when they all fit: half the index stream */
-    int fits = 1;
-    for (int64_t k = 0; k < A->nnz && fits; k++) {
+    /* 16-bit offsets from the row: half the index stream.  Offsets outside
+     * +-32767 (e.g. a slot facing a cell without an equation, which SFR3D's
+     * ELLPACK points at column 0) are escaped and listed, if they are few. */
+    int64_t nexc = 0;
+    for (int64_t k = 0; k < A->nnz && nexc <= SFR_D16_MAXEXC; k++) {
       const int64_t d = (int64_t)hcol[k] - (int64_t)(k % n);
-      if (d < INT16_MIN || d > INT16_MAX) fits = 0;
+      if (d < -32767 || d > 32767) nexc++;
     }
-    if (fits && A->nnz > 0) {
+    if (nexc <= SFR_D16_MAXEXC && A->nnz > 0) {
       int16_t *h16 = (int16_t *)malloc(sizeof(int16_t) * (size_t)A->nnz);
-      if (!h16) { rc = sfr_set_error(ctx, SFR_BAD_ARG, "host allocation"); goto done; }
-      for (int64_t k = 0; k < A->nnz; k++) h16[k] = (int16_t)((int64_t)hcol[k] - (k % n));
+      int64_t *hxk = (int64_t *)malloc(sizeof(int64_t) * (size_t)(nexc + 1));
+      int32_t *hxc = (int32_t *)malloc(sizeof(int32_t) * (size_t)(nexc + 1));
+      if (!h16 || !hxk || !hxc) {
+        free(h16); free(hxk); free(hxc);
+        rc = sfr_set_error(ctx, SFR_BAD_ARG, "host allocation");
+        goto done;
+      }
+      int64_t e = 0;
+      for (int64_t k = 0; k < A->nnz; k++) {
+        const int64_t d = (int64_t)hcol[k] - (k % n);
+        if (d < -32767 || d > 32767) {
+          h16[k] = (int16_t)SFR_D16_ESC;
+          hxk[e] = k;
+          hxc[e] = hcol[k];
+          e++;
+        } else {
+          h16[k] = (int16_t)d;
+        }
+      }
       hipError_t e1 = hipMalloc((void **)&A->col16, sizeof(int16_t) * (size_t)A->nnz);
       if (e1 == hipSuccess)
         e1 = hipMemcpy(A->col16, h16, sizeof(int16_t) * (size_t)A->nnz, hipMemcpyHostToDevice);
-      free(h16);
+      if (e1 == hipSuccess) e1 = hipMalloc((void **)&A->exc_k, sizeof(int64_t) * (size_t)(nexc + 1));
+      if (e1 == hipSuccess) e1 = hipMalloc((void **)&A->exc_col, sizeof(int32_t) * (size_t)(nexc + 1));
+      if (e1 == hipSuccess && nexc)
+        e1 = hipMemcpy(A->exc_k, hxk, sizeof(int64_t) * (size_t)nexc, hipMemcpyHostToDevice);
+      if (e1 == hipSuccess && nexc)
+        e1 = hipMemcpy(A->exc_col, hxc, sizeof(int32_t) * (size_t)nexc, hipMemcpyHostToDevice);
+      free(h16); free(hxk); free(hxc);
       if (e1 != hipSuccess) {
         rc = sfr_set_error(ctx, SFR_HIP_ERROR, "col16 upload: %s", hipGetErrorString(e1));
         goto done;
       }
+      A->nexc = (int)nexc;
     }
   }
   MTRY(hipMemcpyAsync(A->vmap, hmap, sizeof(int32_t) * (size_t)A->nnz, hipMemcpyHostToDevice, ctx->stream));

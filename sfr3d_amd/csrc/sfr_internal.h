@@ -21,6 +21,8 @@ extern "C" {
 #define SFR_GRIDCAP 2048         /* max workgroups of a streaming kernel    */
 #define SFR_MAXDOT  8            /* partial sums one kernel may reduce      */
 #define SFR_MAXSYS  8            /* systems of one shared-pattern batch     */
+#define SFR_D16_ESC (-32768)     /* 16-bit column offset: see the exceptions */
+#define SFR_D16_MAXEXC 256       /* most exceptions a 16-bit ELL may have   */
 
 /* Scalars of one Krylov solve, resident on the device.  The last workgroup
  * of each reducing kernel writes them; later kernels read them. */
@@ -66,6 +68,9 @@ typedef struct sfr_mat_view {
   const float   *val32;     /* D^-1/2 A D^-1/2 off-diagonals (unit diag) */
   const double  *dscale;    /* 1/sqrt(ad)                                */
   const int16_t *col16;     /* ELL columns as 16-bit row offsets, or NULL */
+  const int64_t *exc_k;     /* entries whose offset is SFR_D16_ESC: their  */
+  const int32_t *exc_col;   /*   slot index f*n+i and column               */
+  int            nexc;
 } sfr_mat_view;
 
 typedef struct sfr_geom { int rpt; int grid; } sfr_geom;
@@ -81,7 +86,10 @@ struct sfr_matrix {
   int      have32;       /* FP32 scaled copy is current                    */
   int      rpt;          /* tuned rows per thread                          */
   int32_t *col, *rowptr, *vmap, *dmap;
-  int16_t *col16;        /* ELL: col - row when every offset fits 16 bits  */
+  int16_t *col16;        /* ELL: col - row, SFR_D16_ESC for the few that   */
+  int64_t *exc_k;        /*   do not fit 16 bits, listed in exc_k/exc_col  */
+  int32_t *exc_col;
+  int      nexc;
   double  *raw, *raw_ad, *val, *ad, *dinv, *dscale;
   float   *val32;
   const int32_t *h_ia, *h_ja;   /* host pattern identity (value-only uploads) */
@@ -224,6 +232,9 @@ int sfr_k_mx_a_only(hipStream_t s, const sfr_mat_view *A, sfr_geom g,
 typedef struct sfr_ub_sys {
   const int32_t *col, *rowptr;
   const int16_t *col16;     /* ELL: 16-bit row offsets, or NULL             */
+  const int64_t *exc_k;     /* escaped offsets: slot index and column       */
+  const int32_t *exc_col;
+  int64_t        nexc;
   const double  *val, *ad, *dinv, *dscale;
   const float   *val32;
   int64_t        row0;      /* first row in the batch vectors               */

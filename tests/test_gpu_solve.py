@@ -137,3 +137,42 @@ def test_deterministic(ctx):
     x1, s1 = ctx.solve(B.system, S.PCG, TOL, 5000, B.b)
     x2, s2 = ctx.solve(B.system, S.PCG, TOL, 5000, B.b)
     assert s1["iters"] == s2["iters"] and np.array_equal(x1, x2)
+
+
+def test_ell_16bit_offsets_with_escaped_far_columns(ctx):
+    """SFR3D's ELLPACK points a slot facing a cell without an equation at
+    column 0 (src/EllPack.c:54-60), and with the reference's assembly
+    (ell_quirk) that slot carries the face coefficient.  In rows far from 0
+    the offset does not fit 16 bits: those entries are escaped to an
+    exception list while the rest of the matrix keeps 16-bit offsets.  The
+    solve must see the reference's operator, column-0 couplings included."""
+    from sfr3d_amd import structure as ST
+    from sfr3d_amd.solver import System
+    B = systems.poisson3d(34, seed=6, storage=S.ELLPACK, reorder=False,
+                          pinned_cell=34 ** 3 - 500)
+    al, ad, b = ST.assemble(B.pattern, B.id, B.nelcon, B.nviz, B.extra["lA"],
+                            B.extra["lB"], ell_quirk=True)
+    s0 = B.system
+    # 5 % heavier diagonal: the extra column-0 couplings leave the
+    # all-Neumann system near-singular otherwise
+    sq = System(s0.neq, s0.nad, s0.ia, s0.ja, al, 1.05 * ad, S.ELLPACK, True)
+    far = [k for k in range(len(s0.ja)) if s0.ja[k] == 0 and k // 6 > 32767]
+    assert far and np.any(al[far] != 0.0)         # far, nonzero couplings exist
+    A = ctx.matrix(sq)
+    assert A.info()["index_bytes"] == 2
+    bb = b[:s0.neq].copy()
+    db, dx = ctx.array(bb), ctx.empty(s0.neq)
+    st = ctx.solve_device(A, S.PBICGSTAB, TOL, 5000, db, dx)
+    xo, so = oracle.pbicgstab(sq, bb, TOL, 5000)
+    assert st["status"] == S._lib.OK
+    assert abs(st["iters"] - so["iters"]) <= 2
+    assert relerr(dx.get(), xo) < XTOL
+    A.destroy(); db.free(); dx.free()
+    # the plain assembly (zero in those slots) through the mixed path
+    A = ctx.matrix(s0)
+    assert A.info()["index_bytes"] == 2
+    db, dx = ctx.array(B.b), ctx.empty(s0.neq)
+    st = ctx.solve_device(A, S.PCG, TOL, 20000, db, dx, precision=S.PREC_MIXED)
+    xo, _ = oracle.pcg(s0, B.b, TOL, 20000)
+    assert st["status"] == S._lib.OK and relerr(dx.get(), xo) < XTOL
+    A.destroy(); db.free(); dx.free()

@@ -135,3 +135,34 @@ def test_uneven_device_form(ctx):
         assert sts[s]["status"] == S._lib.OK
         assert relerr(x[r0:r0 + b.system.neq], xo) < XTOL
     Bt.destroy(); db.free(); dx.free()
+
+
+def test_uneven_batch_escaped_far_columns(ctx):
+    """A batch member whose device ELL keeps 16-bit column offsets with
+    escaped far entries: SFR3D's ELLPACK points the slot facing the pinned
+    cell at column 0 (src/EllPack.c:54-60), and with the reference's
+    assembly (ell_quirk) that slot carries the face coefficient.  The batch
+    must solve the reference's operator, far column-0 couplings included
+    (the coupling is tiny next to the 5 % heavier diagonal, so CG still
+    converges; the FP64 BiCGSTAB oracle solves the same system)."""
+    from sfr3d_amd import structure as ST
+    from sfr3d_amd.solver import System
+    B = systems.poisson3d(34, seed=6, storage=S.ELLPACK, reorder=False,
+                          pinned_cell=34 ** 3 - 500)
+    al, ad, b = ST.assemble(B.pattern, B.id, B.nelcon, B.nviz, B.extra["lA"],
+                            B.extra["lB"], ell_quirk=True)
+    s0 = B.system
+    sq = System(s0.neq, s0.nad, s0.ia, s0.ja, al, 1.05 * ad, S.ELLPACK, True)
+    bq = b[:s0.neq].copy()
+    small = systems.uneven_batch(2, seed=4, smin=6, smax=10)
+    sysl = [sq] + [x.system for x in small]
+    bs = [bq] + [x.b for x in small]
+    Bt = ctx.batch(sysl, tune=False)
+    xs, sts = Bt.solve(bs, TOL, 20000)
+    xo, so = oracle.pbicgstab(sq, bq, 1e-10, 20000)
+    assert sts[0]["status"] == S._lib.OK
+    assert relerr(xs[0], xo) < XTOL
+    for k in (1, 2):
+        xk, _ = oracle.pcg(sysl[k], bs[k], TOL, 20000)
+        assert relerr(xs[k], xk) < XTOL
+    Bt.destroy()
