@@ -68,7 +68,8 @@ __device__ __forceinline__ void store_partials(double (&v)[NV], double *part)
 // a fixed order by the whole workgroup and handed to every thread.  Every
 // workgroup that calls it gets bitwise the same totals, so scalars derived
 // from them (alpha, beta, stop tests) agree across the grid without any
-// grid-wide synchronisation.
+// grid-wide synchronisation.  Kernels issue their own first loads before the
+// call, so that the partials' latency and theirs overlap.
 template <int NV>
 __device__ __forceinline__ void reduce_partials(const double *__restrict__ part,
                                                 int g, double (&out)[NV])

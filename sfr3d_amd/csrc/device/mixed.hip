@@ -271,6 +271,17 @@ __global__ __launch_bounds__(SFR_BLOCK) void k_mx_b(int64_t n,
     if (first) sc->run5 = 0;
     return;
   }
+  constexpr int RPC = RPT * SFR_BLOCK;
+  // the first chunk's rows are loaded before the reduction, whose latency
+  // they then hide
+  const int64_t c0 = xcd_chunk(blockIdx.x, gridDim.x);
+  float rv[RPT], qv[RPT];
+#pragma unroll
+  for (int r_ = 0; r_ < RPT; r_++) {
+    const int64_t i = c0 * RPC + r_ * SFR_BLOCK + threadIdx.x;
+    rv[r_] = i < n ? r[i] : 0.0f;
+    qv[r_] = i < n ? q[i] : 0.0f;
+  }
   double pq[1];
   reduce_partials<1>(pin, gin, pq);
   const double rr = sc->dd[par ^ 1];
@@ -291,15 +302,21 @@ __global__ __launch_bounds__(SFR_BLOCK) void k_mx_b(int64_t n,
     sc->run5 = 1;
   }
   const float fa = (float)alpha;
-  constexpr int RPC = RPT * SFR_BLOCK;
   double acc[1] = {0.0};
-  for (int64_t c = xcd_chunk(blockIdx.x, gridDim.x); c * RPC < n;
-       c += gridDim.x) {
+  for (int64_t c = c0; c * RPC < n; c += gridDim.x) {
+    if (c != c0) {
+#pragma unroll
+      for (int r_ = 0; r_ < RPT; r_++) {
+        const int64_t i = c * RPC + r_ * SFR_BLOCK + threadIdx.x;
+        rv[r_] = i < n ? r[i] : 0.0f;
+        qv[r_] = i < n ? q[i] : 0.0f;
+      }
+    }
 #pragma unroll
     for (int r_ = 0; r_ < RPT; r_++) {
       const int64_t i = c * RPC + r_ * SFR_BLOCK + threadIdx.x;
       if (i < n) {
-        const float ri = r[i] - fa * q[i];
+        const float ri = rv[r_] - fa * qv[r_];
         r[i] = ri;
         acc[0] += (double)ri * (double)ri;
       }

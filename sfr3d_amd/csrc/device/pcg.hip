@@ -251,6 +251,17 @@ __global__ __launch_bounds__(SFR_BLOCK) void k_pcg_resid(int64_t n,
     if (first) sc->run3 = 0;
     return;
   }
+  constexpr int RPC = RPT * SFR_BLOCK;
+  // the first chunk's rows are loaded before the reduction (hides it)
+  const int64_t c0 = xcd_chunk(blockIdx.x, gridDim.x);
+  double rv[RPT], qv[RPT], dv[RPT];
+#pragma unroll
+  for (int r_ = 0; r_ < RPT; r_++) {
+    const int64_t i = c0 * RPC + r_ * SFR_BLOCK + threadIdx.x;
+    rv[r_] = i < n ? r[i] : 0.0;
+    qv[r_] = i < n ? q[i] : 0.0;
+    dv[r_] = i < n ? dinv[i] : 0.0;
+  }
   double pq[1];
   reduce_partials<1>(pin, gin, pq);
   const double d = sc->d;
@@ -260,17 +271,24 @@ __global__ __launch_bounds__(SFR_BLOCK) void k_pcg_resid(int64_t n,
     return;
   }
   if (first) { sc->alpha = alpha; sc->d_prev = d; sc->run3 = 1; }
-  constexpr int RPC = RPT * SFR_BLOCK;
   double acc[1] = {0.0};
-  for (int64_t c = xcd_chunk(blockIdx.x, gridDim.x); c * RPC < n;
-       c += gridDim.x) {
+  for (int64_t c = c0; c * RPC < n; c += gridDim.x) {
+    if (c != c0) {
+#pragma unroll
+      for (int r_ = 0; r_ < RPT; r_++) {
+        const int64_t i = c * RPC + r_ * SFR_BLOCK + threadIdx.x;
+        rv[r_] = i < n ? r[i] : 0.0;
+        qv[r_] = i < n ? q[i] : 0.0;
+        dv[r_] = i < n ? dinv[i] : 0.0;
+      }
+    }
 #pragma unroll
     for (int r_ = 0; r_ < RPT; r_++) {
       const int64_t i = c * RPC + r_ * SFR_BLOCK + threadIdx.x;
       if (i < n) {
-        const double ri = r[i] - alpha * q[i];
+        const double ri = rv[r_] - alpha * qv[r_];
         r[i] = ri;
-        acc[0] += ri * (ri * dinv[i]);
+        acc[0] += ri * (ri * dv[r_]);
       }
     }
   }
@@ -284,6 +302,17 @@ __global__ __launch_bounds__(SFR_BLOCK) void k_pcg_dir(int64_t n,
     int gin, sfr_scalars *sc)
 {
   if (!sc->run3) return;
+  constexpr int RPC = RPT * SFR_BLOCK;
+  // the first chunk's rows are loaded before the reduction (hides it)
+  const int64_t c0 = xcd_chunk(blockIdx.x, gridDim.x);
+  double pv[RPT], xv[RPT], zv[RPT];
+#pragma unroll
+  for (int r_ = 0; r_ < RPT; r_++) {
+    const int64_t i = c0 * RPC + r_ * SFR_BLOCK + threadIdx.x;
+    pv[r_] = i < n ? p[i] : 0.0;
+    xv[r_] = i < n ? x[i] : 0.0;
+    zv[r_] = i < n ? r[i] * dinv[i] : 0.0;
+  }
   double rz[1];
   reduce_partials<1>(pin, gin, rz);
   const double di = rz[0];
@@ -296,16 +325,22 @@ __global__ __launch_bounds__(SFR_BLOCK) void k_pcg_dir(int64_t n,
     else if (!isfinite(di)) sc->done = SFR_DONE_BREAK;
     else if (it >= sc->maxit) sc->done = SFR_DONE_MAXIT;
   }
-  constexpr int RPC = RPT * SFR_BLOCK;
-  for (int64_t c = xcd_chunk(blockIdx.x, gridDim.x); c * RPC < n;
-       c += gridDim.x) {
+  for (int64_t c = c0; c * RPC < n; c += gridDim.x) {
+    if (c != c0) {
+#pragma unroll
+      for (int r_ = 0; r_ < RPT; r_++) {
+        const int64_t i = c * RPC + r_ * SFR_BLOCK + threadIdx.x;
+        pv[r_] = i < n ? p[i] : 0.0;
+        xv[r_] = i < n ? x[i] : 0.0;
+        zv[r_] = i < n ? r[i] * dinv[i] : 0.0;
+      }
+    }
 #pragma unroll
     for (int r_ = 0; r_ < RPT; r_++) {
       const int64_t i = c * RPC + r_ * SFR_BLOCK + threadIdx.x;
       if (i < n) {
-        const double pi = p[i];
-        x[i] += alpha * pi;
-        p[i] = r[i] * dinv[i] + beta * pi;
+        x[i] = xv[r_] + alpha * pv[r_];
+        p[i] = zv[r_] + beta * pv[r_];
       }
     }
   }
