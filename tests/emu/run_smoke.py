@@ -1,0 +1,78 @@
+"""TEST INFRASTRUCTURE -- runs the emulated library (argv[1]) on the small
+cases of argv[2] (written by tests/test_emulated_kernels.py with the
+oracle's solutions) in a process of its own: FP64 and mixed PCG, BiCGSTAB,
+the shared-pattern batch, both adjoint paths, 16-bit column offsets with
+escaped far columns, and a tuned uneven batch.  Prints PASS/FAIL lines and
+exits non-zero on any failure."""
+import ctypes as C, sys, os, numpy as np
+lib = C.CDLL(sys.argv[1])
+z = np.load(sys.argv[2])
+class Sys(C.Structure):
+    _fields_ = [("neq", C.c_int32), ("nad", C.c_int32), ("ia", C.c_void_p), ("ja", C.c_void_p),
+                ("al", C.c_void_p), ("ad", C.c_void_p), ("au", C.c_void_p), ("storage", C.c_short), ("unsym", C.c_short)]
+class St(C.Structure):
+    _fields_ = [("status", C.c_int), ("iters", C.c_uint), ("outer", C.c_uint)] + [(k, C.c_double) for k in
+               ("tol", "norm_b", "conv", "xKx", "norm_x", "norm_r", "norm_r_m", "relres", "time_s")]
+vp = C.c_void_p
+lib.sfr_last_error.restype = C.c_char_p
+lib.sfr_solve_device.argtypes = [vp, vp, C.c_int, C.c_int, C.c_double, C.c_uint, vp, vp, C.c_int, vp]
+lib.sfr_solve_adjoint.argtypes = [vp, vp, C.c_int, C.c_double, C.c_uint, vp, vp, vp, vp, vp, vp]
+lib.sfr_solve_shared_device.argtypes = [vp, vp, C.c_int, vp, vp, vp, C.c_int, C.c_double, C.c_uint, C.c_int, vp]
+lib.sfr_batch_solve.argtypes = [vp, vp, C.c_double, C.c_uint, vp, vp, C.c_int, vp]
+ctx = vp(); assert lib.sfr_init(C.byref(ctx), 0) == 0
+keep = []
+def mk(name):
+    a = [np.ascontiguousarray(z[f"{name}_{k}"]) for k in ("ia", "ja", "al", "ad")]
+    keep.append(a)
+    neq, nad, st_, un, maxit = [int(v) for v in z[f"{name}_meta"]]
+    return Sys(neq, nad, *[x.ctypes.data for x in a], None, st_, un), maxit
+def rel(x, y): return np.linalg.norm(x - y) / np.linalg.norm(y)
+fails = 0
+def check(tag, ok):
+    global fails
+    print(("PASS " if ok else "FAIL ") + tag, flush=True); fails += (not ok)
+# Poisson: FP64, mixed, adjoint
+s, maxit = mk("pois"); A = vp(); assert lib.sfr_matrix_create(ctx, C.byref(s), 0, C.byref(A)) == 0
+b = np.ascontiguousarray(z["pois_b"]); xo = z["pois_xo"]
+for prec in (0, 1):
+    x = np.zeros(s.neq); st = St()
+    rc = lib.sfr_solve_device(ctx, A, 1, prec, 1e-8, maxit, b.ctypes.data, x.ctypes.data, 1, C.byref(st))
+    check(f"pcg prec {prec} rc {rc} it {st.iters} rel {rel(x, xo):.1e}", rc == 0 and rel(x, xo) < 1e-6)
+lam = np.zeros(s.neq); gal = np.zeros(len(z["pois_al"])); gad = np.zeros(s.neq); st = St()
+rc = lib.sfr_solve_adjoint(ctx, A, 1, 1e-8, maxit, x.ctypes.data, b.ctypes.data, lam.ctypes.data, gal.ctypes.data, gad.ctypes.data, C.byref(st))
+check(f"adjoint pcg rc {rc} rel {rel(lam, xo):.1e}", rc == 0 and rel(lam, xo) < 1e-6)
+lib.sfr_matrix_destroy(ctx, A)
+# momentum: BiCGSTAB, shared batch, adjoint (transposed operator)
+s, maxit = mk("mom"); A = vp(); assert lib.sfr_matrix_create(ctx, C.byref(s), 0, C.byref(A)) == 0
+bb = np.ascontiguousarray(z["mom_b"]); xo = z["mom_xo"]
+x = np.zeros(s.neq); st = St()
+rc = lib.sfr_solve_device(ctx, A, 2, 0, 1e-8, maxit, bb[0].ctypes.data, x.ctypes.data, 1, C.byref(st))
+check(f"bicg rc {rc} it {st.iters} rel {rel(x, xo):.1e}", rc == 0 and rel(x, xo) < 1e-6)
+xs = np.zeros(3 * s.neq); sts = (St * 3)(); flat = np.ascontiguousarray(bb.reshape(-1))
+rc = lib.sfr_solve_shared_device(ctx, A, 3, None, flat.ctypes.data, xs.ctypes.data, 2, 1e-8, maxit, 1, C.cast(sts, vp))
+check(f"shared rc {rc} rel {rel(xs[:s.neq], xo):.1e}", rc == 0 and rel(xs[:s.neq], x) < 1e-12)
+lam = np.zeros(s.neq); gal = np.zeros(len(z["mom_al"])); gad = np.zeros(s.neq); st = St()
+rc = lib.sfr_solve_adjoint(ctx, A, 2, 1e-8, maxit, x.ctypes.data, bb[0].ctypes.data, lam.ctypes.data, gal.ctypes.data, gad.ctypes.data, C.byref(st))
+check(f"adjoint bicg rc {rc} it {st.iters}", rc == 0 and st.iters > 0)
+lib.sfr_matrix_destroy(ctx, A)
+# far columns (escaped 16-bit offsets): 3 BiCGSTAB iterations vs the oracle's 3
+s, maxit = mk("far"); A = vp(); assert lib.sfr_matrix_create(ctx, C.byref(s), 0, C.byref(A)) == 0
+lib.sfr_matrix_index_bytes.argtypes = [vp]
+b = np.ascontiguousarray(z["far_b"]); xo = z["far_xo"]; x = np.zeros(s.neq); st = St()
+rc = lib.sfr_solve_device(ctx, A, 2, 0, 1e-300, maxit, b.ctypes.data, x.ctypes.data, 1, C.byref(st))
+check(f"far rc {rc} ib {lib.sfr_matrix_index_bytes(A)} it {st.iters} rel {rel(x, xo):.1e}",
+      lib.sfr_matrix_index_bytes(A) == 2 and rel(x, xo) < 1e-12)
+lib.sfr_matrix_destroy(ctx, A)
+# uneven batch (tuned): two copies of the Poisson system
+s1, _ = mk("pois")
+arr = (Sys * 2)(s1, s1)
+Bt = vp(); rc = lib.sfr_batch_create(ctx, 2, arr, 1, C.byref(Bt))
+b = np.ascontiguousarray(z["pois_b"]); P = vp * 2
+xs = [np.zeros(s1.neq), np.zeros(s1.neq)]; sts = (St * 2)()
+rc2 = lib.sfr_batch_solve(ctx, Bt, 1e-8, 1000, C.cast(P(b.ctypes.data, b.ctypes.data), vp),
+                          C.cast(P(xs[0].ctypes.data, xs[1].ctypes.data), vp), 1, C.cast(sts, vp))
+check(f"batch rc {rc}/{rc2} rel {rel(xs[1], z['pois_xo']):.1e}", rc == 0 and rc2 == 0 and rel(xs[1], z['pois_xo']) < 1e-6)
+lib.sfr_batch_destroy(ctx, Bt)
+lib.sfr_finalize(ctx)
+print("SMOKE", "FAILED" if fails else "OK", flush=True)
+os._exit(1 if fails else 0)
