@@ -140,6 +140,7 @@ def main():
     relerr = float(np.linalg.norm(xmx - x64) / np.linalg.norm(x64))
 
     copy_gbs = None if args.no_extra else hbm_copy_ceiling(ctx)
+    assembly = None if args.no_extra else bench_assembly(ctx, S, A, B, db)
 
     cpu = None
     if not args.no_cpu:
@@ -198,6 +199,7 @@ def main():
                                    "bytes_per_launch": b64}},
         "cpu_baseline": cpu,
         "hbm_copy_GBs": copy_gbs,
+        "device_assembly": assembly,
     }
     out.update(extra)
     print(json.dumps(out), flush=True)
@@ -265,6 +267,35 @@ def hbm_copy_ceiling(ctx, gib=2, reps=5):
     gbs = 2.0 * n * reps / (ms.value * 1e-3) / 1e9
     log(f"[bench] HBM copy ceiling ({gib} GiB D2D): {gbs:.0f} GB/s")
     return round(gbs, 1)
+
+
+def bench_assembly(ctx, S, A, B, db, reps=5):
+    """Re-assembly of the 128^3 matrix and right-hand side from per-cell
+    coefficients (the same values it holds): on the host (sfr_assemble) plus
+    the upload, against the device scatter from coefficients already on the
+    GPU."""
+    from sfr3d_amd import structure as ST
+    lA = np.ascontiguousarray(B.extra["lA"])
+    lB = np.ascontiguousarray(B.extra["lB"])
+    th, td = [], []
+    for _ in range(reps):
+        t = time.perf_counter()
+        al, ad, b = ST.assemble(B.pattern, B.id, B.nelcon, B.nviz, lA, lB)
+        A.set_values(al, ad)
+        db.set(b[:B.system.neq])
+        th.append(time.perf_counter() - t)
+    plan = ctx.assembly(A, B.pattern, B.id, B.nelcon, B.nviz)
+    d_lA, d_lB = ctx.array(lA), ctx.array(lB)
+    for _ in range(reps + 1):
+        t = time.perf_counter()
+        plan.assemble(d_lA, d_lB, db)
+        td.append(time.perf_counter() - t)
+    plan.destroy(); d_lA.free(); d_lB.free()
+    out = {"host_assemble_and_upload_ms": round(float(np.median(th)) * 1e3, 2),
+           "device_assemble_ms": round(float(np.median(td[1:])) * 1e3, 3)}
+    log(f"[bench] assembly 128^3: host + upload {out['host_assemble_and_upload_ms']} ms, "
+        f"device {out['device_assemble_ms']} ms")
+    return out
 
 
 def bench_uvw(ctx, S, systems, lib, reps=3):

@@ -219,6 +219,26 @@ int  sfr_batch_info(const sfr_batch *B, int s, int64_t *row0, int32_t *neq,
 int  sfr_batch_buckets(const sfr_batch *B, sfr_bucket_choice *out, int max);
 void sfr_batch_destroy(sfr_context *ctx, sfr_batch *B);
 
+/* ---- device assembly -------------------------------------------------------
+ * The value scatter of sfr_assemble (the reference's assbly / csr / ellPack,
+ * src/Assbly.c:43-95, src/Csr.c:549-975, src/EllPack.c:113-149) done on the
+ * device into a resident matrix, for callers whose cell loops produce the
+ * per-cell coefficients on the GPU: no values cross the host link between
+ * SIMPLE iterations.  sfr_assembly_create makes the plan once per pattern
+ * (pat, id, nelcon, nViz, numel, ell_quirk as for sfr_assemble; pat must be
+ * the pattern A was created from).  sfr_assemble_device then takes d_lA
+ * (numel*(maxViz+1) doubles, layout of sfr_assemble's lA) and optionally
+ * d_lB -> d_b (neq doubles); the stored values are bitwise those
+ * sfr_assemble writes, and the next solve on A uses them. */
+typedef struct sfr_assembly sfr_assembly;
+int  sfr_assembly_create(sfr_context *ctx, sfr_matrix *A,
+                         const sfr_pattern *pat, const int32_t *id,
+                         const int32_t *nelcon, const short *nViz, int numel,
+                         int ell_quirk, sfr_assembly **plan);
+int  sfr_assemble_device(sfr_context *ctx, sfr_assembly *plan,
+                         const double *d_lA, const double *d_lB, double *d_b);
+void sfr_assembly_destroy(sfr_context *ctx, sfr_assembly *plan);
+
 /* Measurement hook: average device time of one launch of a loop kernel,
  * HIP events on the context stream.  which = 0: FP64 PCG SpMV + p.q;
  * which = 1: the first kernel of a mixed-precision FP32 iteration (SpMV of

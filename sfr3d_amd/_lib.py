@@ -108,6 +108,11 @@ _sig("sfr_solve_adjoint", C.c_int, vp, vp, C.c_int, C.c_double, C.c_uint, vp,
 _sig("sfr_kernel_time", C.c_int, vp, vp, C.c_int, C.c_int,
      C.POINTER(C.c_double))
 _sig("sfr_set_option", C.c_int, vp, C.c_char_p, C.c_double)
+# device assembly
+_sig("sfr_assembly_create", C.c_int, vp, vp, C.POINTER(SfrPattern), i32p,
+     i32p, i16p, C.c_int, C.c_int, C.POINTER(vp))
+_sig("sfr_assemble_device", C.c_int, vp, vp, vp, vp, vp)
+_sig("sfr_assembly_destroy", None, vp, vp)
 # uneven batch
 _sig("sfr_batch_create", C.c_int, vp, C.c_int, C.POINTER(SfrSystem), C.c_int,
      C.POINTER(vp))

@@ -23,6 +23,8 @@
 #include <string.h>
 #include "../sfr_internal.h"
 
+#define CHECK_RC(call) do { int rc_ = (call); if (rc_ != SFR_OK) return rc_; } while (0)
+
 int sfr_set_error(sfr_context *ctx, int code, const char *fmt, ...)
 {
   if (ctx) {
@@ -582,10 +584,19 @@ int sfr_matrix_set_values(sfr_context *ctx, sfr_matrix *A, const double *al,
   }
   if (!A->diag_in_al)
     TRY(hipMemcpyAsync(A->raw_ad, ad, sizeof(double) * (size_t)A->n, hipMemcpyHostToDevice, ctx->stream));
+  CHECK_RC(sfr_matrix_refresh(ctx, A));
+  TRY(hipStreamSynchronize(ctx->stream));
+  return SFR_OK;
+}
+
+/* The raw values (and raw diagonal) on the device changed: rebuild the
+ * device-format values, the diagonal and its inverse, and mark the FP32
+ * copy stale.  Enqueued on the context stream. */
+int sfr_matrix_refresh(sfr_context *ctx, sfr_matrix *A)
+{
   TRY(sfr_k_gather(ctx->stream, A->nnz, A->vmap, A->raw, A->val));
   if (A->T) TRY(sfr_k_gather(ctx->stream, A->T->nnz, A->T->vmap, A->raw, A->T->val));
   TRY(sfr_k_diag(ctx->stream, A->n, A->diag_in_al ? A->dmap : NULL, A->raw, A->raw_ad, A->ad, A->dinv));
-  TRY(hipStreamSynchronize(ctx->stream));
   A->have32 = 0;
   return SFR_OK;
 }

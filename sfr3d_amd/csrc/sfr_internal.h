@@ -130,6 +130,8 @@ int sfr_set_error(sfr_context *ctx, int code, const char *fmt, ...);
 /* solve statistics from a scalar block (ms = device time of the solve) */
 void sfr_fill_stats(const sfr_scalars *sc, sfr_stats *st, float ms);
 int  sfr_matrix_prepare_mixed(sfr_context *ctx, sfr_matrix *A);
+/* raw values changed on the device: device-format values, diagonal */
+int  sfr_matrix_refresh(sfr_context *ctx, sfr_matrix *A);
 sfr_mat_view sfr_view(const sfr_matrix *A);
 sfr_geom     sfr_geom_for(const sfr_context *ctx, int64_t n, int rpt);
 
@@ -223,6 +225,11 @@ int sfr_k_mx_inner_iter(hipStream_t s, const sfr_mat_view *A, sfr_geom g,
 int sfr_k_mx_a_only(hipStream_t s, const sfr_mat_view *A, sfr_geom g,
                     const float *r, const float *po, float *pn, float *q,
                     float *y, sfr_scalars *sc, double *part);
+
+/* device assembly: dst[k] = src[map[k] - 1] (map > 0), 0 (map == 0),
+ * unchanged (map < 0) */
+int sfr_k_scatter(hipStream_t s, int64_t n, const int32_t *map,
+                  const double *src, double *dst);
 
 /* ---- uneven batch: independent systems, mixed-precision PCG ------------- */
 #define SFR_UB_MAXSYS 1024       /* systems of one uneven batch             */

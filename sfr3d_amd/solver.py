@@ -102,6 +102,32 @@ class DeviceMatrix:
             self.h = None
 
 
+class Assembly:
+    """Device assembly plan of one resident matrix (sfr_assembly_*): per-cell
+    coefficients on the device scattered into the matrix's values."""
+
+    def __init__(self, ctx, A, pattern, id_, nelcon, nviz, ell_quirk=False):
+        self.ctx, self.A = ctx, A
+        nelcon = np.ascontiguousarray(nelcon, np.int32)
+        self._keep = (pattern, nelcon)
+        h = C.c_void_p()
+        ctx._check(lib.sfr_assembly_create(
+            ctx.h, A.h, C.byref(pattern._c), np.ascontiguousarray(id_, np.int32),
+            nelcon.ravel(), np.ascontiguousarray(nviz, np.int16), nelcon.shape[0],
+            1 if ell_quirk else 0, C.byref(h)))
+        self.h = h.value
+
+    def assemble(self, d_lA, d_lB=None, d_b=None):
+        self.ctx._check(lib.sfr_assemble_device(
+            self.ctx.h, self.h, d_lA.ptr, d_lB.ptr if d_lB else None,
+            d_b.ptr if d_b else None))
+
+    def destroy(self):
+        if self.h:
+            lib.sfr_assembly_destroy(self.ctx.h, self.h)
+            self.h = None
+
+
 class Batch:
     """Uneven batch of independent systems (sfr_batch_*): mixed-precision
     PCG over all of them in one stream of launches."""
@@ -202,6 +228,9 @@ class Context:
 
     def matrix(self, system, fmt=L.FMT_AUTO):
         return DeviceMatrix(self, system, fmt)
+
+    def assembly(self, A, pattern, id_, nelcon, nviz, ell_quirk=False):
+        return Assembly(self, A, pattern, id_, nelcon, nviz, ell_quirk)
 
     def batch(self, systems, tune=True):
         return Batch(self, systems, tune)

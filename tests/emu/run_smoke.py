@@ -73,6 +73,41 @@ rc2 = lib.sfr_batch_solve(ctx, Bt, 1e-8, 1000, C.cast(P(b.ctypes.data, b.ctypes.
                           C.cast(P(xs[0].ctypes.data, xs[1].ctypes.data), vp), 1, C.cast(sts, vp))
 check(f"batch rc {rc}/{rc2} rel {rel(xs[1], z['pois_xo']):.1e}", rc == 0 and rc2 == 0 and rel(xs[1], z['pois_xo']) < 1e-6)
 lib.sfr_batch_destroy(ctx, Bt)
+# device assembly vs host assembly + set_values: bitwise the same solve
+class Pat(C.Structure):
+    _fields_ = [("neq", C.c_int32), ("nad", C.c_int32), ("nia", C.c_int32),
+                ("nja", C.c_int64), ("nval", C.c_int64), ("storage", C.c_short),
+                ("unsym", C.c_short), ("maxViz", C.c_short),
+                ("ia", vp), ("ja", vp)]
+s, maxit = mk("asm")
+pm = [int(v) for v in z["asm_pat"]]
+pia, pja = np.ascontiguousarray(z["asm_pia"]), np.ascontiguousarray(z["asm_pja"])
+pat = Pat(pm[0], pm[1], pm[2], pm[3], pm[4], pm[5], pm[6], pm[7], pia.ctypes.data, pja.ctypes.data)
+aid = np.ascontiguousarray(z["asm_id"]); anel = np.ascontiguousarray(z["asm_nelcon"])
+anv = np.ascontiguousarray(z["asm_nviz"])
+lA = np.ascontiguousarray(z["asm_lA"]); lB = np.ascontiguousarray(z["asm_lB"])
+A1, A2 = vp(), vp()
+assert lib.sfr_matrix_create(ctx, C.byref(s), 0, C.byref(A1)) == 0
+assert lib.sfr_matrix_create(ctx, C.byref(s), 0, C.byref(A2)) == 0
+al2 = np.ascontiguousarray(z["asm_al2"]); ad2 = np.ascontiguousarray(z["asm_ad2"])
+lib.sfr_matrix_set_values.argtypes = [vp, vp, vp, vp, vp]
+assert lib.sfr_matrix_set_values(ctx, A1, al2.ctypes.data, ad2.ctypes.data, None) == 0
+plan = vp()
+lib.sfr_assembly_create.argtypes = [vp, vp, vp, vp, vp, vp, C.c_int, C.c_int, vp]
+rc = lib.sfr_assembly_create(ctx, A2, C.byref(pat), aid.ctypes.data, anel.ctypes.data,
+                             anv.ctypes.data, anel.shape[0], 0, C.byref(plan))
+db = np.zeros(s.neq)
+lib.sfr_assemble_device.argtypes = [vp, vp, vp, vp, vp]
+rc2 = lib.sfr_assemble_device(ctx, plan, lA.ctypes.data, lB.ctypes.data, db.ctypes.data)
+b2 = np.ascontiguousarray(z["asm_b2"])
+x1, x2 = np.zeros(s.neq), np.zeros(s.neq)
+st1, st2 = St(), St()
+lib.sfr_solve_device(ctx, A1, 1, 0, 1e-8, 1000, b2.ctypes.data, x1.ctypes.data, 1, C.byref(st1))
+lib.sfr_solve_device(ctx, A2, 1, 0, 1e-8, 1000, db.ctypes.data, x2.ctypes.data, 1, C.byref(st2))
+check(f"assembly rc {rc}/{rc2} b {np.array_equal(db, b2)} it {st1.iters}/{st2.iters}",
+      rc == 0 and rc2 == 0 and np.array_equal(db, b2) and np.array_equal(x1, x2))
+lib.sfr_assembly_destroy(ctx, plan)
+lib.sfr_matrix_destroy(ctx, A1); lib.sfr_matrix_destroy(ctx, A2)
 lib.sfr_finalize(ctx)
 print("SMOKE", "FAILED" if fails else "OK", flush=True)
 os._exit(1 if fails else 0)

@@ -3,7 +3,8 @@ with g++ against a CPU emulation of the HIP kernel model (tests/emu/), and
 the resulting library is driven through the same C ABI as the GPU build.
 This checks kernel logic -- indexing, the scalar protocol between kernels,
 graph capture and replay, the fused mixed-precision iteration, escaped
-16-bit columns, the uneven batch -- against the oracle without a GPU.
+16-bit columns, the uneven batch, device assembly -- against the oracle and
+the host assembler without a GPU.
 Set SFR_EMU_ASAN=1 to build it under AddressSanitizer (slower), which
 turns any out-of-bounds access of a kernel or of the host code into a
 failure."""
@@ -50,6 +51,19 @@ def write_cases(path):
     sq = System(B.system.neq, B.system.nad, B.system.ia, B.system.ja, al,
                 1.05 * ad, S.ELLPACK, True)
     put("far", sq, b[:B.system.neq].copy(), 2, 3)
+    # device assembly: structure of one ELLPACK system with a pinned cell,
+    # coefficients of another conductance field, host-assembled reference
+    B = systems.poisson3d(6, seed=3, storage=S.ELLPACK, pinned_cell=7)
+    B2 = systems.poisson3d(6, seed=4, storage=S.ELLPACK, pinned_cell=7)
+    al2, ad2, b2 = ST.assemble(B.pattern, B.id, B.nelcon, B.nviz,
+                               B2.extra["lA"], B2.extra["lB"])
+    put("asm", B.system, B.b, 1, 1000)
+    pat = B.pattern
+    d.update(asm_pat=np.array([pat.neq, pat.nad, len(pat.ia), len(pat.ja), pat.nval,
+                               pat.storage, int(pat.unsym), pat.max_viz]),
+             asm_pia=pat.ia, asm_pja=pat.ja, asm_id=B.id, asm_nelcon=B.nelcon,
+             asm_nviz=B.nviz, asm_lA=np.ascontiguousarray(B2.extra["lA"]),
+             asm_lB=B2.extra["lB"], asm_al2=al2, asm_ad2=ad2, asm_b2=b2[:B.system.neq])
     np.savez(path, **d)
 
 
@@ -73,4 +87,4 @@ def test_kernels_under_cpu_emulation(tmp_path):
     print(res.stdout)
     assert res.returncode == 0, res.stdout + res.stderr[-3000:]
     assert "SMOKE OK" in res.stdout
-    assert res.stdout.count("PASS") == 8
+    assert res.stdout.count("PASS") == 9
