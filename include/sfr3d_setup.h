@@ -124,6 +124,29 @@ short  sfr_solver_id(const char *word);   /* setSolver, Solv.c:507-528     */
 short  sfr_storage_id(const char *word);  /* setDataStruct, Datastruct.c:1707 */
 double sfr_smachn(void);                  /* smachn, Solv.c:614-622        */
 
+/* ---- MatrixMarket --------------------------------------------------------
+ * sfr_write_mtx: the reference's writeCoo (src/WriteMtx.c:26-161) -- CSRD
+ * storage, every row as its diagonal then its stored entries, 1-based,
+ * "%d %d %30.14e" lines under a "coordinate real symmetric|general" banner;
+ * bin = 1 writes 1.0 for every value (the pattern files of the reference's
+ * test/output/coo).  sfr_write_mtx_vector: writeCooB (src/WriteMtx.c:165-176).
+ * sfr_read_mtx: any real coordinate file into CSRD (symmetric file: strictly
+ * lower sorted rows, unsym = 0; general: full off-diagonal sorted rows,
+ * unsym = 1; diagonal in ad); release with sfr_mtx_free.  Status codes are
+ * those of sfr3d.h (0 = success). */
+typedef struct sfr_mtx {
+  int32_t  neq, nad;
+  short    unsym;
+  int32_t *ia, *ja;
+  double  *al, *ad;
+} sfr_mtx;
+
+struct sfr_system;
+int  sfr_write_mtx(const char *path, const struct sfr_system *sys, int bin);
+int  sfr_write_mtx_vector(const char *path, const double *b, int32_t neq);
+int  sfr_read_mtx(const char *path, sfr_mtx *out);
+void sfr_mtx_free(sfr_mtx *m);
+
 #ifdef __cplusplus
 }
 #endif

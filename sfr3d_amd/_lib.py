@@ -68,6 +68,12 @@ class SfrPattern(C.Structure):
                 ("ia", C.POINTER(C.c_int32)), ("ja", C.POINTER(C.c_int32))]
 
 
+class SfrMtx(C.Structure):
+    _fields_ = [("neq", C.c_int32), ("nad", C.c_int32), ("unsym", C.c_short),
+                ("ia", C.POINTER(C.c_int32)), ("ja", C.POINTER(C.c_int32)),
+                ("al", C.POINTER(C.c_double)), ("ad", C.POINTER(C.c_double))]
+
+
 def _sig(name, res, *args):
     f = getattr(lib, name)
     f.restype = res
@@ -145,6 +151,10 @@ _sig("sfr_assemble", C.c_int, C.POINTER(SfrPattern), i32p, i32p, i16p,
      C.c_int, f64p, f64p, vp, vp, vp, C.c_int)
 _sig("sfr_read_solver_config", C.c_int, C.c_char_p, C.POINTER(C.c_short),
      C.POINTER(C.c_double), C.POINTER(C.c_uint), C.POINTER(C.c_short))
+_sig("sfr_write_mtx", C.c_int, C.c_char_p, C.POINTER(SfrSystem), C.c_int)
+_sig("sfr_write_mtx_vector", C.c_int, C.c_char_p, f64p, C.c_int32)
+_sig("sfr_read_mtx", C.c_int, C.c_char_p, C.POINTER(SfrMtx))
+_sig("sfr_mtx_free", None, C.POINTER(SfrMtx))
 _sig("sfr_solver_id", C.c_short, C.c_char_p)
 _sig("sfr_storage_id", C.c_short, C.c_char_p)
 _sig("sfr_smachn", C.c_double)

@@ -10,7 +10,7 @@ from dataclasses import dataclass
 
 import numpy as np
 
-from ._lib import lib, SfrPattern, ptr, CSR, CSRC, CSRCCOO, ELLPACK
+from ._lib import lib, SfrPattern, SfrMtx, ptr, CSR, CSRC, CSRCCOO, CSRD, ELLPACK
 
 
 def _chk(rc, what):
@@ -155,3 +155,37 @@ def read_solver_config(path):
         raise RuntimeError(f"sfr_read_solver_config({path}) = {rc}")
     return {"solver": s.value, "tol": t.value, "maxit": m.value,
             "storage": st.value}
+
+
+# ---------------------------------------------------------------------------
+# MatrixMarket (reference src/WriteMtx.c; C side in host/mtx.c)
+
+def write_mtx(path, system, bin=False):
+    """The reference's writeCoo of a CSRD system; returns the status."""
+    c = system.c()
+    return lib.sfr_write_mtx(path.encode(), C.byref(c), 1 if bin else 0)
+
+
+def write_mtx_vector(path, b):
+    b = np.ascontiguousarray(b, np.float64)
+    return lib.sfr_write_mtx_vector(path.encode(), b, len(b))
+
+
+def read_mtx(path):
+    """A coordinate real file as a CSRD System (symmetric file: lower rows;
+    general: full rows)."""
+    from .solver import System
+    m = SfrMtx()
+    _chk(lib.sfr_read_mtx(path.encode(), C.byref(m)), "sfr_read_mtx")
+    try:
+        n, nad = m.neq, m.nad
+        ia = np.ctypeslib.as_array(m.ia, shape=(n + 1,)).copy()
+        ja = (np.ctypeslib.as_array(m.ja, shape=(nad,)).copy() if nad
+              else np.zeros(0, np.int32))
+        al = (np.ctypeslib.as_array(m.al, shape=(nad,)).copy() if nad
+              else np.zeros(0))
+        ad = np.ctypeslib.as_array(m.ad, shape=(n,)).copy()
+        unsym = bool(m.unsym)
+    finally:
+        lib.sfr_mtx_free(C.byref(m))
+    return System(n, nad, ia, ja, al, ad, CSRD, unsym)
