@@ -1,6 +1,7 @@
 /*
  * sfr3d.h -- MI355X (gfx950) linear-solve path for SFR3D-assembled systems:
- * Jacobi-preconditioned CG and BiCGSTAB over CSR-family and ELLPACK matrices.
+ * Jacobi-preconditioned CG, BiCGSTAB and GMRES(15) over CSR-family and
+ * ELLPACK matrices.
  *
  * The boundary keeps the contract of the reference's solver dispatch
  * (solverC, reference include/Solv.h:59-70, src/Solv.c:64-501): a matrix in
@@ -12,7 +13,11 @@
  *   PCG      sqrt|r.D^-1 r| < tol * sqrt|b.D^-1 b|   (src/Cg.c:73-78,141-142)
  *   BiCGSTAB ||r||_2 < tol * ||b||_2, checked after each half step
  *            (src/Bicgstab.c:79-81,129-131,153-154)
- * and the iteration count reported is the reference's j+1.
+ *   GMRES    |e_k| < tol * ||D^-1 b||_2 after each restart cycle of k = 15
+ *            Arnoldi steps, left Jacobi preconditioning
+ *            (src/Iters.c:625-628,740); maxit counts cycles
+ * and the iteration count reported is the reference's (PCG, BiCGSTAB: j+1;
+ * GMRES: Arnoldi steps + 1, with the cycles in stats.outer).
  *
  * Every entry point returns an SFR_* status.  All arrays are caller-owned and
  * never freed or reallocated; x is read (newX = 0) and written, b is read.
@@ -35,6 +40,7 @@ extern "C" {
 /* solver ids (include/Define.h:392-397) */
 #define SFR_PCG        1
 #define SFR_PBICGSTAB  2
+#define SFR_GMRES      4
 
 /* status codes */
 #define SFR_OK             0
@@ -72,14 +78,14 @@ typedef struct sfr_system {
 typedef struct sfr_stats {
   int      status;
   unsigned iters;     /* Krylov iterations (reference j+1); MIXED: sum of inner */
-  unsigned outer;     /* MIXED: FP64 refinement steps                          */
+  unsigned outer;     /* MIXED: FP64 refinement steps; GMRES: restart cycles   */
   double   tol;
-  double   norm_b;    /* PCG: sqrt|b.D^-1 b|  BiCGSTAB: ||b||                   */
+  double   norm_b;    /* PCG: sqrt|b.D^-1 b|  BiCGSTAB: ||b||  GMRES: ||D^-1 b|| */
   double   conv;      /* tol * norm_b                                          */
   double   xKx;       /* x.Ax            (src/Cg.c:155-158)                    */
   double   norm_x;    /* ||x||           (src/Cg.c:162)                        */
   double   norm_r;    /* ||b - Ax||, FP64                                      */
-  double   norm_r_m;  /* sqrt|r.D^-1 r|, FP64                                  */
+  double   norm_r_m;  /* sqrt|r.D^-1 r|, FP64; GMRES: |e_k| of the last cycle  */
   double   relres;    /* ||b - Ax|| / ||b||                                    */
   double   time_s;    /* solve time on the device (events on the context stream) */
 } sfr_stats;
@@ -122,8 +128,9 @@ int  sfr_matrix_info(const sfr_matrix *A, int32_t *neq, int *format,
 int  sfr_matrix_index_bytes(const sfr_matrix *A);
 
 /* ---- solves ---------------------------------------------------------------
- * sfr_solve: host arrays in and out, the drop-in for solverC's PCG and
- * PBICGSTAB branches (reference src/Solv.c:93-212).
+ * sfr_solve: host arrays in and out, the drop-in for solverC's PCG,
+ * PBICGSTAB and GMRES branches (reference src/Solv.c:93-212, 302-372).
+ * GMRES runs in FP64 only (SFR_PREC_MIXED returns SFR_NOT_SUPPORTED).
  * sfr_solve_device: b and x already on the device (d_b, d_x: neq doubles).
  * precision SFR_PREC_MIXED runs FP32 inner solves under an FP64 refinement
  * loop whose residual, update and stopping test are FP64 and use the same
@@ -246,7 +253,7 @@ void sfr_assembly_destroy(sfr_context *ctx, sfr_assembly *plan);
 int sfr_kernel_time(sfr_context *ctx, sfr_matrix *A, int which, int reps,
                     double *ms_per_launch);
 /* Tuning knobs of a context: "chunk" = iterations per captured graph chunk
- * (1..256); "ru_delta" = MIXED: drop of the FP32 residual (relative to its
+ * (1..256; GMRES captures chunk/8 restart cycles, at least one); "ru_delta" = MIXED: drop of the FP32 residual (relative to its
  * largest value since the last fold) that triggers an FP64 reliable update. */
 int sfr_set_option(sfr_context *ctx, const char *name, double value);
 
@@ -254,9 +261,9 @@ int sfr_set_option(sfr_context *ctx, const char *name, double value);
  * with the reference's opaque Memoria/Interface/BufferOmp pointers passed
  * through untouched.  Non-convergence and bad ids end the process with the
  * reference's messages and exit codes (src/Cg.c:195-200,
- * src/Bicgstab.c:225-231, src/Solv.c:496); with fLog one line in the
- * reference's format goes to fSolvLog (src/Cg.c:201-208,
- * src/Bicgstab.c:235-242). */
+ * src/Bicgstab.c:225-231, src/Iters.c:770-774, src/Solv.c:496); with fLog
+ * one line in the reference's format goes to fSolvLog (src/Cg.c:201-208,
+ * src/Bicgstab.c:235-242, src/Iters.c:778-786). */
 void sfr_solverC(void *m, int nEq, int nEqNov, int nAd, int nAdR,
                  int *ia, int *ja, double *al, double *ad, double *au,
                  double *b, double *x, void *iNeq, void *bOmp,

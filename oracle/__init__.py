@@ -1,7 +1,7 @@
 """ORACLE -- test infrastructure only.
 
 CPU restatement of the reference's linear-solve path (sfr_oracle.c: pcg,
-pbicgstab, the serial matvecs, dot, preCondDiag) and of its index builders
+pbicgstab, gmres, the serial matvecs, dot, preCondDiag) and of its index builders
 (patterns.py: convGraph, GENRCM, numEqV1/V2, csrIa/csrJa, ellPackJa, the csr/
 ellPack value scatter), plus, where the reference tree exists, a harness
 around the reference's own compiled C (_ref/libsfr_ref.so, built by
@@ -53,6 +53,11 @@ def orc():
             f.argtypes = [C.c_int, C.c_int, i32p, i32p, f64p, f64p, f64p, f64p,
                           C.c_double, C.c_uint, C.c_int, C.c_int, C.c_int,
                           C.POINTER(OrcStats)]
+        lib.orc_gmres.restype = C.c_int
+        lib.orc_gmres.argtypes = [C.c_int, C.c_int, i32p, i32p, f64p, f64p,
+                                  f64p, f64p, C.c_double, C.c_uint, C.c_int,
+                                  C.c_int, C.c_int, C.c_int,
+                                  C.POINTER(OrcStats), C.POINTER(C.c_uint)]
         lib.orc_matvec.restype = C.c_int
         lib.orc_matvec.argtypes = [C.c_int, C.c_int, C.c_int, i32p, i32p, f64p,
                                    f64p, f64p, f64p]
@@ -112,6 +117,26 @@ def pcg(system, b, tol, maxit, x=None, newX=True):
 
 def pbicgstab(system, b, tol, maxit, x=None, newX=True):
     return _krylov("orc_pbicgstab", system, b, tol, maxit, x, newX)
+
+
+def gmres(system, b, tol, ncycles, x=None, newX=True, nkrylov=15):
+    """Reference-order serial GMRES(nkrylov) (oracle).  Returns (x, stats);
+    stats["cycles"] is the number of restart cycles run."""
+    b = np.ascontiguousarray(b, np.float64)
+    x = np.zeros(system.neq) if x is None else np.array(x, np.float64)
+    st = OrcStats()
+    cyc = C.c_uint(0)
+    ad = system.ad if system.ad is not None else system.al
+    rc = orc().orc_gmres(system.neq, system.nad, system.ia, system.ja,
+                         _al_contig(system), np.ascontiguousarray(ad, np.float64),
+                         b, x, tol, ncycles, 1 if newX else 0, system.storage,
+                         1 if system.unsym else 0, nkrylov, C.byref(st),
+                         C.byref(cyc))
+    if rc:
+        raise RuntimeError(f"orc_gmres = {rc}")
+    d = st.as_dict()
+    d["cycles"] = cyc.value
+    return x, d
 
 
 def _al_contig(system):

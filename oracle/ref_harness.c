@@ -147,7 +147,8 @@ int ref_assemble(int *id, int *nelcon, short *nViz, int numel, short maxViz,
 }
 
 /* solverC with the given arrays; out[0] = iterations, out[1] = xKx,
- * out[2] = ||x|| (parsed back from the reference's own log line). */
+ * out[2] = ||x||, out[3] = time, and for GMRES out[4] = cycles (parsed back
+ * from the reference's own log line). */
 int ref_solve(int neq, int nad, int *ia, int *ja, double *al, double *ad,
               double *au, double *b, double *x, double tol, unsigned maxit,
               short storage, short solver, int newX, int unsym, double *out,
@@ -168,6 +169,12 @@ int ref_solve(int neq, int nad, int *ia, int *ja, double *al, double *ad,
   if (solver == PCG) {
     if (sscanf(buf, "PCG: tol %lf iteration %d xKx %lf norma(x*x) %lf time %lf",
                &t, &it, &xkx, &nx, &tm_) != 5) return -3;
+  } else if (solver == GMRES) {
+    int cyc, nk;
+    if (sscanf(buf, "GMRES: tol %lf Cycles %d iteration %d nKrylov %d xKx %lf "
+               "norma(x*x) %lf time %lf", &t, &cyc, &it, &nk, &xkx, &nx,
+               &tm_) != 7) return -3;
+    out[4] = cyc;                       /* GMRES callers pass out[5] */
   } else {
     if (sscanf(buf, "PBICGSTAB: tol %lf iteration %d xKx %lf ||x*x|| %lf time %lf",
                &t, &it, &xkx, &nx, &tm_) != 5) return -3;

@@ -20,6 +20,7 @@
  *   preCondDiag      reference src/PreCond.c:18-26
  *   pcg              reference src/Cg.c:47-209, driven as src/Solv.c:93-145
  *   pbicgstab        reference src/Bicgstab.c:50-245, as src/Solv.c:150-212
+ *   gmres            reference src/Iters.c:587-790, as src/Solv.c:302-372
  * The OpenMP variants (orc_*_omp) keep the same recurrences with parallel
  * loops and reductions over full rows; they are the all-cores CPU baseline
  * (the reference's pcgOmp, src/Cg.c:490-715, plays that role there).
@@ -297,6 +298,119 @@ int orc_pbicgstab(int n, int nad, const int *ia, const int *ja,
     st->time_s = t1 - t0;
   }
   free(buf);
+  return 0;
+}
+
+/* GMRES(k) with left Jacobi preconditioning, reference src/Iters.c:587-790,
+ * driven as src/Solv.c:302-372 (k = nKrylov = 15, cycles = maxIt).
+ * Kept as the reference runs it:
+ *   - g_0 = D^-1 (b - A x); Arnoldi on D^-1 A with modified Gram-Schmidt;
+ *   - the in-cycle convergence test is an empty statement (Iters.c:702), so
+ *     every cycle runs all k Arnoldi steps;
+ *   - the back substitution for y stops its inner sum at j < k-1
+ *     (Iters.c:714-719), so y_{k-1} never enters the rows above it, and the
+ *     update x += sum_{j<k-1} y_j g_j leaves out the last basis vector
+ *     (Iters.c:723-728);
+ *   - the cycle loop stops when |e_k| < tol ||D^-1 b|| (Iters.c:740).
+ * Reported: iters = Arnoldi steps + 1 and *cycles = cycles run (the
+ * reference's log line, Iters.c:778-786).  status 1 is the reference's
+ * MAXIT exit (Iters.c:770-774).  norm_r_m is |e_k| of the last cycle. */
+int orc_gmres(int n, int nad, const int *ia, const int *ja, const double *al,
+              const double *ad, const double *b, double *x, double tol,
+              unsigned ncycles, int newX, int storage, int unsym, int nk,
+              orc_stats *st, unsigned *cycles_out)
+{
+  (void)nad;
+  matvec_fn matvec = pick(storage, unsym);
+  if (!matvec || n <= 0 || nk < 1) return -1;
+  const size_t N = (size_t)n;
+  double *m = malloc(N * sizeof(double));
+  double *g = calloc(N * (size_t)(nk + 1), sizeof(double));
+  double *h = calloc((size_t)nk * (nk + 1), sizeof(double));
+  double *e = calloc((size_t)nk + 1, sizeof(double));
+  double *c = calloc((size_t)nk, sizeof(double));
+  double *s = calloc((size_t)nk, sizeof(double));
+  double *y = calloc((size_t)nk, sizeof(double));
+  if (!m || !g || !h || !e || !c || !s || !y) {
+    free(m); free(g); free(h); free(e); free(c); free(s); free(y);
+    return -2;
+  }
+#define H(i, j) h[(size_t)(i) * nk + (j)]
+  for (int i = 0; i < n; i++) m[i] = 1.0 / ad[i];
+  const double t0 = now();
+  if (newX) for (int i = 0; i < n; i++) x[i] = 0.0;
+  for (int i = 0; i < n; i++) g[i] = b[i] * m[i];
+  const double norm_b = sqrt(orc_dot(g, g, n));
+  const double conv = tol * norm_b;
+  unsigned l;
+  unsigned nit = 0;
+  int ni = 0;
+  for (l = 0; l < ncycles; l++) {
+    matvec(n, ia, ja, al, ad, x, g);
+    for (int i = 0; i < n; i++) g[i] = (b[i] - g[i]) * m[i];
+    e[0] = sqrt(orc_dot(g, g, n));
+    double tmp = 1.0 / e[0];
+    for (int i = 0; i < n; i++) g[i] *= tmp;
+    for (ni = 0; ni < nk; ni++, nit++) {
+      double *v = g + (size_t)ni * N, *w = g + (size_t)(ni + 1) * N;
+      matvec(n, ia, ja, al, ad, v, w);
+      for (int i = 0; i < n; i++) w[i] *= m[i];
+      for (int j = 0; j <= ni; j++) {
+        const double *gj = g + (size_t)j * N;
+        const double beta = orc_dot(w, gj, n);
+        for (int i = 0; i < n; i++) w[i] -= beta * gj[i];
+        H(j, ni) = beta;
+      }
+      const double nrm = sqrt(orc_dot(w, w, n));
+      H(ni + 1, ni) = nrm;
+      tmp = 1.0 / nrm;
+      for (int i = 0; i < n; i++) w[i] *= tmp;
+      for (int j = 0; j < ni; j++) {
+        const double h1 = H(j, ni), h2 = H(j + 1, ni);
+        H(j, ni) = c[j] * h1 + s[j] * h2;
+        H(j + 1, ni) = -s[j] * h1 + c[j] * h2;
+      }
+      const double h1 = H(ni, ni), h2 = H(ni + 1, ni);
+      const double r = sqrt(h1 * h1 + h2 * h2);
+      tmp = 1.0 / r;
+      c[ni] = h1 * tmp;
+      s[ni] = h2 * tmp;
+      H(ni, ni) = r;
+      H(ni + 1, ni) = 0.0;
+      e[ni + 1] = -s[ni] * e[ni];
+      e[ni] = c[ni] * e[ni];
+    }
+    ni--;
+    y[ni] = e[ni] / H(ni, ni);
+    for (int i = ni - 1; i >= 0; i--) {
+      y[i] = 0.0;
+      for (int j = i + 1; j < ni; j++) y[i] -= H(i, j) * y[j];
+      y[i] = (y[i] + e[i]) / H(i, i);
+    }
+    for (int j = 0; j < ni; j++) {
+      const double yj = y[j];
+      const double *gj = g + (size_t)j * N;
+      for (int i = 0; i < n; i++) x[i] += gj[i] * yj;
+    }
+    if (fabs(e[ni + 1]) < conv) break;
+  }
+  const double last = fabs(e[ni + 1]);
+  matvec(n, ia, ja, al, ad, x, g);
+  const double xKx = orc_dot(x, g, n);
+  const double norm = sqrt(orc_dot(x, x, n));
+  for (int i = 0; i < n; i++) g[N + i] = b[i] - g[i];
+  const double norm_r = sqrt(orc_dot(g + N, g + N, n));
+  const double t1 = now();
+#undef H
+  if (st) {
+    st->status = (l == ncycles) ? 1 : 0;
+    st->iters = nit + 1;
+    st->xKx = xKx; st->norm_x = norm; st->norm_r = norm_r;
+    st->norm_r_m = last; st->norm_b = norm_b; st->conv = conv;
+    st->time_s = t1 - t0;
+  }
+  if (cycles_out) *cycles_out = (l == ncycles) ? l : l + 1;
+  free(m); free(g); free(h); free(e); free(c); free(s); free(y);
   return 0;
 }
 

@@ -114,6 +114,7 @@ void sfr_finalize(sfr_context *ctx)
   for (int i = 0; i < 10; i++) hipFree(ctx->w64[i]);
   for (int i = 0; i < 6; i++) hipFree(ctx->w32[i]);
   hipFree(ctx->stage);
+  hipFree(ctx->gm_buf);
   hipFree(ctx->d_sc);
   hipFree(ctx->part);
   if (ctx->h_sc) hipHostFree(ctx->h_sc);
@@ -178,6 +179,24 @@ int sfr_workspace(sfr_context *ctx, int64_t n)
   for (int i = 0; i < 10; i++) TRY(hipMalloc((void **)&ctx->w64[i], sizeof(double) * (size_t)n));
   for (int i = 0; i < 6; i++) TRY(hipMalloc((void **)&ctx->w32[i], sizeof(float) * (size_t)n));
   ctx->ws_n = n;
+  return SFR_OK;
+}
+
+/* GMRES buffers: SFR_GM_K basis vectors and two raw vectors at a stride of
+ * ld (n rounded up to 64 doubles), then the Hessenberg state. */
+int sfr_gm_workspace(sfr_context *ctx, int64_t n)
+{
+  if (n <= ctx->gm_n) return SFR_OK;
+  TRY(hipStreamSynchronize(ctx->stream));
+  drop_graphs(ctx);
+  hipFree(ctx->gm_buf);
+  ctx->gm_buf = NULL;
+  ctx->gm_n = 0;
+  const int64_t ld = (n + 63) & ~(int64_t)63;
+  TRY(hipMalloc((void **)&ctx->gm_buf,
+                sizeof(double) * ((size_t)(SFR_GM_K + 2) * (size_t)ld + SFR_GM_STATE)));
+  ctx->gm_n = n;
+  ctx->gm_ld = ld;
   return SFR_OK;
 }
 

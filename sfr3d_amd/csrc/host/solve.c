@@ -30,7 +30,14 @@ int sfr_workspace(sfr_context *ctx, int64_t n);
 int sfr_matrix_prepare_mixed(sfr_context *ctx, sfr_matrix *A);
 int sfr_matrix_prepare_adjoint(sfr_context *ctx, sfr_matrix *A);
 
-enum { G_PCG = 1, G_BICG = 2, G_MX = 3, G_BB = 4 };
+enum { G_PCG = 1, G_BICG = 2, G_MX = 3, G_BB = 4, G_GM = 5 };
+
+/* GMRES restart cycles per captured chunk (137 kernels each) */
+static int gm_cycles_per_chunk(const sfr_context *ctx)
+{
+  const int c = ctx->chunk / 8;
+  return c < 1 ? 1 : c;
+}
 
 /* Capture `chunk` iterations of one loop body, or reuse a captured one. */
 static int get_graph(sfr_context *ctx, int kind, const void *k0,
@@ -39,12 +46,14 @@ static int get_graph(sfr_context *ctx, int kind, const void *k0,
 {
   /* the mixed loop alternates its direction buffers: an even chunk ends
    * where it started */
-  const int chunk = kind == G_MX ? (ctx->chunk + 1) & ~1 : ctx->chunk;
+  const int chunk = kind == G_MX ? (ctx->chunk + 1) & ~1
+                  : kind == G_GM ? gm_cycles_per_chunk(ctx) : ctx->chunk;
   for (int i = 0; i < ctx->ngraph; i++) {
     sfr_graph *e = &ctx->graphs[i];
     if (e->kind == kind && e->chunk == chunk && e->key[0] == k0 &&
         e->key[1] == k1 && e->key[2] == k2 &&
-        e->key[3] == (const void *)ctx->w64[0] && e->key[4] == (const void *)(intptr_t)g.rpt) {
+        e->key[3] == (const void *)ctx->w64[0] && e->key[4] == (const void *)(intptr_t)g.rpt &&
+        e->key[5] == (const void *)ctx->gm_buf) {
       *exec = e->exec;
       return SFR_OK;
     }
@@ -70,6 +79,13 @@ static int get_graph(sfr_context *ctx, int kind, const void *k0,
       lrc = sfr_k_mx_inner_iter(s, v, g, w32[0], w32[1], pb, w32[3], it & 1, ctx->d_sc,
                                 ctx->part);
     }
+    else if (kind == G_GM) {  /* k2 is b */
+      double *basis = ctx->gm_buf;
+      const int64_t ld = ctx->gm_ld;
+      double *const wr[2] = {basis + SFR_GM_K * ld, basis + (SFR_GM_K + 1) * ld};
+      lrc = sfr_k_gm_cycle(s, v, g, (const double *)k2, (double *)k1, basis, ld, wr,
+                           basis + (SFR_GM_K + 2) * ld, ctx->d_sc, ctx->part);
+    }
     else                    /* G_BB: k2 carries the batch size */
       lrc = sfr_k_bb_iter(s, v, g, (int)(intptr_t)k2, w[8], w[9], w[0], w[1], w[2], w[3],
                           w[4], w[5], w[6], ctx->d_sc, ctx->part);
@@ -92,6 +108,7 @@ static int get_graph(sfr_context *ctx, int kind, const void *k0,
   e->key[0] = k0; e->key[1] = k1; e->key[2] = k2;
   e->key[3] = (const void *)ctx->w64[0];
   e->key[4] = (const void *)(intptr_t)g.rpt;
+  e->key[5] = (const void *)ctx->gm_buf;
   *exec = ex;
   return SFR_OK;
 }
@@ -203,6 +220,37 @@ static int solve_bicg64(sfr_context *ctx, sfr_matrix *A, double tol,
                        maxit, ctx->d_sc, ctx->part));
   CHECK(run_loop(ctx, ex, 0, chunk_budget(ctx, maxit), 1));
   return finish(ctx, &v, g, d_b, d_x, st);
+}
+
+/* GMRES(SFR_GM_K), reference src/Iters.c:587-790 as src/Solv.c:302-372 runs
+ * it: maxit counts restart cycles.  Statistics as the reference reports
+ * them: iters = Arnoldi steps + 1 (0 when the first residual is exactly
+ * zero), outer = cycles, norm_b = ||D^-1 b||, norm_r_m = |e_k| of the last
+ * cycle (the quantity of the stop test). */
+static int solve_gmres64(sfr_context *ctx, sfr_matrix *A, double tol,
+                         unsigned maxit, const double *d_b, double *d_x,
+                         int newX, sfr_stats *st)
+{
+  CHECK(sfr_workspace(ctx, A->n));
+  CHECK(sfr_gm_workspace(ctx, A->n));
+  sfr_mat_view v = sfr_view(A);
+  sfr_geom g = {A->rpt, 0};
+  hipGraphExec_t ex;
+  CHECK(get_graph(ctx, G_GM, A, d_x, d_b, &v, g, &ex));
+  TRY(hipEventRecord(ctx->ev_start, ctx->stream));
+  TRY(sfr_k_gm_setup(ctx->stream, &v, g, d_b, d_x, newX, tol, maxit, ctx->d_sc,
+                     ctx->part));
+  const unsigned cyc = (unsigned)gm_cycles_per_chunk(ctx);
+  CHECK(run_loop(ctx, ex, 0, maxit / cyc + 2, 1));
+  sfr_stats tmp;
+  const int rc = finish(ctx, &v, g, d_b, d_x, &tmp);
+  if (rc == SFR_HIP_ERROR) return rc;
+  const sfr_scalars *sc = ctx->h_sc;
+  tmp.iters = sc->iter ? sc->iter + 1 : 0;
+  tmp.outer = sc->outer;
+  tmp.norm_r_m = sc->res[4];
+  if (st) *st = tmp;
+  return rc;
 }
 
 static int solve_pcg_mixed(sfr_context *ctx, sfr_matrix *A, double tol,
@@ -363,6 +411,11 @@ int sfr_solve_device(sfr_context *ctx, sfr_matrix *A, int solver,
       return sfr_set_error(ctx, SFR_NOT_SUPPORTED, "mixed BiCGSTAB");
     return solve_bicg64(ctx, A, tol, maxit, d_b, d_x, newX, st);
   }
+  if (solver == SFR_GMRES) {
+    if (precision == SFR_PREC_MIXED)
+      return sfr_set_error(ctx, SFR_NOT_SUPPORTED, "mixed GMRES");
+    return solve_gmres64(ctx, A, tol, maxit, d_b, d_x, newX, st);
+  }
   return sfr_set_error(ctx, SFR_BAD_ARG, "solver id %d", solver);
 }
 
@@ -486,7 +539,7 @@ void sfr_solverC(void *m, int nEq, int nEqNov, int nAd, int nAdR, int *ia,
                  const char *name)
 {
   (void)m; (void)iNeq; (void)bOmp;
-  if (solver != SFR_PCG && solver != SFR_PBICGSTAB) {
+  if (solver != SFR_PCG && solver != SFR_PBICGSTAB && solver != SFR_GMRES) {
     fprintf(stderr, "Opcao %d e invalida!!\n", solver);
     fprintf(stderr, "Arquivo:%s\nFonte:  %s\n", __FILE__, __func__);
     exit(EXIT_FAILURE);
@@ -517,6 +570,11 @@ void sfr_solverC(void *m, int nEq, int nEqNov, int nAd, int nAdR, int *ia,
     fprintf(stderr, "sfr_solverC: %s: %s\n", sfr_status_string(rc), sfr_last_error(g_ctx));
     exit(EXIT_FAILURE);
   }
+  if (rc != SFR_OK && solver == SFR_GMRES) {  /* src/Iters.c:770-774 */
+    printf(" GMRES *** WARNING: no convergende reached !\n");
+    printf("MAXIT = %d \n", maxIt);
+    exit(EXIT_FAILURE);
+  }
   if (rc != SFR_OK) {                 /* j == maxIt (and breakdowns) */
     printf("name : %s\n", name ? name : "");
     if (solver == SFR_PCG) {
@@ -532,6 +590,10 @@ void sfr_solverC(void *m, int nEq, int nEqNov, int nAd, int nAdR, int *ia,
     if (solver == SFR_PCG)
       fprintf(fSolvLog, "PCG: tol %20.2e iteration %d xKx %20.12e norma(x*x) %20.12e "
               "time %20.5lf\n", tol, (int)st.iters, st.xKx, st.norm_x, st.time_s);
+    else if (solver == SFR_GMRES)          /* src/Iters.c:778-786 */
+      fprintf(fSolvLog, "GMRES: tol %20.2e  Cycles %d iteration %d  nKrylov %d  xKx %20.12e "
+              " norma(x*x) %20.12e  time %20.5lf\n", tol, (int)st.outer, (int)st.iters,
+              SFR_GM_K, st.xKx, st.norm_x, st.time_s);
     else
       fprintf(fSolvLog, "PBICGSTAB: tol %20.2e iteration %d xKx %20.12e ||x*x|| %20.12e "
               "time %20.5lf\n", tol, (int)st.iters, st.xKx, st.norm_x, st.time_s);

@@ -23,6 +23,8 @@ extern "C" {
 #define SFR_MAXSYS  8            /* systems of one shared-pattern batch     */
 #define SFR_D16_ESC (-32768)     /* 16-bit column offset: see the exceptions */
 #define SFR_D16_MAXEXC 256       /* most exceptions a 16-bit ELL may have   */
+#define SFR_GM_K    15           /* GMRES Krylov dimension (src/Solv.c:82)  */
+#define SFR_GM_STATE 512         /* doubles of GMRES state: h, c, s, e      */
 
 /* Scalars of one Krylov solve, resident on the device.  The last workgroup
  * of each reducing kernel writes them; later kernels read them. */
@@ -123,6 +125,8 @@ struct sfr_context {
   int          ngraph;
   sfr_matrix  *cached;          /* pattern cache of the host-pointer path  */
   int          chunk;           /* iterations per captured chunk           */
+  double      *gm_buf;          /* GMRES: basis, raw vectors, state        */
+  int64_t      gm_n, gm_ld;
   double       ru_delta;        /* MIXED: reliable-update residual drop    */
 };
 
@@ -169,6 +173,16 @@ int sfr_k_pcg_norm(hipStream_t s, const sfr_mat_view *A, sfr_geom g,
 int sfr_k_final(hipStream_t s, const sfr_mat_view *A, sfr_geom g,
                 const double *b, const double *x, sfr_scalars *sc,
                 double *part);
+
+/* GMRES(SFR_GM_K), FP64 (reference src/Iters.c:587-790) */
+int sfr_gm_workspace(sfr_context *ctx, int64_t n);
+int sfr_k_gm_setup(hipStream_t s, const sfr_mat_view *A, sfr_geom g,
+                   const double *b, double *x, int newX, double tol,
+                   unsigned maxit, sfr_scalars *sc, double *part);
+int sfr_k_gm_cycle(hipStream_t s, const sfr_mat_view *A, sfr_geom g,
+                   const double *b, double *x, double *basis, int64_t ld,
+                   double *const w[2], double *st, sfr_scalars *sc,
+                   double *part);
 
 /* BiCGSTAB, FP64 (reference src/Bicgstab.c:50-245) */
 int sfr_k_bicg_setup(hipStream_t s, const sfr_mat_view *A, sfr_geom g,

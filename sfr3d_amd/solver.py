@@ -2,7 +2,7 @@
 
 Mirrors the reference's solver boundary, solverC (reference
 include/Solv.h:59-70): a system in one of SFR3D's storages, a solver id
-(PCG = 1, PBICGSTAB = 2), tol, maxit, b and x.  Everything here calls the
+(PCG = 1, PBICGSTAB = 2, GMRES = 4), tol, maxit, b and x.  Everything here calls the
 HIP library; there is no CPU path.
 """
 import ctypes as C

@@ -1,6 +1,7 @@
 """TEST INFRASTRUCTURE -- runs the emulated library (argv[1]) on the small
 cases of argv[2] (written by tests/test_emulated_kernels.py with the
 oracle's solutions) in a process of its own: FP64 and mixed PCG, BiCGSTAB,
+GMRES,
 the shared-pattern batch, both adjoint paths, 16-bit column offsets with
 escaped far columns, and a tuned uneven batch.  Prints PASS/FAIL lines and
 exits non-zero on any failure."""
@@ -41,6 +42,10 @@ for prec in (0, 1):
 lam = np.zeros(s.neq); gal = np.zeros(len(z["pois_al"])); gad = np.zeros(s.neq); st = St()
 rc = lib.sfr_solve_adjoint(ctx, A, 1, 1e-8, maxit, x.ctypes.data, b.ctypes.data, lam.ctypes.data, gal.ctypes.data, gad.ctypes.data, C.byref(st))
 check(f"adjoint pcg rc {rc} rel {rel(lam, xo):.1e}", rc == 0 and rel(lam, xo) < 1e-6)
+x = np.zeros(s.neq); st = St(); cyc = z["pois_gm_cyc"]
+rc = lib.sfr_solve_device(ctx, A, 4, 0, 1e-8, maxit, b.ctypes.data, x.ctypes.data, 1, C.byref(st))
+check(f"gmres pois rc {rc} cycles {st.outer}/{cyc[0]} it {st.iters} rel {rel(x, z['pois_gm_xo']):.1e}",
+      rc == 0 and abs(int(st.outer) - int(cyc[0])) <= 1 and rel(x, z["pois_gm_xo"]) < 1e-6)
 lib.sfr_matrix_destroy(ctx, A)
 # momentum: BiCGSTAB, shared batch, adjoint (transposed operator)
 s, maxit = mk("mom"); A = vp(); assert lib.sfr_matrix_create(ctx, C.byref(s), 0, C.byref(A)) == 0
@@ -48,6 +53,10 @@ bb = np.ascontiguousarray(z["mom_b"]); xo = z["mom_xo"]
 x = np.zeros(s.neq); st = St()
 rc = lib.sfr_solve_device(ctx, A, 2, 0, 1e-8, maxit, bb[0].ctypes.data, x.ctypes.data, 1, C.byref(st))
 check(f"bicg rc {rc} it {st.iters} rel {rel(x, xo):.1e}", rc == 0 and rel(x, xo) < 1e-6)
+xg = np.zeros(s.neq); st = St(); cyc = z["mom_gm_cyc"]
+rc = lib.sfr_solve_device(ctx, A, 4, 0, 1e-10, maxit, bb[0].ctypes.data, xg.ctypes.data, 1, C.byref(st))
+check(f"gmres mom rc {rc} cycles {st.outer}/{cyc[0]} it {st.iters}/{cyc[1]} rel {rel(xg, z['mom_gm_xo']):.1e}",
+      rc == 0 and st.outer == cyc[0] and st.iters == cyc[1] and rel(xg, z["mom_gm_xo"]) < 1e-8)
 xs = np.zeros(3 * s.neq); sts = (St * 3)(); flat = np.ascontiguousarray(bb.reshape(-1))
 rc = lib.sfr_solve_shared_device(ctx, A, 3, None, flat.ctypes.data, xs.ctypes.data, 2, 1e-8, maxit, 1, C.cast(sts, vp))
 check(f"shared rc {rc} rel {rel(xs[:s.neq], xo):.1e}", rc == 0 and rel(xs[:s.neq], x) < 1e-12)

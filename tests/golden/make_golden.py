@@ -10,6 +10,8 @@ tree and oracle/_ref exist: `python tests/golden/make_golden.py`).
                              (oracle/_ref): GENRCM orderings (digest + first
                              entries) and solverC results on small systems
   ref_solutions.npz          the solution vectors of those solves
+  ref_gmres.json,            solverC with GMRES(15) (cycles, iterations, xKx,
+  ref_gmres_solutions.npz    ||x||, digest of x) and the solution vectors
   cavity_ref.json            solverC (PCG, tol 1e-8) on pressure-correction
                              systems of the 128x128 lid-driven cavity (SIMPLE
                              outer iterations 0, 5, 11; CSRD and ELLPACK):
@@ -109,6 +111,47 @@ def ref_vectors():
     np.savez_compressed(os.path.join(HERE, "ref_solutions.npz"), **sols)
 
 
+GMRES_CASES = [
+    ("gmres_poisson3d_12_csrd", "poisson3d", dict(n=12, seed=3, storage=2), None),
+    ("gmres_poisson3d_12_ell", "poisson3d", dict(n=12, seed=3, storage=4), None),
+    ("gmres_momentum_10_csrd", "momentum3d", dict(n=10), 0),
+    ("gmres_momentum_10_csrc", "momentum3d", dict(n=10, storage=3), 1),
+    ("gmres_momentum_10_ell", "momentum3d", dict(n=10, storage=4), 2),
+    ("gmres_poisson2d_20x16_pinned", "poisson2d", dict(nx=20, ny=16, seed=5, storage=2), None),
+]
+
+
+def gmres_vectors():
+    """solverC with solver GMRES (4), tol 1e-8 and 1e-11, maxIt 5000."""
+    import oracle
+    from sfr3d_amd import systems as Y
+    R = oracle.ref()
+    R.ref_init(1 << 30)
+    out, sols = {"solves": []}, {}
+    for name, fn, kw, comp in GMRES_CASES:
+        B = getattr(Y, fn)(**kw)
+        s = B.system
+        b = B.b if comp is None else B.extra["b_all"][comp]
+        for tol in (1e-8, 1e-11):
+            x = np.zeros(s.neq)
+            res = np.zeros(5)
+            ad = s.ad if s.ad is not None else s.al
+            rc = R.ref_solve(s.neq, s.nad, s.ia, s.ja, oracle._al_contig(s),
+                             np.ascontiguousarray(ad), None,
+                             np.ascontiguousarray(b), x, tol, 5000, s.storage,
+                             4, 1, int(s.unsym), res, None, 0)
+            assert rc == 0
+            key = f"{name}_tol{tol:.0e}"
+            out["solves"].append({"name": key, "iters": int(res[0]),
+                                  "cycles": int(res[4]), "xKx": res[1],
+                                  "norm_x": res[2], "x_sha256": sha(x)})
+            sols[key] = x
+            print("gmres", key, int(res[4]), int(res[0]), flush=True)
+    with open(os.path.join(HERE, "ref_gmres.json"), "w") as f:
+        json.dump(out, f, indent=1)
+    np.savez_compressed(os.path.join(HERE, "ref_gmres_solutions.npz"), **sols)
+
+
 CAVITY_N, CAVITY_OUTER, CAVITY_PICK = 128, 12, (0, 5, 11)
 
 
@@ -147,10 +190,12 @@ def cavity_vectors():
 
 
 if __name__ == "__main__":
-    what = sys.argv[1:] or ["mesh", "ref", "cavity"]
+    what = sys.argv[1:] or ["mesh", "ref", "gmres", "cavity"]
     if "mesh" in what:
         mesh_fixture()
     if "ref" in what:
         ref_vectors()
+    if "gmres" in what:
+        gmres_vectors()
     if "cavity" in what:
         cavity_vectors()

@@ -2,8 +2,9 @@
 with g++ against a CPU emulation of the HIP kernel model (tests/emu/), and
 the resulting library is driven through the same C ABI as the GPU build.
 This checks kernel logic -- indexing, the scalar protocol between kernels,
-graph capture and replay, the fused mixed-precision iteration, escaped
-16-bit columns, the uneven batch, device assembly -- against the oracle and
+graph capture and replay, the fused mixed-precision iteration, GMRES's
+restart cycle, escaped 16-bit columns, the uneven batch, device assembly --
+against the oracle and
 the host assembler without a GPU.
 Set SFR_EMU_ASAN=1 to build it under AddressSanitizer (slower), which
 turns any out-of-bounds access of a kernel or of the host code into a
@@ -41,6 +42,11 @@ def write_cases(path):
     put("pois", B.system, B.b, 1, 1000)
     M = systems.momentum3d(5)
     put("mom", M.system, M.extra["b_all"], 2, 1000)
+    xg, sg = oracle.gmres(M.system, M.extra["b_all"][0], 1e-10, 1000)
+    d["mom_gm_xo"], d["mom_gm_cyc"] = xg, np.array([sg["cycles"], sg["iters"]])
+    B = systems.poisson3d(6, seed=11)
+    xg, sg = oracle.gmres(B.system, B.b, 1e-8, 1000)
+    d["pois_gm_xo"], d["pois_gm_cyc"] = xg, np.array([sg["cycles"], sg["iters"]])
     # > 32768 rows with the pinned cell near the end: its neighbours' slots
     # point at column 0 (SFR3D's ELLPACK) with the face coefficient in them
     n = 33
@@ -87,4 +93,4 @@ def test_kernels_under_cpu_emulation(tmp_path):
     print(res.stdout)
     assert res.returncode == 0, res.stdout + res.stderr[-3000:]
     assert "SMOKE OK" in res.stdout
-    assert res.stdout.count("PASS") == 9
+    assert res.stdout.count("PASS") == 11
