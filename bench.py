@@ -152,6 +152,7 @@ def main():
         extra["momentum_uvw"] = bench_uvw(ctx, S, systems, lib)
         extra["uneven_batch"] = bench_uneven(ctx, S, systems)
         extra["adjoint"] = bench_adjoint(ctx, S, systems)
+        extra["gmres"] = bench_gmres(ctx, S, systems, cpu=not args.no_cpu)
 
     out = {
         "metric": "Krylov solves/s + HBM GB/s, 128³ Poisson, mixed vs FP64 @ relres 1e-8",
@@ -438,6 +439,69 @@ def bench_adjoint(ctx, S, systems, reps=5):
     for d in (db, dx, dg, dl, dgal, dgad):
         d.free()
     return out
+
+
+def bench_gmres(ctx, S, systems, cpu=True, reps=3):
+    """GMRES(15) FP64 (solver id 4) on the 96^3 upwind momentum system
+    (u component), device times; BiCGSTAB on the same system beside it; the
+    oracle's serial gmres timed over one restart cycle as the CPU leg."""
+    M = systems.momentum3d(96)
+    n = M.system.neq
+    b = M.extra["b_all"][0]
+    A = ctx.matrix(M.system)
+    info = A.info()
+    db, dx = ctx.array(b), ctx.empty(n)
+    st = ctx.solve_device(A, S.GMRES, TOL, 5000, db, dx)
+    tg = []
+    for _ in range(reps):
+        t = time.perf_counter()
+        st = ctx.solve_device(A, S.GMRES, TOL, 5000, db, dx)
+        tg.append(time.perf_counter() - t)
+    tgm = float(np.median(tg))
+    sb = ctx.solve_device(A, S.PBICGSTAB, TOL, 5000, db, dx)
+    tb = []
+    for _ in range(reps):
+        t = time.perf_counter()
+        sb = ctx.solve_device(A, S.PBICGSTAB, TOL, 5000, db, dx)
+        tb.append(time.perf_counter() - t)
+    cycles = st["outer"]
+    # algorithmic bytes per row of one restart cycle: 16 SpMV-type kernels
+    # (m = s*(8 + ib) + ad + gathered x) with their vector streams, the 120
+    # fused Gram-Schmidt kernels (w read+written, g_j, g_j+1) and the update
+    # (14 basis vectors, x read+written)
+    m = info["slots"] * (8 + info["index_bytes"]) + 16
+    per_row = 16 * m + 24 + 14 * 32 + 24 + 3720 + 128
+    cyc_s = tgm / max(cycles, 1)
+    out = {"workload": "96^3 Taylor-Green upwind momentum (u), CSRD storage, "
+                       "Jacobi-GMRES(15) FP64 to tol 1e-8",
+           "neq": n, "solve_ms": round(tgm * 1e3, 3), "cycles": cycles,
+           "iters": st["iters"], "relres_true": st["relres"],
+           "ref_criterion": [st["norm_r_m"], st["conv"]],
+           "ms_per_cycle": round(cyc_s * 1e3, 3),
+           "cycle_GBs_algorithmic": round(n * per_row / cyc_s / 1e9, 1),
+           "bicgstab_ms": round(float(np.median(tb)) * 1e3, 3),
+           "bicgstab_iters": sb["iters"]}
+    if cpu:
+        out["cpu_baseline"] = cpu_baseline_gmres(M.system, b)
+        out["cpu_baseline"]["gpu_speedup_per_cycle"] = round(
+            out["cpu_baseline"]["ms_per_cycle"] / (cyc_s * 1e3), 1)
+    log(f"[bench] GMRES 96^3 momentum: {out['solve_ms']} ms, {cycles} cycles "
+        f"({out['ms_per_cycle']} ms/cycle, {out['cycle_GBs_algorithmic']} GB/s); "
+        f"BiCGSTAB {out['bicgstab_ms']} ms")
+    A.destroy(); db.free(); dx.free()
+    return out
+
+
+def cpu_baseline_gmres(sysm, b):
+    """The oracle's serial gmres (the reference's src/Iters.c:587-790
+    restated), one restart cycle of the same system, on one host core."""
+    import oracle
+    t = time.perf_counter()
+    _, so = oracle.gmres(sysm, b, TOL, 1)
+    dt = time.perf_counter() - t
+    return {"ms_per_cycle": round(dt * 1e3, 2), "cores": 1, "kind": "port",
+            "sample": "one GMRES(15) restart cycle of the 96^3 momentum system "
+                      "(oracle, serial C)"}
 
 
 def cpu_baseline(sysm, b, threads, x_gpu64):

@@ -212,21 +212,39 @@ __global__ __launch_bounds__(SFR_BLOCK) void k_gm_mgs(int64_t n,
     const double *pin, int gin, double *pout, sfr_scalars *sc)
 {
   if (!sc->run2) return;
+  constexpr int RPC = RPT * SFR_BLOCK;
+  // the first chunk's rows are loaded before the reduction (hides it)
+  const int64_t c0 = xcd_chunk(blockIdx.x, gridDim.x);
+  double wv[RPT], gv[RPT], nv[RPT];
+#pragma unroll
+  for (int r_ = 0; r_ < RPT; r_++) {
+    const int64_t i = c0 * RPC + r_ * SFR_BLOCK + threadIdx.x;
+    wv[r_] = i < n ? w[i] : 0.0;
+    gv[r_] = i < n ? gj[i] : 0.0;
+    nv[r_] = (i < n && gnext) ? gnext[i] : 0.0;
+  }
   double t[1];
   reduce_partials<1>(pin, gin, t);
   const double beta = t[0];
   if (blockIdx.x == 0 && threadIdx.x == 0) st[hidx] = beta;
-  constexpr int RPC = RPT * SFR_BLOCK;
   double acc[1] = {0.0};
-  for (int64_t c = xcd_chunk(blockIdx.x, gridDim.x); c * RPC < n;
-       c += gridDim.x) {
+  for (int64_t c = c0; c * RPC < n; c += gridDim.x) {
+    if (c != c0) {
+#pragma unroll
+      for (int r_ = 0; r_ < RPT; r_++) {
+        const int64_t i = c * RPC + r_ * SFR_BLOCK + threadIdx.x;
+        wv[r_] = i < n ? w[i] : 0.0;
+        gv[r_] = i < n ? gj[i] : 0.0;
+        nv[r_] = (i < n && gnext) ? gnext[i] : 0.0;
+      }
+    }
 #pragma unroll
     for (int r_ = 0; r_ < RPT; r_++) {
       const int64_t i = c * RPC + r_ * SFR_BLOCK + threadIdx.x;
       if (i < n) {
-        const double wi = w[i] - beta * gj[i];
+        const double wi = wv[r_] - beta * gv[r_];
         w[i] = wi;
-        acc[0] += wi * (gnext ? gnext[i] : wi);
+        acc[0] += wi * (gnext ? nv[r_] : wi);
       }
     }
   }

@@ -35,7 +35,7 @@ enum { G_PCG = 1, G_BICG = 2, G_MX = 3, G_BB = 4, G_GM = 5 };
 /* GMRES restart cycles per captured chunk (137 kernels each) */
 static int gm_cycles_per_chunk(const sfr_context *ctx)
 {
-  const int c = ctx->chunk / 8;
+  const int c = ctx->chunk / 16;
   return c < 1 ? 1 : c;
 }
 
@@ -115,15 +115,16 @@ static int get_graph(sfr_context *ctx, int kind, const void *k0,
 
 /* Replay the loop graph until the device reports the loop finished for
  * every one of the nsys systems: which = 0 waits for sc->done, which = 1 for
- * sc->done or sc->ru (a mixed-precision fold request).  Two chunks are kept
- * in flight; after each one the scalar blocks are copied to pinned memory. */
-static int run_loop(sfr_context *ctx, hipGraphExec_t ex, int which,
-                    unsigned max_chunks, int nsys)
+ * sc->done or sc->ru (a mixed-precision fold request).  `inflight` chunks
+ * (1 or 2) are kept in flight; after each one the scalar blocks are copied
+ * to pinned memory. */
+static int run_loop_n(sfr_context *ctx, hipGraphExec_t ex, int which,
+                      unsigned max_chunks, int nsys, unsigned inflight)
 {
   hipStream_t s = ctx->stream;
   unsigned launched = 0, seen = 0;
   for (;;) {
-    while (launched - seen < 2 && launched < max_chunks) {
+    while (launched - seen < inflight && launched < max_chunks) {
       const int slot = (int)(launched & 1);
       TRY(hipGraphLaunch(ex, s));
       TRY(hipMemcpyAsync(ctx->h_poll + SFR_MAXSYS * slot, ctx->d_sc,
@@ -142,6 +143,12 @@ static int run_loop(sfr_context *ctx, hipGraphExec_t ex, int which,
     if (finished) break;
   }
   return SFR_OK;
+}
+
+static int run_loop(sfr_context *ctx, hipGraphExec_t ex, int which,
+                    unsigned max_chunks, int nsys)
+{
+  return run_loop_n(ctx, ex, which, max_chunks, nsys, 2);
 }
 
 void sfr_fill_stats(const sfr_scalars *sc, sfr_stats *st, float ms)
@@ -240,8 +247,11 @@ static int solve_gmres64(sfr_context *ctx, sfr_matrix *A, double tol,
   TRY(hipEventRecord(ctx->ev_start, ctx->stream));
   TRY(sfr_k_gm_setup(ctx->stream, &v, g, d_b, d_x, newX, tol, maxit, ctx->d_sc,
                      ctx->part));
+  /* A restart cycle of a large system outlasts the host's poll turnaround
+   * many times over: one chunk in flight then costs less than a second
+   * chunk's early-exit launches after convergence. */
   const unsigned cyc = (unsigned)gm_cycles_per_chunk(ctx);
-  CHECK(run_loop(ctx, ex, 0, maxit / cyc + 2, 1));
+  CHECK(run_loop_n(ctx, ex, 0, maxit / cyc + 2, 1, A->n >= (1 << 17) ? 1u : 2u));
   sfr_stats tmp;
   const int rc = finish(ctx, &v, g, d_b, d_x, &tmp);
   if (rc == SFR_HIP_ERROR) return rc;
