@@ -1,7 +1,7 @@
 """ORACLE -- test infrastructure only.
 
 CPU restatement of the reference's linear-solve path (sfr_oracle.c: pcg,
-pbicgstab, gmres, the serial matvecs, dot, preCondDiag) and of its index builders
+pbicgstab, gmres, pminres, the serial matvecs, dot, preCondDiag) and of its index builders
 (patterns.py: convGraph, GENRCM, numEqV1/V2, csrIa/csrJa, ellPackJa, the csr/
 ellPack value scatter), plus, where the reference tree exists, a harness
 around the reference's own compiled C (_ref/libsfr_ref.so, built by
@@ -47,7 +47,7 @@ def orc():
         if not os.path.exists(ORACLE_LIB):
             raise ImportError(f"{ORACLE_LIB} missing: run `make -C oracle`")
         lib = C.CDLL(ORACLE_LIB)
-        for name in ("orc_pcg", "orc_pbicgstab"):
+        for name in ("orc_pcg", "orc_pbicgstab", "orc_minres"):
             f = getattr(lib, name)
             f.restype = C.c_int
             f.argtypes = [C.c_int, C.c_int, i32p, i32p, f64p, f64p, f64p, f64p,
@@ -117,6 +117,11 @@ def pcg(system, b, tol, maxit, x=None, newX=True):
 
 def pbicgstab(system, b, tol, maxit, x=None, newX=True):
     return _krylov("orc_pbicgstab", system, b, tol, maxit, x, newX)
+
+
+def minres(system, b, tol, maxit, x=None, newX=True):
+    """Reference-order serial Jacobi-MINRES (pminres) (oracle)."""
+    return _krylov("orc_minres", system, b, tol, maxit, x, newX)
 
 
 def gmres(system, b, tol, ncycles, x=None, newX=True, nkrylov=15):

@@ -169,6 +169,9 @@ int ref_solve(int neq, int nad, int *ia, int *ja, double *al, double *ad,
   if (solver == PCG) {
     if (sscanf(buf, "PCG: tol %lf iteration %d xKx %lf norma(x*x) %lf time %lf",
                &t, &it, &xkx, &nx, &tm_) != 5) return -3;
+  } else if (solver == MINRES) {
+    if (sscanf(buf, "PMINRES: tol %lf iteration %d xKx %lf norma(x*x) %lf time %lf",
+               &t, &it, &xkx, &nx, &tm_) != 5) return -3;
   } else if (solver == GMRES) {
     int cyc, nk;
     if (sscanf(buf, "GMRES: tol %lf Cycles %d iteration %d nKrylov %d xKx %lf "

@@ -21,6 +21,8 @@
  *   pcg              reference src/Cg.c:47-209, driven as src/Solv.c:93-145
  *   pbicgstab        reference src/Bicgstab.c:50-245, as src/Solv.c:150-212
  *   gmres            reference src/Iters.c:587-790, as src/Solv.c:302-372
+ *   pminres          reference src/Iters.c:309-555, as src/Solv.c:380-445
+ *   symOrtho         reference src/Iters.c:1171-1208
  * The OpenMP variants (orc_*_omp) keep the same recurrences with parallel
  * loops and reductions over full rows; they are the all-cores CPU baseline
  * (the reference's pcgOmp, src/Cg.c:490-715, plays that role there).
@@ -411,6 +413,127 @@ int orc_gmres(int n, int nad, const int *ia, const int *ja, const double *al,
   }
   if (cycles_out) *cycles_out = (l == ncycles) ? l : l + 1;
   free(m); free(g); free(h); free(e); free(c); free(s); free(y);
+  return 0;
+}
+
+/* Plane rotation of src/Iters.c:1171-1208 (symOrtho): c, s, r with
+ * [c s; -s c] [a; b] = [r; 0].  Kept as written: the third test compares
+ * |b| with a itself, and when none of the four cases holds (a == b > 0) the
+ * outputs keep the values they had. */
+static void sym_ortho(double a, double b, double *c, double *s, double *r)
+{
+  const double ma = fabs(a), mb = fabs(b);
+  const double sa = a > 0.0 ? 1.0 : -1.0, sb = b > 0.0 ? 1.0 : -1.0;
+  if (b == 0.0) {
+    *s = 0.0;
+    *r = ma;
+    *c = a == 0.0 ? 1.0 : sa;
+  } else if (a == 0.0) {
+    *c = 0.0;
+    *s = sb;
+    *r = mb;
+  } else if (mb > a) {
+    const double t = a / b;
+    *s = sb / sqrt(1.0 + t * t);
+    *c = *s * t;
+    *r = b / *s;
+  } else if (ma > mb) {
+    const double t = b / a;
+    *c = sa / sqrt(1.0 + t * t);
+    *s = *c * t;
+    *r = a / *c;
+  }
+}
+
+/* Jacobi-preconditioned MINRES, reference src/Iters.c:309-555 (pminres),
+ * driven as src/Solv.c:380-445: conv = tol ||D^-1/2 b||; the Lanczos
+ * vectors v, z = D^-1 v with beta = sqrt(z.v); the QR step by the rotation
+ * above; the stop test on the recurrence |s| ||r||.  Reported iterations:
+ * the loop index + 1 at the break; at maxit the reference prints a warning
+ * and returns (no exit), status 1 here, iters = maxit. */
+int orc_minres(int n, int nad, const int *ia, const int *ja, const double *al,
+               const double *ad, const double *b, double *x, double tol,
+               unsigned maxit, int newX, int storage, int unsym,
+               orc_stats *st)
+{
+  (void)nad;
+  matvec_fn matvec = pick(storage, unsym);
+  if (!matvec || n <= 0) return -1;
+  const size_t N = (size_t)n;
+  double *buf = calloc(N * 10, sizeof(double));
+  if (!buf) return -2;
+  double *m = buf, *v0 = buf + N, *v = buf + 2 * N, *w = buf + 3 * N;
+  double *w0 = buf + 4 * N, *w00 = buf + 5 * N, *z = buf + 6 * N;
+  double *z0 = buf + 7 * N, *p = buf + 8 * N;
+  for (int i = 0; i < n; i++) m[i] = 1.0 / ad[i];
+  const double t0 = now();
+  double ro1 = 1.0, ro2, ro3;
+  if (newX) for (int i = 0; i < n; i++) x[i] = 0.0;
+  const double norm_b = sqrt(orc_dot(b, b, n));
+  for (int i = 0; i < n; i++) z[i] = b[i] * sqrt(m[i]);
+  const double conv = tol * sqrt(orc_dot(z, z, n));
+  matvec(n, ia, ja, al, ad, x, z);
+  for (int i = 0; i < n; i++) {
+    v[i] = b[i] - z[i];
+    v0[i] = w0[i] = w00[i] = 0.0;
+    z0[i] = v[i] * m[i];
+  }
+  double beta = sqrt(fabs(orc_dot(v, z0, n)));
+  double norm_r = beta, neta = beta, beta_old = beta;
+  double c_old = 1.0, c = 1.0, s_old = 0.0, s = 0.0;
+  unsigned j;
+  for (j = 0; j < maxit; j++) {
+    matvec(n, ia, ja, al, ad, z0, p);
+    double tmp1 = 1.0 / (beta * beta);
+    const double alpha = orc_dot(z0, p, n) * tmp1;
+    tmp1 = 1.0 / beta;
+    double tmp2 = -alpha / beta, tmp3 = -beta / beta_old;
+    for (int i = 0; i < n; i++) {
+      p[i] = tmp1 * p[i] + tmp2 * v[i] + tmp3 * v0[i];
+      z[i] = p[i] * m[i];
+    }
+    beta_old = beta;
+    beta = sqrt(orc_dot(z, p, n));
+    const double delta = c * alpha - c_old * s * beta_old;
+    ro2 = s * alpha + c_old * c * beta_old;
+    ro3 = s_old * beta_old;
+    c_old = c;
+    s_old = s;
+    sym_ortho(delta, beta, &c, &s, &ro1);
+    tmp1 = 1.0 / (beta_old * ro1);
+    tmp2 = -ro2 / ro1;
+    tmp3 = -ro3 / ro1;
+    const double tmp4 = c * neta;
+    for (int i = 0; i < n; i++) {
+      w[i] = tmp1 * z0[i] + tmp2 * w0[i] + tmp3 * w00[i];
+      x[i] += tmp4 * w[i];
+      v0[i] = v[i];
+      v[i] = p[i];
+      z0[i] = z[i];
+      w00[i] = w0[i];
+      w0[i] = w[i];
+    }
+    norm_r = fabs(s) * norm_r;
+    if (norm_r < conv) break;
+    neta = -s * neta;
+  }
+  matvec(n, ia, ja, al, ad, x, z);
+  const double xKx = orc_dot(x, z, n);
+  const double norm = sqrt(orc_dot(x, x, n));
+  for (int i = 0; i < n; i++) {
+    v[i] = b[i] - z[i];
+    z[i] = v[i] * sqrt(fabs(m[i]));
+  }
+  const double nr = sqrt(orc_dot(v, v, n));
+  const double nrm = sqrt(orc_dot(z, z, n));
+  const double t1 = now();
+  if (st) {
+    st->status = j == maxit ? 1 : 0;
+    st->iters = j == maxit ? maxit : j + 1;
+    st->xKx = xKx; st->norm_x = norm; st->norm_r = nr; st->norm_r_m = nrm;
+    st->norm_b = norm_b; st->conv = conv; st->time_s = t1 - t0;
+  }
+  free(buf);
   return 0;
 }
 

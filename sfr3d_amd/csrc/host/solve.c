@@ -30,7 +30,21 @@ int sfr_workspace(sfr_context *ctx, int64_t n);
 int sfr_matrix_prepare_mixed(sfr_context *ctx, sfr_matrix *A);
 int sfr_matrix_prepare_adjoint(sfr_context *ctx, sfr_matrix *A);
 
-enum { G_PCG = 1, G_BICG = 2, G_MX = 3, G_BB = 4, G_GM = 5 };
+enum { G_PCG = 1, G_BICG = 2, G_MX = 3, G_BB = 4, G_GM = 5, G_MR = 6 };
+
+/* MINRES ring buffers in the workspace (w64[8], w64[9] stay free for the
+ * host-pointer path's b and x) */
+#define MR_RINGS(ctx)                                                          \
+  double *const V_[3] = {(ctx)->w64[0], (ctx)->w64[1], (ctx)->w64[2]};         \
+  double *const W_[3] = {(ctx)->w64[3], (ctx)->w64[4], (ctx)->w64[5]};         \
+  double *const Z_[2] = {(ctx)->w64[6], (ctx)->w64[7]}
+
+/* MINRES iterations per captured chunk: a multiple of the rings' period 6 */
+static int mr_chunk(const sfr_context *ctx)
+{
+  const int c = (ctx->chunk / 6) * 6;
+  return c < 6 ? 6 : c;
+}
 
 /* GMRES restart cycles per captured chunk (137 kernels each) */
 static int gm_cycles_per_chunk(const sfr_context *ctx)
@@ -47,7 +61,8 @@ static int get_graph(sfr_context *ctx, int kind, const void *k0,
   /* the mixed loop alternates its direction buffers: an even chunk ends
    * where it started */
   const int chunk = kind == G_MX ? (ctx->chunk + 1) & ~1
-                  : kind == G_GM ? gm_cycles_per_chunk(ctx) : ctx->chunk;
+                  : kind == G_GM ? gm_cycles_per_chunk(ctx)
+                  : kind == G_MR ? mr_chunk(ctx) : ctx->chunk;
   for (int i = 0; i < ctx->ngraph; i++) {
     sfr_graph *e = &ctx->graphs[i];
     if (e->kind == kind && e->chunk == chunk && e->key[0] == k0 &&
@@ -85,6 +100,10 @@ static int get_graph(sfr_context *ctx, int kind, const void *k0,
       double *const wr[2] = {basis + SFR_GM_K * ld, basis + (SFR_GM_K + 1) * ld};
       lrc = sfr_k_gm_cycle(s, v, g, (const double *)k2, (double *)k1, basis, ld, wr,
                            basis + (SFR_GM_K + 2) * ld, ctx->d_sc, ctx->part);
+    }
+    else if (kind == G_MR) {
+      MR_RINGS(ctx);
+      lrc = sfr_k_mr_iter(s, v, g, (double *)k1, V_, W_, Z_, it % 6, ctx->d_sc, ctx->part);
     }
     else                    /* G_BB: k2 carries the batch size */
       lrc = sfr_k_bb_iter(s, v, g, (int)(intptr_t)k2, w[8], w[9], w[0], w[1], w[2], w[3],
@@ -263,6 +282,25 @@ static int solve_gmres64(sfr_context *ctx, sfr_matrix *A, double tol,
   return rc;
 }
 
+/* MINRES, reference src/Iters.c:309-555 as src/Solv.c:380-445 runs it.
+ * iters = iterations performed (the reference's j+1 on convergence). */
+static int solve_minres64(sfr_context *ctx, sfr_matrix *A, double tol,
+                          unsigned maxit, const double *d_b, double *d_x,
+                          int newX, sfr_stats *st)
+{
+  CHECK(sfr_workspace(ctx, A->n));
+  sfr_mat_view v = sfr_view(A);
+  sfr_geom g = {A->rpt, 0};
+  hipGraphExec_t ex;
+  CHECK(get_graph(ctx, G_MR, A, d_x, NULL, &v, g, &ex));
+  MR_RINGS(ctx);
+  TRY(hipEventRecord(ctx->ev_start, ctx->stream));
+  TRY(sfr_k_mr_setup(ctx->stream, &v, g, d_b, d_x, V_, W_, Z_, newX, tol, maxit,
+                     ctx->d_sc, ctx->part));
+  CHECK(run_loop(ctx, ex, 0, maxit / (unsigned)mr_chunk(ctx) + 2, 1));
+  return finish(ctx, &v, g, d_b, d_x, st);
+}
+
 static int solve_pcg_mixed(sfr_context *ctx, sfr_matrix *A, double tol,
                            unsigned maxit, const double *d_b, double *d_x,
                            int newX, sfr_stats *st)
@@ -426,6 +464,11 @@ int sfr_solve_device(sfr_context *ctx, sfr_matrix *A, int solver,
       return sfr_set_error(ctx, SFR_NOT_SUPPORTED, "mixed GMRES");
     return solve_gmres64(ctx, A, tol, maxit, d_b, d_x, newX, st);
   }
+  if (solver == SFR_MINRES) {
+    if (precision == SFR_PREC_MIXED)
+      return sfr_set_error(ctx, SFR_NOT_SUPPORTED, "mixed MINRES");
+    return solve_minres64(ctx, A, tol, maxit, d_b, d_x, newX, st);
+  }
   return sfr_set_error(ctx, SFR_BAD_ARG, "solver id %d", solver);
 }
 
@@ -549,7 +592,8 @@ void sfr_solverC(void *m, int nEq, int nEqNov, int nAd, int nAdR, int *ia,
                  const char *name)
 {
   (void)m; (void)iNeq; (void)bOmp;
-  if (solver != SFR_PCG && solver != SFR_PBICGSTAB && solver != SFR_GMRES) {
+  if (solver != SFR_PCG && solver != SFR_PBICGSTAB && solver != SFR_GMRES &&
+      solver != SFR_MINRES) {
     fprintf(stderr, "Opcao %d e invalida!!\n", solver);
     fprintf(stderr, "Arquivo:%s\nFonte:  %s\n", __FILE__, __func__);
     exit(EXIT_FAILURE);
@@ -580,12 +624,16 @@ void sfr_solverC(void *m, int nEq, int nEqNov, int nAd, int nAdR, int *ia,
     fprintf(stderr, "sfr_solverC: %s: %s\n", sfr_status_string(rc), sfr_last_error(g_ctx));
     exit(EXIT_FAILURE);
   }
+  if (rc != SFR_OK && solver == SFR_MINRES) { /* src/Iters.c:540-544: no exit */
+    printf("PMINRES: *** WARNING: no convergende reached !\n");
+    printf("MAXIT = %d \n", maxIt);
+  }
   if (rc != SFR_OK && solver == SFR_GMRES) {  /* src/Iters.c:770-774 */
     printf(" GMRES *** WARNING: no convergende reached !\n");
     printf("MAXIT = %d \n", maxIt);
     exit(EXIT_FAILURE);
   }
-  if (rc != SFR_OK) {                 /* j == maxIt (and breakdowns) */
+  if (rc != SFR_OK && solver != SFR_MINRES) { /* j == maxIt (and breakdowns) */
     printf("name : %s\n", name ? name : "");
     if (solver == SFR_PCG) {
       printf("PCG: *** WARNING: no convergende reached !\n");
@@ -600,6 +648,10 @@ void sfr_solverC(void *m, int nEq, int nEqNov, int nAd, int nAdR, int *ia,
     if (solver == SFR_PCG)
       fprintf(fSolvLog, "PCG: tol %20.2e iteration %d xKx %20.12e norma(x*x) %20.12e "
               "time %20.5lf\n", tol, (int)st.iters, st.xKx, st.norm_x, st.time_s);
+    else if (solver == SFR_MINRES)         /* src/Iters.c:545-552; j+1 at maxIt */
+      fprintf(fSolvLog, "PMINRES: tol %20.2e iteration %d xKx %20.12e norma(x*x) %20.12e "
+              "time %20.5lf\n", tol, (int)st.iters + (rc != SFR_OK ? 1 : 0), st.xKx,
+              st.norm_x, st.time_s);
     else if (solver == SFR_GMRES)          /* src/Iters.c:778-786 */
       fprintf(fSolvLog, "GMRES: tol %20.2e  Cycles %d iteration %d  nKrylov %d  xKx %20.12e "
               " norma(x*x) %20.12e  time %20.5lf\n", tol, (int)st.outer, (int)st.iters,

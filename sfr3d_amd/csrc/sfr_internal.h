@@ -184,6 +184,16 @@ int sfr_k_gm_cycle(hipStream_t s, const sfr_mat_view *A, sfr_geom g,
                    double *const w[2], double *st, sfr_scalars *sc,
                    double *part);
 
+/* MINRES, FP64 (reference src/Iters.c:309-555); V, W: rings of 3 vectors,
+ * Z: ring of 2; one iteration per call at ring phase q = j % 6 */
+int sfr_k_mr_setup(hipStream_t s, const sfr_mat_view *A, sfr_geom g,
+                   const double *b, double *x, double *const V[3],
+                   double *const W[3], double *const Z[2], int newX,
+                   double tol, unsigned maxit, sfr_scalars *sc, double *part);
+int sfr_k_mr_iter(hipStream_t s, const sfr_mat_view *A, sfr_geom g, double *x,
+                  double *const V[3], double *const W[3], double *const Z[2],
+                  int q, sfr_scalars *sc, double *part);
+
 /* BiCGSTAB, FP64 (reference src/Bicgstab.c:50-245) */
 int sfr_k_bicg_setup(hipStream_t s, const sfr_mat_view *A, sfr_geom g,
                      const double *b, double *x, double *r, double *r0,

@@ -2,7 +2,8 @@
 
 Mirrors the reference's solver boundary, solverC (reference
 include/Solv.h:59-70): a system in one of SFR3D's storages, a solver id
-(PCG = 1, PBICGSTAB = 2, GMRES = 4), tol, maxit, b and x.  Everything here calls the
+(PCG = 1, PBICGSTAB = 2, GMRES = 4, MINRES = 5), tol, maxit, b and
+x.  Everything here calls the
 HIP library; there is no CPU path.
 """
 import ctypes as C
@@ -211,6 +212,11 @@ class Context:
         if self.h:
             lib.sfr_finalize(self.h)
             self.h = None
+
+    def set_option(self, name, value):
+        """sfr_set_option: "chunk" (iterations per captured graph chunk) or
+        "ru_delta" (mixed-precision reliable-update threshold)."""
+        self._check(lib.sfr_set_option(self.h, name.encode(), float(value)))
 
     def __enter__(self):
         return self

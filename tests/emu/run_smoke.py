@@ -1,7 +1,7 @@
 """TEST INFRASTRUCTURE -- runs the emulated library (argv[1]) on the small
 cases of argv[2] (written by tests/test_emulated_kernels.py with the
 oracle's solutions) in a process of its own: FP64 and mixed PCG, BiCGSTAB,
-GMRES,
+GMRES, MINRES,
 the shared-pattern batch, both adjoint paths, 16-bit column offsets with
 escaped far columns, and a tuned uneven batch.  Prints PASS/FAIL lines and
 exits non-zero on any failure."""
@@ -46,6 +46,10 @@ x = np.zeros(s.neq); st = St(); cyc = z["pois_gm_cyc"]
 rc = lib.sfr_solve_device(ctx, A, 4, 0, 1e-8, maxit, b.ctypes.data, x.ctypes.data, 1, C.byref(st))
 check(f"gmres pois rc {rc} cycles {st.outer}/{cyc[0]} it {st.iters} rel {rel(x, z['pois_gm_xo']):.1e}",
       rc == 0 and abs(int(st.outer) - int(cyc[0])) <= 1 and rel(x, z["pois_gm_xo"]) < 1e-6)
+x = np.zeros(s.neq); st = St(); itm = int(z["pois_mr_it"][0])
+rc = lib.sfr_solve_device(ctx, A, 5, 0, 1e-8, maxit, b.ctypes.data, x.ctypes.data, 1, C.byref(st))
+check(f"minres pois rc {rc} it {st.iters}/{itm} rel {rel(x, z['pois_mr_xo']):.1e}",
+      rc == 0 and abs(int(st.iters) - itm) <= 2 and rel(x, z["pois_mr_xo"]) < 1e-6)
 lib.sfr_matrix_destroy(ctx, A)
 # momentum: BiCGSTAB, shared batch, adjoint (transposed operator)
 s, maxit = mk("mom"); A = vp(); assert lib.sfr_matrix_create(ctx, C.byref(s), 0, C.byref(A)) == 0

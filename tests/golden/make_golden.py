@@ -12,6 +12,8 @@ tree and oracle/_ref exist: `python tests/golden/make_golden.py`).
   ref_solutions.npz          the solution vectors of those solves
   ref_gmres.json,            solverC with GMRES(15) (cycles, iterations, xKx,
   ref_gmres_solutions.npz    ||x||, digest of x) and the solution vectors
+  ref_minres.json,           solverC with MINRES (logged iterations, xKx,
+  ref_minres_solutions.npz   ||x||, digest of x) and the solution vectors
   cavity_ref.json            solverC (PCG, tol 1e-8) on pressure-correction
                              systems of the 128x128 lid-driven cavity (SIMPLE
                              outer iterations 0, 5, 11; CSRD and ELLPACK):
@@ -152,6 +154,43 @@ def gmres_vectors():
     np.savez_compressed(os.path.join(HERE, "ref_gmres_solutions.npz"), **sols)
 
 
+MINRES_CASES = [
+    ("minres_poisson3d_12_csrd", "poisson3d", dict(n=12, seed=3, storage=2), 1e-8, 5000),
+    ("minres_poisson3d_12_csrc", "poisson3d", dict(n=12, seed=3, storage=3), 1e-11, 5000),
+    ("minres_poisson3d_12_ell", "poisson3d", dict(n=12, seed=3, storage=4), 1e-8, 5000),
+    ("minres_poisson2d_20x16_pinned", "poisson2d", dict(nx=20, ny=16, seed=5, storage=2), 1e-8, 5000),
+    ("minres_poisson3d_10_maxit", "poisson3d", dict(n=10, seed=7, storage=2), 1e-10, 9),
+]
+
+
+def minres_vectors():
+    """solverC with solver MINRES (5).  The reference's log line reports
+    maxIt + 1 iterations when the limit is reached (src/Iters.c:540-552)."""
+    import oracle
+    from sfr3d_amd import systems as Y
+    R = oracle.ref()
+    R.ref_init(1 << 30)
+    out, sols = {"solves": []}, {}
+    for name, fn, kw, tol, maxit in MINRES_CASES:
+        B = getattr(Y, fn)(**kw)
+        s = B.system
+        x = np.zeros(s.neq)
+        res = np.zeros(5)
+        ad = s.ad if s.ad is not None else s.al
+        rc = R.ref_solve(s.neq, s.nad, s.ia, s.ja, oracle._al_contig(s),
+                         np.ascontiguousarray(ad), None, np.ascontiguousarray(B.b),
+                         x, tol, maxit, s.storage, 5, 1, int(s.unsym), res, None, 0)
+        assert rc == 0
+        out["solves"].append({"name": name, "tol": tol, "maxit": maxit,
+                              "log_iters": int(res[0]), "xKx": res[1],
+                              "norm_x": res[2], "x_sha256": sha(x)})
+        sols[name] = x
+        print("minres", name, int(res[0]), flush=True)
+    with open(os.path.join(HERE, "ref_minres.json"), "w") as f:
+        json.dump(out, f, indent=1)
+    np.savez_compressed(os.path.join(HERE, "ref_minres_solutions.npz"), **sols)
+
+
 CAVITY_N, CAVITY_OUTER, CAVITY_PICK = 128, 12, (0, 5, 11)
 
 
@@ -190,12 +229,14 @@ def cavity_vectors():
 
 
 if __name__ == "__main__":
-    what = sys.argv[1:] or ["mesh", "ref", "gmres", "cavity"]
+    what = sys.argv[1:] or ["mesh", "ref", "gmres", "minres", "cavity"]
     if "mesh" in what:
         mesh_fixture()
     if "ref" in what:
         ref_vectors()
     if "gmres" in what:
         gmres_vectors()
+    if "minres" in what:
+        minres_vectors()
     if "cavity" in what:
         cavity_vectors()

@@ -3,7 +3,7 @@ with g++ against a CPU emulation of the HIP kernel model (tests/emu/), and
 the resulting library is driven through the same C ABI as the GPU build.
 This checks kernel logic -- indexing, the scalar protocol between kernels,
 graph capture and replay, the fused mixed-precision iteration, GMRES's
-restart cycle, escaped 16-bit columns, the uneven batch, device assembly --
+restart cycle, MINRES's ring buffers and parity-alternated scalars, escaped 16-bit columns, the uneven batch, device assembly --
 against the oracle and
 the host assembler without a GPU.
 Set SFR_EMU_ASAN=1 to build it under AddressSanitizer (slower), which
@@ -47,6 +47,8 @@ def write_cases(path):
     B = systems.poisson3d(6, seed=11)
     xg, sg = oracle.gmres(B.system, B.b, 1e-8, 1000)
     d["pois_gm_xo"], d["pois_gm_cyc"] = xg, np.array([sg["cycles"], sg["iters"]])
+    xm, sm = oracle.minres(B.system, B.b, 1e-8, 1000)
+    d["pois_mr_xo"], d["pois_mr_it"] = xm, np.array([sm["iters"]])
     # > 32768 rows with the pinned cell near the end: its neighbours' slots
     # point at column 0 (SFR3D's ELLPACK) with the face coefficient in them
     n = 33
@@ -93,4 +95,4 @@ def test_kernels_under_cpu_emulation(tmp_path):
     print(res.stdout)
     assert res.returncode == 0, res.stdout + res.stderr[-3000:]
     assert "SMOKE OK" in res.stdout
-    assert res.stdout.count("PASS") == 11
+    assert res.stdout.count("PASS") == 12
