@@ -1,7 +1,7 @@
 /*
  * sfr3d.h -- MI355X (gfx950) linear-solve path for SFR3D-assembled systems:
- * Jacobi-preconditioned CG, BiCGSTAB, GMRES(15) and MINRES over CSR-family
- * and ELLPACK matrices.
+ * Jacobi-preconditioned CG, BiCGSTAB, BiCGSTAB(2), GMRES(15) and MINRES
+ * over CSR-family and ELLPACK matrices.
  *
  * The boundary keeps the contract of the reference's solver dispatch
  * (solverC, reference include/Solv.h:59-70, src/Solv.c:64-501): a matrix in
@@ -13,6 +13,8 @@
  *   PCG      sqrt|r.D^-1 r| < tol * sqrt|b.D^-1 b|   (src/Cg.c:73-78,141-142)
  *   BiCGSTAB ||r||_2 < tol * ||b||_2, checked after each half step
  *            (src/Bicgstab.c:79-81,129-131,153-154)
+ *   BiCGSTAB(2) ||r||_2 < tol * ||b||_2 after each iteration of two BiCG
+ *            steps and a GCR(2) step (src/Bicgstab.c:1202-1205,1326-1327)
  *   GMRES    |e_k| < tol * ||D^-1 b||_2 after each restart cycle of k = 15
  *            Arnoldi steps, left Jacobi preconditioning
  *            (src/Iters.c:625-628,740); maxit counts cycles
@@ -43,6 +45,7 @@ extern "C" {
 /* solver ids (include/Define.h:392-397) */
 #define SFR_PCG        1
 #define SFR_PBICGSTAB  2
+#define SFR_PBICGSTABL2 3
 #define SFR_GMRES      4
 #define SFR_MINRES     5
 
@@ -133,9 +136,9 @@ int  sfr_matrix_index_bytes(const sfr_matrix *A);
 
 /* ---- solves ---------------------------------------------------------------
  * sfr_solve: host arrays in and out, the drop-in for solverC's PCG,
- * PBICGSTAB, GMRES and MINRES branches (reference src/Solv.c:93-212,
- * 302-445).  GMRES and MINRES run in FP64 only (SFR_PREC_MIXED returns
- * SFR_NOT_SUPPORTED).
+ * PBICGSTAB, PBICGSTABL2, GMRES and MINRES branches (reference
+ * src/Solv.c:93-445).  BiCGSTAB(2), GMRES and MINRES run in FP64 only
+ * (SFR_PREC_MIXED returns SFR_NOT_SUPPORTED).
  * sfr_solve_device: b and x already on the device (d_b, d_x: neq doubles).
  * precision SFR_PREC_MIXED runs FP32 inner solves under an FP64 refinement
  * loop whose residual, update and stopping test are FP64 and use the same
@@ -266,10 +269,10 @@ int sfr_set_option(sfr_context *ctx, const char *name, double value);
  * with the reference's opaque Memoria/Interface/BufferOmp pointers passed
  * through untouched.  Non-convergence and bad ids end the process with the
  * reference's messages and exit codes (src/Cg.c:195-200,
- * src/Bicgstab.c:225-231, src/Iters.c:770-774, src/Solv.c:496; MINRES
- * only warns, src/Iters.c:540-544); with fLog one line in the reference's
- * format goes to fSolvLog (src/Cg.c:201-208, src/Bicgstab.c:235-242,
- * src/Iters.c:545-552, 778-786). */
+ * src/Bicgstab.c:225-231, 1362-1366, src/Iters.c:770-774, src/Solv.c:496;
+ * MINRES only warns, src/Iters.c:540-544); with fLog one line in the
+ * reference's format goes to fSolvLog (src/Cg.c:201-208,
+ * src/Bicgstab.c:235-242, 1368-1376, src/Iters.c:545-552, 778-786). */
 void sfr_solverC(void *m, int nEq, int nEqNov, int nAd, int nAdR,
                  int *ia, int *ja, double *al, double *ad, double *au,
                  double *b, double *x, void *iNeq, void *bOmp,

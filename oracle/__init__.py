@@ -1,7 +1,7 @@
 """ORACLE -- test infrastructure only.
 
 CPU restatement of the reference's linear-solve path (sfr_oracle.c: pcg,
-pbicgstab, gmres, pminres, the serial matvecs, dot, preCondDiag) and of its index builders
+pbicgstab, pbicgstabl2, gmres, pminres, the serial matvecs, dot, preCondDiag) and of its index builders
 (patterns.py: convGraph, GENRCM, numEqV1/V2, csrIa/csrJa, ellPackJa, the csr/
 ellPack value scatter), plus, where the reference tree exists, a harness
 around the reference's own compiled C (_ref/libsfr_ref.so, built by
@@ -47,7 +47,7 @@ def orc():
         if not os.path.exists(ORACLE_LIB):
             raise ImportError(f"{ORACLE_LIB} missing: run `make -C oracle`")
         lib = C.CDLL(ORACLE_LIB)
-        for name in ("orc_pcg", "orc_pbicgstab", "orc_minres"):
+        for name in ("orc_pcg", "orc_pbicgstab", "orc_minres", "orc_pbicgstabl2"):
             f = getattr(lib, name)
             f.restype = C.c_int
             f.argtypes = [C.c_int, C.c_int, i32p, i32p, f64p, f64p, f64p, f64p,
@@ -117,6 +117,12 @@ def pcg(system, b, tol, maxit, x=None, newX=True):
 
 def pbicgstab(system, b, tol, maxit, x=None, newX=True):
     return _krylov("orc_pbicgstab", system, b, tol, maxit, x, newX)
+
+
+def pbicgstabl2(system, b, tol, maxit, x=None, newX=True):
+    """Reference-order serial BiCGSTAB(2) (oracle); stats["norm_r_m"] is
+    the recurrence residual norm at the end."""
+    return _krylov("orc_pbicgstabl2", system, b, tol, maxit, x, newX)
 
 
 def minres(system, b, tol, maxit, x=None, newX=True):

@@ -169,6 +169,9 @@ int ref_solve(int neq, int nad, int *ia, int *ja, double *al, double *ad,
   if (solver == PCG) {
     if (sscanf(buf, "PCG: tol %lf iteration %d xKx %lf norma(x*x) %lf time %lf",
                &t, &it, &xkx, &nx, &tm_) != 5) return -3;
+  } else if (solver == PBICGSTABL2) {
+    if (sscanf(buf, "PBICGSTAB(2): tol %lf iteration %d xKx %lf norma(x*x) %lf time %lf",
+               &t, &it, &xkx, &nx, &tm_) != 5) return -3;
   } else if (solver == MINRES) {
     if (sscanf(buf, "PMINRES: tol %lf iteration %d xKx %lf norma(x*x) %lf time %lf",
                &t, &it, &xkx, &nx, &tm_) != 5) return -3;

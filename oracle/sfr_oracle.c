@@ -20,6 +20,7 @@
  *   preCondDiag      reference src/PreCond.c:18-26
  *   pcg              reference src/Cg.c:47-209, driven as src/Solv.c:93-145
  *   pbicgstab        reference src/Bicgstab.c:50-245, as src/Solv.c:150-212
+ *   pbicgstabl2      reference src/Bicgstab.c:1172-1378, as src/Solv.c:217-295
  *   gmres            reference src/Iters.c:587-790, as src/Solv.c:302-372
  *   pminres          reference src/Iters.c:309-555, as src/Solv.c:380-445
  *   symOrtho         reference src/Iters.c:1171-1208
@@ -413,6 +414,114 @@ int orc_gmres(int n, int nad, const int *ia, const int *ja, const double *al,
   }
   if (cycles_out) *cycles_out = (l == ncycles) ? l : l + 1;
   free(m); free(g); free(h); free(e); free(c); free(s); free(y);
+  return 0;
+}
+
+/* BiCGSTAB(2) with right Jacobi preconditioning, reference
+ * src/Bicgstab.c:1172-1378 (pbicgstabl2), driven as src/Solv.c:217-295:
+ * conv = tol ||b||; per iteration an even and an odd BiCG step (two SpMVs
+ * each) and the GCR(2) minimisation over (s, t); the stop test after the
+ * double step, on the recurrence residual.  norm_r_m holds that recurrence
+ * ||r|| at the end (the true ||b - Ax|| is norm_r), so the two can be
+ * compared.  status 1 is the reference's MAXIT exit. */
+int orc_pbicgstabl2(int n, int nad, const int *ia, const int *ja,
+                    const double *al, const double *ad, const double *b,
+                    double *x, double tol, unsigned maxit, int newX,
+                    int storage, int unsym, orc_stats *st)
+{
+  (void)nad;
+  matvec_fn matvec = pick(storage, unsym);
+  if (!matvec || n <= 0) return -1;
+  const size_t N = (size_t)n;
+  double *buf = calloc(N * 12, sizeof(double));
+  if (!buf) return -2;
+  double *m = buf, *t = buf + N, *v = buf + 2 * N, *r = buf + 3 * N;
+  double *u = buf + 4 * N, *r0 = buf + 5 * N, *w = buf + 6 * N;
+  double *s = buf + 7 * N, *p = buf + 8 * N, *h = buf + 9 * N, *z = buf + 10 * N;
+  for (int i = 0; i < n; i++) m[i] = 1.0 / ad[i];
+  const double t0 = now();
+  if (newX) for (int i = 0; i < n; i++) x[i] = 0.0;
+  double d = orc_dot(b, b, n);
+  const double norm_b = sqrt(d);
+  const double conv = tol * norm_b;
+  matvec(n, ia, ja, al, ad, x, t);
+  for (int i = 0; i < n; i++) {
+    r0[i] = b[i] - t[i];
+    r[i] = r0[i];
+    u[i] = 0.0;
+  }
+  double rr0 = 1.0, alpha = 0.0, omega2 = 1.0, omega1, rr1, beta, gamma;
+  unsigned j;
+  for (j = 0; j < maxit; j++) {
+    rr0 = -omega2 * rr0;
+    /* even BiCG step */
+    rr1 = orc_dot(r, r0, n);
+    beta = alpha * rr1 / rr0;
+    rr0 = rr1;
+    for (int i = 0; i < n; i++) {
+      u[i] = r[i] - beta * u[i];
+      z[i] = u[i] * m[i];
+    }
+    matvec(n, ia, ja, al, ad, z, v);
+    gamma = orc_dot(v, r0, n);
+    alpha = rr0 / gamma;
+    for (int i = 0; i < n; i++) {
+      r[i] -= alpha * v[i];
+      p[i] = r[i] * m[i];
+    }
+    matvec(n, ia, ja, al, ad, p, s);
+    for (int i = 0; i < n; i++) x[i] += alpha * z[i];
+    /* odd BiCG step */
+    rr1 = orc_dot(s, r0, n);
+    beta = alpha * rr1 / rr0;
+    rr0 = rr1;
+    for (int i = 0; i < n; i++) {
+      v[i] = s[i] - beta * v[i];
+      p[i] = v[i] * m[i];
+    }
+    matvec(n, ia, ja, al, ad, p, w);
+    gamma = orc_dot(w, r0, n);
+    alpha = rr0 / gamma;
+    for (int i = 0; i < n; i++) {
+      u[i] = r[i] - beta * u[i];
+      r[i] -= alpha * v[i];
+      s[i] -= alpha * w[i];
+      p[i] = s[i] * m[i];
+      z[i] = u[i] * m[i];
+      h[i] = r[i] * m[i];
+    }
+    matvec(n, ia, ja, al, ad, p, t);
+    /* GCR(2) part */
+    omega1 = orc_dot(r, s, n);
+    const double mi = orc_dot(s, s, n);
+    const double ni = orc_dot(s, t, n);
+    double tau = orc_dot(t, t, n);
+    omega2 = orc_dot(r, t, n);
+    tau = tau - ni * ni / mi;
+    omega2 = (omega2 - ni * omega1 / mi) / tau;
+    omega1 = (omega1 - ni * omega2) / mi;
+    for (int i = 0; i < n; i++) {
+      x[i] += omega1 * h[i] + omega2 * p[i] + alpha * z[i];
+      r[i] -= (omega1 * s[i] + omega2 * t[i]);
+    }
+    d = orc_dot(r, r, n);
+    if (sqrt(d) < conv) break;
+    for (int i = 0; i < n; i++) u[i] -= (omega1 * v[i] + omega2 * w[i]);
+  }
+  const double rec = sqrt(fabs(d));
+  matvec(n, ia, ja, al, ad, x, z);
+  const double xKx = orc_dot(x, z, n);
+  const double norm = sqrt(orc_dot(x, x, n));
+  for (int i = 0; i < n; i++) r[i] = b[i] - z[i];
+  const double nr = sqrt(orc_dot(r, r, n));
+  const double t1 = now();
+  if (st) {
+    st->status = j == maxit ? 1 : 0;
+    st->iters = j == maxit ? maxit : j + 1;
+    st->xKx = xKx; st->norm_x = norm; st->norm_r = nr; st->norm_r_m = rec;
+    st->norm_b = norm_b; st->conv = conv; st->time_s = t1 - t0;
+  }
+  free(buf);
   return 0;
 }
 

@@ -26,7 +26,7 @@ i16p = np.ctypeslib.ndpointer(dtype=np.int16, flags="C_CONTIGUOUS")
 f64p = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
 
 # ---- ids (include/Define.h:392-397, 407-412 of the reference) --------------
-PCG, PBICGSTAB, GMRES, MINRES = 1, 2, 4, 5
+PCG, PBICGSTAB, PBICGSTABL2, GMRES, MINRES = 1, 2, 3, 4, 5
 CSR, CSRD, CSRC, ELLPACK, CSRDCOO, CSRCCOO = 1, 2, 3, 4, 5, 6
 OK, MAXIT, BREAKDOWN, BAD_ARG, HIP_ERROR, NO_DEVICE, NOT_SUPPORTED = range(7)
 PREC_FP64, PREC_MIXED = 0, 1

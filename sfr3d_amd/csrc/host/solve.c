@@ -30,7 +30,15 @@ int sfr_workspace(sfr_context *ctx, int64_t n);
 int sfr_matrix_prepare_mixed(sfr_context *ctx, sfr_matrix *A);
 int sfr_matrix_prepare_adjoint(sfr_context *ctx, sfr_matrix *A);
 
-enum { G_PCG = 1, G_BICG = 2, G_MX = 3, G_BB = 4, G_GM = 5, G_MR = 6 };
+enum { G_PCG = 1, G_BICG = 2, G_MX = 3, G_BB = 4, G_GM = 5, G_MR = 6, G_B2 = 7 };
+
+/* BiCGSTAB(2) iterations (two BiCG steps and a GCR(2) step, nine kernels)
+ * per captured chunk */
+static int b2_chunk(const sfr_context *ctx)
+{
+  const int c = ctx->chunk / 2;
+  return c < 1 ? 1 : c;
+}
 
 /* MINRES ring buffers in the workspace (w64[8], w64[9] stay free for the
  * host-pointer path's b and x) */
@@ -62,7 +70,8 @@ static int get_graph(sfr_context *ctx, int kind, const void *k0,
    * where it started */
   const int chunk = kind == G_MX ? (ctx->chunk + 1) & ~1
                   : kind == G_GM ? gm_cycles_per_chunk(ctx)
-                  : kind == G_MR ? mr_chunk(ctx) : ctx->chunk;
+                  : kind == G_MR ? mr_chunk(ctx)
+                  : kind == G_B2 ? b2_chunk(ctx) : ctx->chunk;
   for (int i = 0; i < ctx->ngraph; i++) {
     sfr_graph *e = &ctx->graphs[i];
     if (e->kind == kind && e->chunk == chunk && e->key[0] == k0 &&
@@ -101,6 +110,8 @@ static int get_graph(sfr_context *ctx, int kind, const void *k0,
       lrc = sfr_k_gm_cycle(s, v, g, (const double *)k2, (double *)k1, basis, ld, wr,
                            basis + (SFR_GM_K + 2) * ld, ctx->d_sc, ctx->part);
     }
+    else if (kind == G_B2)
+      lrc = sfr_k_b2_iter(s, v, g, (double *)k1, w, ctx->d_sc, ctx->part);
     else if (kind == G_MR) {
       MR_RINGS(ctx);
       lrc = sfr_k_mr_iter(s, v, g, (double *)k1, V_, W_, Z_, it % 6, ctx->d_sc, ctx->part);
@@ -282,6 +293,24 @@ static int solve_gmres64(sfr_context *ctx, sfr_matrix *A, double tol,
   return rc;
 }
 
+/* BiCGSTAB(2), reference src/Bicgstab.c:1172-1378 as src/Solv.c:217-295
+ * runs it; iters = iterations of two BiCG steps each (the reference's j+1). */
+static int solve_bicgl2_64(sfr_context *ctx, sfr_matrix *A, double tol,
+                           unsigned maxit, const double *d_b, double *d_x,
+                           int newX, sfr_stats *st)
+{
+  CHECK(sfr_workspace(ctx, A->n));
+  sfr_mat_view v = sfr_view(A);
+  sfr_geom g = {A->rpt, 0};
+  hipGraphExec_t ex;
+  CHECK(get_graph(ctx, G_B2, A, d_x, NULL, &v, g, &ex));
+  TRY(hipEventRecord(ctx->ev_start, ctx->stream));
+  TRY(sfr_k_b2_setup(ctx->stream, &v, g, d_b, d_x, ctx->w64, newX, tol, maxit,
+                     ctx->d_sc, ctx->part));
+  CHECK(run_loop(ctx, ex, 0, maxit / (unsigned)b2_chunk(ctx) + 2, 1));
+  return finish(ctx, &v, g, d_b, d_x, st);
+}
+
 /* MINRES, reference src/Iters.c:309-555 as src/Solv.c:380-445 runs it.
  * iters = iterations performed (the reference's j+1 on convergence). */
 static int solve_minres64(sfr_context *ctx, sfr_matrix *A, double tol,
@@ -459,6 +488,11 @@ int sfr_solve_device(sfr_context *ctx, sfr_matrix *A, int solver,
       return sfr_set_error(ctx, SFR_NOT_SUPPORTED, "mixed BiCGSTAB");
     return solve_bicg64(ctx, A, tol, maxit, d_b, d_x, newX, st);
   }
+  if (solver == SFR_PBICGSTABL2) {
+    if (precision == SFR_PREC_MIXED)
+      return sfr_set_error(ctx, SFR_NOT_SUPPORTED, "mixed BiCGSTAB(2)");
+    return solve_bicgl2_64(ctx, A, tol, maxit, d_b, d_x, newX, st);
+  }
   if (solver == SFR_GMRES) {
     if (precision == SFR_PREC_MIXED)
       return sfr_set_error(ctx, SFR_NOT_SUPPORTED, "mixed GMRES");
@@ -592,8 +626,8 @@ void sfr_solverC(void *m, int nEq, int nEqNov, int nAd, int nAdR, int *ia,
                  const char *name)
 {
   (void)m; (void)iNeq; (void)bOmp;
-  if (solver != SFR_PCG && solver != SFR_PBICGSTAB && solver != SFR_GMRES &&
-      solver != SFR_MINRES) {
+  if (solver != SFR_PCG && solver != SFR_PBICGSTAB && solver != SFR_PBICGSTABL2 &&
+      solver != SFR_GMRES && solver != SFR_MINRES) {
     fprintf(stderr, "Opcao %d e invalida!!\n", solver);
     fprintf(stderr, "Arquivo:%s\nFonte:  %s\n", __FILE__, __func__);
     exit(EXIT_FAILURE);
@@ -628,6 +662,11 @@ void sfr_solverC(void *m, int nEq, int nEqNov, int nAd, int nAdR, int *ia,
     printf("PMINRES: *** WARNING: no convergende reached !\n");
     printf("MAXIT = %d \n", maxIt);
   }
+  if (rc != SFR_OK && solver == SFR_PBICGSTABL2) {  /* src/Bicgstab.c:1362-1366 */
+    printf(" PBICGSTAB(2) *** WARNING: no convergende reached !\n");
+    printf("MAXIT = %d \n", maxIt);
+    exit(EXIT_FAILURE);
+  }
   if (rc != SFR_OK && solver == SFR_GMRES) {  /* src/Iters.c:770-774 */
     printf(" GMRES *** WARNING: no convergende reached !\n");
     printf("MAXIT = %d \n", maxIt);
@@ -648,6 +687,10 @@ void sfr_solverC(void *m, int nEq, int nEqNov, int nAd, int nAdR, int *ia,
     if (solver == SFR_PCG)
       fprintf(fSolvLog, "PCG: tol %20.2e iteration %d xKx %20.12e norma(x*x) %20.12e "
               "time %20.5lf\n", tol, (int)st.iters, st.xKx, st.norm_x, st.time_s);
+    else if (solver == SFR_PBICGSTABL2)    /* src/Bicgstab.c:1368-1376 */
+      fprintf(fSolvLog, "PBICGSTAB(2): tol %20.2e iteration %d xKx %20.12e "
+              "norma(x*x) %20.12e time %20.5lf\n", tol, (int)st.iters, st.xKx,
+              st.norm_x, st.time_s);
     else if (solver == SFR_MINRES)         /* src/Iters.c:545-552; j+1 at maxIt */
       fprintf(fSolvLog, "PMINRES: tol %20.2e iteration %d xKx %20.12e norma(x*x) %20.12e "
               "time %20.5lf\n", tol, (int)st.iters + (rc != SFR_OK ? 1 : 0), st.xKx,

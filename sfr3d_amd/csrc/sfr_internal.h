@@ -184,6 +184,14 @@ int sfr_k_gm_cycle(hipStream_t s, const sfr_mat_view *A, sfr_geom g,
                    double *const w[2], double *st, sfr_scalars *sc,
                    double *part);
 
+/* BiCGSTAB(2), FP64 (reference src/Bicgstab.c:1172-1378); W: r0, r, u, v,
+ * w, s, p, z/t */
+int sfr_k_b2_setup(hipStream_t s, const sfr_mat_view *A, sfr_geom g,
+                   const double *b, double *x, double *const W[8], int newX,
+                   double tol, unsigned maxit, sfr_scalars *sc, double *part);
+int sfr_k_b2_iter(hipStream_t s, const sfr_mat_view *A, sfr_geom g, double *x,
+                  double *const W[8], sfr_scalars *sc, double *part);
+
 /* MINRES, FP64 (reference src/Iters.c:309-555); V, W: rings of 3 vectors,
  * Z: ring of 2; one iteration per call at ring phase q = j % 6 */
 int sfr_k_mr_setup(hipStream_t s, const sfr_mat_view *A, sfr_geom g,

@@ -2,8 +2,8 @@
 
 Mirrors the reference's solver boundary, solverC (reference
 include/Solv.h:59-70): a system in one of SFR3D's storages, a solver id
-(PCG = 1, PBICGSTAB = 2, GMRES = 4, MINRES = 5), tol, maxit, b and
-x.  Everything here calls the
+(PCG = 1, PBICGSTAB = 2, PBICGSTABL2 = 3, GMRES = 4, MINRES = 5), tol,
+maxit, b and x.  Everything here calls the
 HIP library; there is no CPU path.
 """
 import ctypes as C

@@ -1,7 +1,7 @@
 """TEST INFRASTRUCTURE -- runs the emulated library (argv[1]) on the small
 cases of argv[2] (written by tests/test_emulated_kernels.py with the
 oracle's solutions) in a process of its own: FP64 and mixed PCG, BiCGSTAB,
-GMRES, MINRES,
+BiCGSTAB(2), GMRES, MINRES,
 the shared-pattern batch, both adjoint paths, 16-bit column offsets with
 escaped far columns, and a tuned uneven batch.  Prints PASS/FAIL lines and
 exits non-zero on any failure."""
@@ -57,6 +57,10 @@ bb = np.ascontiguousarray(z["mom_b"]); xo = z["mom_xo"]
 x = np.zeros(s.neq); st = St()
 rc = lib.sfr_solve_device(ctx, A, 2, 0, 1e-8, maxit, bb[0].ctypes.data, x.ctypes.data, 1, C.byref(st))
 check(f"bicg rc {rc} it {st.iters} rel {rel(x, xo):.1e}", rc == 0 and rel(x, xo) < 1e-6)
+xb = np.zeros(s.neq); st = St(); itb = int(z["mom_b2_it"][0])
+rc = lib.sfr_solve_device(ctx, A, 3, 0, 1e-10, maxit, bb[1].ctypes.data, xb.ctypes.data, 1, C.byref(st))
+check(f"bicgl2 mom rc {rc} it {st.iters}/{itb} rel {rel(xb, z['mom_b2_xo']):.1e}",
+      rc == 0 and abs(int(st.iters) - itb) <= 1 and rel(xb, z["mom_b2_xo"]) < 1e-8)
 xg = np.zeros(s.neq); st = St(); cyc = z["mom_gm_cyc"]
 rc = lib.sfr_solve_device(ctx, A, 4, 0, 1e-10, maxit, bb[0].ctypes.data, xg.ctypes.data, 1, C.byref(st))
 check(f"gmres mom rc {rc} cycles {st.outer}/{cyc[0]} it {st.iters}/{cyc[1]} rel {rel(xg, z['mom_gm_xo']):.1e}",

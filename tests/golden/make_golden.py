@@ -14,6 +14,8 @@ tree and oracle/_ref exist: `python tests/golden/make_golden.py`).
   ref_gmres_solutions.npz    ||x||, digest of x) and the solution vectors
   ref_minres.json,           solverC with MINRES (logged iterations, xKx,
   ref_minres_solutions.npz   ||x||, digest of x) and the solution vectors
+  ref_bicgl2.json,           solverC with BiCGSTAB(2) (iterations, xKx, ||x||,
+  ref_bicgl2_solutions.npz   digest of x) and the solution vectors
   cavity_ref.json            solverC (PCG, tol 1e-8) on pressure-correction
                              systems of the 128x128 lid-driven cavity (SIMPLE
                              outer iterations 0, 5, 11; CSRD and ELLPACK):
@@ -154,6 +156,42 @@ def gmres_vectors():
     np.savez_compressed(os.path.join(HERE, "ref_gmres_solutions.npz"), **sols)
 
 
+BICGL2_CASES = [
+    ("bicgl2_momentum_10_csrd", "momentum3d", dict(n=10), 0, 1e-8),
+    ("bicgl2_momentum_10_csrc", "momentum3d", dict(n=10, storage=3), 1, 1e-11),
+    ("bicgl2_momentum_10_ell", "momentum3d", dict(n=10, storage=4), 2, 1e-8),
+    ("bicgl2_poisson3d_12_csrd", "poisson3d", dict(n=12, seed=3, storage=2), None, 1e-8),
+    ("bicgl2_poisson2d_20x16_pinned", "poisson2d", dict(nx=20, ny=16, seed=5, storage=2), None, 1e-8),
+]
+
+
+def bicgl2_vectors():
+    """solverC with solver PBICGSTABL2 (3), maxIt 5000."""
+    import oracle
+    from sfr3d_amd import systems as Y
+    R = oracle.ref()
+    R.ref_init(1 << 30)
+    out, sols = {"solves": []}, {}
+    for name, fn, kw, comp, tol in BICGL2_CASES:
+        B = getattr(Y, fn)(**kw)
+        s = B.system
+        b = B.b if comp is None else B.extra["b_all"][comp]
+        x = np.zeros(s.neq)
+        res = np.zeros(5)
+        ad = s.ad if s.ad is not None else s.al
+        rc = R.ref_solve(s.neq, s.nad, s.ia, s.ja, oracle._al_contig(s),
+                         np.ascontiguousarray(ad), None, np.ascontiguousarray(b),
+                         x, tol, 5000, s.storage, 3, 1, int(s.unsym), res, None, 0)
+        assert rc == 0
+        out["solves"].append({"name": name, "tol": tol, "iters": int(res[0]),
+                              "xKx": res[1], "norm_x": res[2], "x_sha256": sha(x)})
+        sols[name] = x
+        print("bicgl2", name, int(res[0]), flush=True)
+    with open(os.path.join(HERE, "ref_bicgl2.json"), "w") as f:
+        json.dump(out, f, indent=1)
+    np.savez_compressed(os.path.join(HERE, "ref_bicgl2_solutions.npz"), **sols)
+
+
 MINRES_CASES = [
     ("minres_poisson3d_12_csrd", "poisson3d", dict(n=12, seed=3, storage=2), 1e-8, 5000),
     ("minres_poisson3d_12_csrc", "poisson3d", dict(n=12, seed=3, storage=3), 1e-11, 5000),
@@ -229,7 +267,7 @@ def cavity_vectors():
 
 
 if __name__ == "__main__":
-    what = sys.argv[1:] or ["mesh", "ref", "gmres", "minres", "cavity"]
+    what = sys.argv[1:] or ["mesh", "ref", "gmres", "minres", "bicgl2", "cavity"]
     if "mesh" in what:
         mesh_fixture()
     if "ref" in what:
@@ -238,5 +276,7 @@ if __name__ == "__main__":
         gmres_vectors()
     if "minres" in what:
         minres_vectors()
+    if "bicgl2" in what:
+        bicgl2_vectors()
     if "cavity" in what:
         cavity_vectors()

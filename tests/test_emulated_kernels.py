@@ -3,7 +3,8 @@ with g++ against a CPU emulation of the HIP kernel model (tests/emu/), and
 the resulting library is driven through the same C ABI as the GPU build.
 This checks kernel logic -- indexing, the scalar protocol between kernels,
 graph capture and replay, the fused mixed-precision iteration, GMRES's
-restart cycle, MINRES's ring buffers and parity-alternated scalars, escaped 16-bit columns, the uneven batch, device assembly --
+restart cycle, MINRES's ring buffers and parity-alternated scalars,
+BiCGSTAB(2)'s nine-kernel iteration, escaped 16-bit columns, the uneven batch, device assembly --
 against the oracle and
 the host assembler without a GPU.
 Set SFR_EMU_ASAN=1 to build it under AddressSanitizer (slower), which
@@ -42,6 +43,8 @@ def write_cases(path):
     put("pois", B.system, B.b, 1, 1000)
     M = systems.momentum3d(5)
     put("mom", M.system, M.extra["b_all"], 2, 1000)
+    xb, sb = oracle.pbicgstabl2(M.system, M.extra["b_all"][1], 1e-10, 1000)
+    d["mom_b2_xo"], d["mom_b2_it"] = xb, np.array([sb["iters"]])
     xg, sg = oracle.gmres(M.system, M.extra["b_all"][0], 1e-10, 1000)
     d["mom_gm_xo"], d["mom_gm_cyc"] = xg, np.array([sg["cycles"], sg["iters"]])
     B = systems.poisson3d(6, seed=11)
@@ -95,4 +98,4 @@ def test_kernels_under_cpu_emulation(tmp_path):
     print(res.stdout)
     assert res.returncode == 0, res.stdout + res.stderr[-3000:]
     assert "SMOKE OK" in res.stdout
-    assert res.stdout.count("PASS") == 12
+    assert res.stdout.count("PASS") == 13
