@@ -152,7 +152,7 @@ def main():
         extra["momentum_uvw"] = bench_uvw(ctx, S, systems, lib)
         extra["uneven_batch"] = bench_uneven(ctx, S, systems)
         extra["adjoint"] = bench_adjoint(ctx, S, systems)
-        extra["gmres"] = bench_gmres(ctx, S, systems, cpu=not args.no_cpu)
+        extra["solver_ids"] = bench_solvers(ctx, S, systems, cpu=not args.no_cpu)
 
     out = {
         "metric": "Krylov solves/s + HBM GB/s, 128³ Poisson, mixed vs FP64 @ relres 1e-8",
@@ -441,67 +441,73 @@ def bench_adjoint(ctx, S, systems, reps=5):
     return out
 
 
-def bench_gmres(ctx, S, systems, cpu=True, reps=3):
-    """GMRES(15) FP64 (solver id 4) on the 96^3 upwind momentum system
-    (u component), device times; BiCGSTAB on the same system beside it; the
-    oracle's serial gmres timed over one restart cycle as the CPU leg."""
+def bench_solvers(ctx, S, systems, cpu=True, reps=3):
+    """Every solver id on the system it is meant for, device times:
+    BiCGSTAB, BiCGSTAB(2) and GMRES(15) on the 96^3 upwind momentum system
+    (u component), PCG and MINRES on the 96^3 variable-coefficient Poisson
+    system, FP64, tol 1e-8.  The CPU leg times the oracle's serial solvers
+    on one bounded sample each (one GMRES restart cycle, 20 iterations of
+    the others)."""
+    out = {}
     M = systems.momentum3d(96)
-    n = M.system.neq
-    b = M.extra["b_all"][0]
-    A = ctx.matrix(M.system)
-    info = A.info()
-    db, dx = ctx.array(b), ctx.empty(n)
-    st = ctx.solve_device(A, S.GMRES, TOL, 5000, db, dx)
-    tg = []
-    for _ in range(reps):
-        t = time.perf_counter()
-        st = ctx.solve_device(A, S.GMRES, TOL, 5000, db, dx)
-        tg.append(time.perf_counter() - t)
-    tgm = float(np.median(tg))
-    sb = ctx.solve_device(A, S.PBICGSTAB, TOL, 5000, db, dx)
-    tb = []
-    for _ in range(reps):
-        t = time.perf_counter()
-        sb = ctx.solve_device(A, S.PBICGSTAB, TOL, 5000, db, dx)
-        tb.append(time.perf_counter() - t)
-    cycles = st["outer"]
-    # algorithmic bytes per row of one restart cycle: 16 SpMV-type kernels
-    # (m = s*(8 + ib) + ad + gathered x) with their vector streams, the 120
-    # fused Gram-Schmidt kernels (w read+written, g_j, g_j+1) and the update
-    # (14 basis vectors, x read+written)
-    m = info["slots"] * (8 + info["index_bytes"]) + 16
-    per_row = 16 * m + 24 + 14 * 32 + 24 + 3720 + 128
-    cyc_s = tgm / max(cycles, 1)
-    out = {"workload": "96^3 Taylor-Green upwind momentum (u), CSRD storage, "
-                       "Jacobi-GMRES(15) FP64 to tol 1e-8",
-           "neq": n, "solve_ms": round(tgm * 1e3, 3), "cycles": cycles,
-           "iters": st["iters"], "relres_true": st["relres"],
-           "ref_criterion": [st["norm_r_m"], st["conv"]],
-           "ms_per_cycle": round(cyc_s * 1e3, 3),
-           "cycle_GBs_algorithmic": round(n * per_row / cyc_s / 1e9, 1),
-           "bicgstab_ms": round(float(np.median(tb)) * 1e3, 3),
-           "bicgstab_iters": sb["iters"]}
-    if cpu:
-        out["cpu_baseline"] = cpu_baseline_gmres(M.system, b)
-        out["cpu_baseline"]["gpu_speedup_per_cycle"] = round(
-            out["cpu_baseline"]["ms_per_cycle"] / (cyc_s * 1e3), 1)
-    log(f"[bench] GMRES 96^3 momentum: {out['solve_ms']} ms, {cycles} cycles "
-        f"({out['ms_per_cycle']} ms/cycle, {out['cycle_GBs_algorithmic']} GB/s); "
-        f"BiCGSTAB {out['bicgstab_ms']} ms")
-    A.destroy(); db.free(); dx.free()
+    P = systems.poisson3d(96, seed=2)
+    legs = [("PBICGSTAB", S.PBICGSTAB, M, M.extra["b_all"][0]),
+            ("PBICGSTABL2", S.PBICGSTABL2, M, M.extra["b_all"][0]),
+            ("GMRES", S.GMRES, M, M.extra["b_all"][0]),
+            ("PCG", S.PCG, P, P.b),
+            ("MINRES", S.MINRES, P, P.b)]
+    mats = {}
+    for name, sid, B, b in legs:
+        key = id(B)
+        if key not in mats:
+            mats[key] = ctx.matrix(B.system)
+        A = mats[key]
+        n = B.system.neq
+        db, dx = ctx.array(b), ctx.empty(n)
+        st = ctx.solve_device(A, sid, TOL, 20000, db, dx)
+        ts = []
+        for _ in range(reps):
+            t = time.perf_counter()
+            st = ctx.solve_device(A, sid, TOL, 20000, db, dx)
+            ts.append(time.perf_counter() - t)
+        tm = float(np.median(ts))
+        leg = {"neq": n, "solve_ms": round(tm * 1e3, 3), "iters": st["iters"],
+               "status": st["status"], "relres_true": st["relres"]}
+        if sid == S.GMRES:
+            leg["cycles"] = st["outer"]
+            leg["ms_per_cycle"] = round(tm * 1e3 / max(st["outer"], 1), 3)
+        else:
+            leg["us_per_iter"] = round(tm * 1e6 / max(st["iters"], 1), 2)
+        if cpu:
+            leg["cpu_baseline"] = cpu_baseline_solver(name, B.system, b)
+        out[name] = leg
+        log(f"[bench] {name:12s} {n} rows: {leg['solve_ms']} ms, iters {st['iters']}"
+            + (f", cycles {st['outer']}" if sid == S.GMRES else ""))
+        db.free(); dx.free()
+    for A in mats.values():
+        A.destroy()
+    out["workload"] = ("96^3 Taylor-Green upwind momentum (u; BiCGSTAB, "
+                       "BiCGSTAB(2), GMRES(15)) and 96^3 variable-coefficient "
+                       "Poisson (PCG, MINRES), FP64 to tol 1e-8")
     return out
 
 
-def cpu_baseline_gmres(sysm, b):
-    """The oracle's serial gmres (the reference's src/Iters.c:587-790
-    restated), one restart cycle of the same system, on one host core."""
+def cpu_baseline_solver(name, sysm, b):
+    """The oracle's serial restatement of the reference's solver, one host
+    core, on a bounded sample of the same solve: one GMRES(15) restart
+    cycle, or 20 iterations of the others."""
     import oracle
+    f = {"PBICGSTAB": oracle.pbicgstab, "PBICGSTABL2": oracle.pbicgstabl2,
+         "GMRES": oracle.gmres, "PCG": oracle.pcg, "MINRES": oracle.minres}[name]
+    k = 1 if name == "GMRES" else 20
     t = time.perf_counter()
-    _, so = oracle.gmres(sysm, b, TOL, 1)
+    f(sysm, b, 1e-30, k)
     dt = time.perf_counter() - t
-    return {"ms_per_cycle": round(dt * 1e3, 2), "cores": 1, "kind": "port",
-            "sample": "one GMRES(15) restart cycle of the 96^3 momentum system "
-                      "(oracle, serial C)"}
+    if name == "GMRES":
+        return {"ms_per_cycle": round(dt * 1e3, 2), "cores": 1, "kind": "port",
+                "sample": "one restart cycle"}
+    return {"us_per_iter": round(dt * 1e6 / k, 1), "cores": 1, "kind": "port",
+            "sample": f"{k} iterations"}
 
 
 def cpu_baseline(sysm, b, threads, x_gpu64):
