@@ -169,6 +169,26 @@ int sfr_solve_shared_device(sfr_context *ctx, sfr_matrix *A, int nsys,
                             double *d_x, int solver, double tol,
                             unsigned maxit, int newX, sfr_stats *stats);
 
+/* Species batch (SFR3D's transported species, reference
+ * src/Combustion.c:76-92, 222-240: one solverC call per species on the same
+ * ia/ja, with al + i*(2*nad+nadr), ad + i*neq, au, b and x of its own):
+ * nsys (1..SFR_BATCH_MAX) systems on sys's pattern, system c with al[c],
+ * ad[c], au[c] (au or au[c] may be NULL: CSRC unsym then reads al[c] + nad),
+ * b[c] and x[c].  sys supplies neq, nad, ia, ja, storage and unsym; its
+ * value pointers are ignored.  With solver PBICGSTAB the systems run in
+ * groups of 2-4 through one set of batched launches per group, every SpMV
+ * reading the pattern once for the group and each system's own values;
+ * other solver ids and storage CSR solve the systems one after another.
+ * Each system keeps its own stop test; stats[c] holds system c's results.
+ * (The reference skips a species whose ||b|| < SZERO before calling the
+ * solver; pass only the systems to solve, a zero b converges at once.) */
+#define SFR_BATCH_MAX 10       /* MAXSPECIES, include/Define.h:92           */
+int sfr_solve_batched(sfr_context *ctx, const sfr_system *sys, int nsys,
+                      const double *const *al, const double *const *ad,
+                      const double *const *au, const double *const *b,
+                      double *const *x, int solver, double tol,
+                      unsigned maxit, int newX, sfr_stats *stats);
+
 /* Adjoint of a solve x = A^-1 b, for the gradient of a loss L(x): solves
  * A^T lambda = g with g = dL/dx (solver SFR_PCG when A is symmetric,
  * A^T = A; SFR_PBICGSTAB runs on the transposed operator, built on first

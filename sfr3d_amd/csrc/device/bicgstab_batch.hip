@@ -1,6 +1,9 @@
 // Shared-pattern batch of BiCGSTAB solves (SFR3D's u/v/w momentum systems,
 // reference src/Simple.c:2980-3036: three solverC calls on the same ia/ja/al
-// with their own ad, b and x).  NB systems run in one set of launches:
+// with their own ad, b and x; and its transported species,
+// src/Combustion.c:222-240, on the same ia/ja with their own al, ad, au, b
+// and x, when the view carries per-system values).  NB systems run in one
+// set of launches:
 //   - vectors are interleaved per row, x[i*NB + c], so the neighbour gathers
 //     of all NB systems hit the same cache lines;
 //   - every SpMV reads the shared columns and off-diagonal values once for
@@ -494,7 +497,7 @@ static void bb_setup(hipStream_t s, const sfr_mat_view *A, sfr_geom g0,
   const sfr_geom gm = {g0.rpt, grid_for(A->n, g0.rpt, op_lanes(A))};
   double *rc = REGION(part, 2), *rb = REGION(part, 1);
   BB_LAUNCH_V(k_bb_s1, NB, gv, s, A->n, b, x, newX, rc);
-  with_op64(A, [&](auto op) {
+  with_op64_b(A, [&](auto op) {
     using Op = decltype(op);
     BB_LAUNCH_M(k_bb_s2, Op, NB, gm, s, op, ad, b, dinv, x, r, r0, p, z, newX,
                 tol, maxit, (const double *)rc, gv.grid, rb, sc);
@@ -513,13 +516,13 @@ static void bb_iter(hipStream_t s, const sfr_mat_view *A, sfr_geom g0,
   const sfr_geom gm = {g0.rpt, grid_for(A->n, g0.rpt, op_lanes(A))};
   double *ra = REGION(part, 0), *rb = REGION(part, 1);
   double *rc = REGION(part, 2), *rd = REGION(part, 3);
-  with_op64(A, [&](auto op) {
+  with_op64_b(A, [&](auto op) {
     using Op = decltype(op);
     BB_LAUNCH_M(k_bb_v, Op, NB, gm, s, op, ad, (const double *)z, r0, v, ra, sc);
   });
   BB_LAUNCH_V(k_bb_s, NB, gv, s, A->n, (const double *)v, dinv, x, r, z,
               (const double *)ra, gm.grid, rb, sc);
-  with_op64(A, [&](auto op) {
+  with_op64_b(A, [&](auto op) {
     using Op = decltype(op);
     BB_LAUNCH_M(k_bb_t, Op, NB, gm, s, op, ad, (const double *)z,
                 (const double *)r, t, (const double *)rb, gv.grid, rc, sc);
@@ -537,7 +540,7 @@ static void bb_final(hipStream_t s, const sfr_mat_view *A, sfr_geom g0,
 {
   const sfr_geom gm = {g0.rpt, grid_for(A->n, g0.rpt, op_lanes(A))};
   double *rd = REGION(part, 3);
-  with_op64(A, [&](auto op) {
+  with_op64_b(A, [&](auto op) {
     using Op = decltype(op);
     BB_LAUNCH_M(k_bb_final, Op, NB, gm, s, op, ad, b, x, dinv, rd);
   });
@@ -586,6 +589,26 @@ extern "C" int sfr_k_bb_final(hipStream_t s, const sfr_mat_view *A,
 #define CALL(NB) bb_final<NB>(s, A, g, ad, dinv, b, x, sc, part)
   NB_DISPATCH(nb, CALL)
 #undef CALL
+  return hipGetLastError();
+}
+
+__global__ __launch_bounds__(SFR_BLOCK) void k_gather_nb(int64_t nnz,
+    const int32_t *__restrict__ vmap, const double *__restrict__ raw,
+    double *__restrict__ valnb, int nb, int c)
+{
+  const int64_t k = (int64_t)blockIdx.x * SFR_BLOCK + threadIdx.x;
+  if (k < nnz) {
+    const int32_t m = vmap[k];
+    valnb[k * nb + c] = m >= 0 ? raw[m] : 0.0;
+  }
+}
+
+extern "C" int sfr_k_gather_nb(hipStream_t s, int64_t nnz, const int32_t *vmap,
+                               const double *raw, double *valnb, int nb, int c)
+{
+  if (nnz <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_gather_nb, dim3((unsigned)((nnz + SFR_BLOCK - 1) / SFR_BLOCK)),
+                     dim3(SFR_BLOCK), 0, s, nnz, vmap, raw, valnb, nb, c);
   return hipGetLastError();
 }
 

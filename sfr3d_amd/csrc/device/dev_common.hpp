@@ -324,6 +324,75 @@ __device__ __forceinline__ void row_multi(const CsrOp<T, L, UNIT> &A, int64_t i,
   csr_row_multi<T, L, UNIT, NB>(A, i, valid, ad, x, acc);
 }
 
+// NB systems sharing the pattern, each with its own values: the operator's
+// val holds them interleaved per entry, val[k*NB + c] for device entry k
+// (SFR3D's transported species, one matrix per species on one pattern).
+// The wave's loads of slot f stay one contiguous segment (64*NB values).
+template <class Op>
+struct PerSys : Op {};
+
+template <typename T, int S, bool UNIT, bool D16, int NB>
+__device__ __forceinline__ void row_multi(const PerSys<EllOp<T, S, UNIT, D16>> &A,
+    int64_t i, bool valid, const T *__restrict__ ad, const T *__restrict__ x,
+    T (&acc)[NB])
+{
+#pragma unroll
+  for (int c = 0; c < NB; c++) acc[c] = T(0);
+  if (!valid) return;
+#pragma unroll
+  for (int c = 0; c < NB; c++) acc[c] = ad[i * NB + c] * x[i * NB + c];
+  bool esc = false;
+  auto slot = [&](int f) {
+    const int64_t k = ((int64_t)f * A.n + i) * NB;
+    const int64_t j = A.cg(f, i, esc);
+#pragma unroll
+    for (int c = 0; c < NB; c++) acc[c] += A.val[k + c] * x[j * NB + c];
+  };
+  if constexpr (S > 0) {
+#pragma unroll
+    for (int f = 0; f < S; f++) slot(f);
+  } else {
+    for (int f = 0; f < A.s; f++) slot(f);
+  }
+  if constexpr (D16)
+    if (esc) {
+      for (int64_t e = 0; e < A.nx; e++) {
+        const int64_t k = A.xk[e];
+        if (k % A.n != i) continue;
+        const int64_t j = A.xc[e];
+#pragma unroll
+        for (int c = 0; c < NB; c++)
+          acc[c] += A.val[k * NB + c] * (x[j * NB + c] - x[i * NB + c]);
+      }
+    }
+}
+
+template <typename T, int L, bool UNIT, int NB>
+__device__ __forceinline__ void row_multi(const PerSys<CsrOp<T, L, UNIT>> &A,
+    int64_t i, bool valid, const T *__restrict__ ad, const T *__restrict__ x,
+    T (&acc)[NB])
+{
+#pragma unroll
+  for (int c = 0; c < NB; c++) acc[c] = T(0);
+  if (valid) {
+    const int sub = threadIdx.x & (L - 1);
+    const int e = A.ptr[i + 1];
+    for (int k = A.ptr[i] + sub; k < e; k += L) {
+      const int64_t j = A.col[k];
+#pragma unroll
+      for (int c = 0; c < NB; c++) acc[c] += A.val[(int64_t)k * NB + c] * x[j * NB + c];
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < NB; c++)
+#pragma unroll
+    for (int off = L / 2; off > 0; off >>= 1) acc[c] += __shfl_xor(acc[c], off, L);
+  if (valid) {
+#pragma unroll
+    for (int c = 0; c < NB; c++) acc[c] += ad[i * NB + c] * x[i * NB + c];
+  }
+}
+
 // Call f(op) with the FP64 operator matching the view.
 template <typename F>
 __host__ inline void with_op64(const sfr_mat_view *A, F &&f)
@@ -350,6 +419,20 @@ __host__ inline void with_op64(const sfr_mat_view *A, F &&f)
   case 4: f(CsrOp<double, 4, false>{A->val, A->col, A->rowptr, A->ad, A->n, 0}); return;
   default: f(CsrOp<double, 8, false>{A->val, A->col, A->rowptr, A->ad, A->n, 0}); return;
   }
+}
+
+// The batched kernels' operator: the view's own values, or, when the view
+// carries per-system values (valnb), the same pattern over those.
+template <typename F>
+__host__ inline void with_op64_b(const sfr_mat_view *A, F &&f)
+{
+  if (!A->valnb) {
+    with_op64(A, f);
+    return;
+  }
+  sfr_mat_view v = *A;
+  v.val = A->valnb;
+  with_op64(&v, [&](auto op) { f(PerSys<decltype(op)>{op}); });
 }
 
 // Call f(op) with the FP32 Jacobi-scaled (unit diagonal) operator.

@@ -115,6 +115,8 @@ void sfr_finalize(sfr_context *ctx)
   for (int i = 0; i < 6; i++) hipFree(ctx->w32[i]);
   hipFree(ctx->stage);
   hipFree(ctx->gm_buf);
+  hipFree(ctx->nb_val);
+  hipFree(ctx->nb_raw);
   hipFree(ctx->d_sc);
   hipFree(ctx->part);
   if (ctx->h_sc) hipHostFree(ctx->h_sc);
@@ -200,6 +202,26 @@ int sfr_gm_workspace(sfr_context *ctx, int64_t n)
   return SFR_OK;
 }
 
+/* Species batch buffers: interleaved values of up to 4 systems (nnz each)
+ * and one system's raw values.  Captured graphs hold the old pointers, so
+ * they go when the buffers grow. */
+int sfr_nb_buffers(sfr_context *ctx, int64_t nnz, int64_t nraw)
+{
+  const int64_t nv = 4 * (nnz + 1), nr = nraw + 1;
+  if (nv <= ctx->nb_val_n && nr <= ctx->nb_raw_n) return SFR_OK;
+  TRY(hipStreamSynchronize(ctx->stream));
+  drop_graphs(ctx);
+  hipFree(ctx->nb_val);
+  hipFree(ctx->nb_raw);
+  ctx->nb_val = ctx->nb_raw = NULL;
+  ctx->nb_val_n = ctx->nb_raw_n = 0;
+  TRY(hipMalloc((void **)&ctx->nb_val, sizeof(double) * (size_t)nv));
+  TRY(hipMalloc((void **)&ctx->nb_raw, sizeof(double) * (size_t)nr));
+  ctx->nb_val_n = nv;
+  ctx->nb_raw_n = nr;
+  return SFR_OK;
+}
+
 sfr_mat_view sfr_view(const sfr_matrix *A)
 {
   sfr_mat_view v;
@@ -208,6 +230,7 @@ sfr_mat_view sfr_view(const sfr_matrix *A)
   v.dinv = A->dinv; v.val32 = A->val32; v.dscale = A->dscale;
   v.col16 = A->col16;
   v.exc_k = A->exc_k; v.exc_col = A->exc_col; v.nexc = A->nexc;
+  v.valnb = NULL;
   return v;
 }
 

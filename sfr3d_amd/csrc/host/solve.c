@@ -30,7 +30,8 @@ int sfr_workspace(sfr_context *ctx, int64_t n);
 int sfr_matrix_prepare_mixed(sfr_context *ctx, sfr_matrix *A);
 int sfr_matrix_prepare_adjoint(sfr_context *ctx, sfr_matrix *A);
 
-enum { G_PCG = 1, G_BICG = 2, G_MX = 3, G_BB = 4, G_GM = 5, G_MR = 6, G_B2 = 7 };
+enum { G_PCG = 1, G_BICG = 2, G_MX = 3, G_BB = 4, G_GM = 5, G_MR = 6, G_B2 = 7,
+       G_BBV = 8 };
 
 /* BiCGSTAB(2) iterations (two BiCG steps and a GCR(2) step, nine kernels)
  * per captured chunk */
@@ -116,7 +117,7 @@ static int get_graph(sfr_context *ctx, int kind, const void *k0,
       MR_RINGS(ctx);
       lrc = sfr_k_mr_iter(s, v, g, (double *)k1, V_, W_, Z_, it % 6, ctx->d_sc, ctx->part);
     }
-    else                    /* G_BB: k2 carries the batch size */
+    else                    /* G_BB, G_BBV: k2 carries the batch size */
       lrc = sfr_k_bb_iter(s, v, g, (int)(intptr_t)k2, w[8], w[9], w[0], w[1], w[2], w[3],
                           w[4], w[5], w[6], ctx->d_sc, ctx->part);
   }
@@ -367,6 +368,15 @@ static int solve_pcg_mixed(sfr_context *ctx, sfr_matrix *A, double tol,
 /* Shared-pattern batch: interleave the system-major inputs, run the batched
  * BiCGSTAB, de-interleave x.  Workspace (interleaved, nsys*n each):
  * w0 x, w1 r, w2 r0, w3 p, w4 z, w5 v, w6 t, w7 b, w8 ad, w9 dinv. */
+/* The batched BiCGSTAB of nsys (2..4) systems on A's pattern: d_ad, d_b, d_x
+ * system-major (system c at c*n); valnb = the systems' own values
+ * interleaved per device entry (species), or NULL for A's shared values
+ * (u/v/w momentum). */
+static int shared_run(sfr_context *ctx, sfr_matrix *A, int nsys,
+                      const double *d_ad, const double *d_b, double *d_x,
+                      const double *valnb, double tol, unsigned maxit,
+                      int newX, sfr_stats *stats);
+
 int sfr_solve_shared_device(sfr_context *ctx, sfr_matrix *A, int nsys,
                             const double *d_ad, const double *d_b, double *d_x,
                             int solver, double tol, unsigned maxit, int newX,
@@ -382,14 +392,24 @@ int sfr_solve_shared_device(sfr_context *ctx, sfr_matrix *A, int nsys,
     return sfr_set_error(ctx, SFR_NOT_SUPPORTED, "shared-pattern batch: solver %d", solver);
   if (maxit == 0) return sfr_set_error(ctx, SFR_BAD_ARG, "maxit = 0");
   TRY(hipSetDevice(ctx->device));
+  return shared_run(ctx, A, nsys, d_ad, d_b, d_x, NULL, tol, maxit, newX, stats);
+}
+
+static int shared_run(sfr_context *ctx, sfr_matrix *A, int nsys,
+                      const double *d_ad, const double *d_b, double *d_x,
+                      const double *valnb, double tol, unsigned maxit,
+                      int newX, sfr_stats *stats)
+{
   const int64_t n = A->n, tot = n * nsys;
   CHECK(sfr_workspace(ctx, tot));
   sfr_mat_view v = sfr_view(A);
+  v.valnb = valnb;
   sfr_geom g = {A->rpt, 0};
   double *const *w = ctx->w64;
   hipStream_t s = ctx->stream;
   hipGraphExec_t ex;
-  CHECK(get_graph(ctx, G_BB, A, w[0], (const void *)(intptr_t)nsys, &v, g, &ex));
+  CHECK(get_graph(ctx, valnb ? G_BBV : G_BB, A, w[0], (const void *)(intptr_t)nsys, &v, g,
+                  &ex));
   TRY(hipEventRecord(ctx->ev_start, s));
   if (d_ad) TRY(sfr_k_interleave(s, n, nsys, d_ad, w[8], w[9]));
   else for (int c = 0; c < nsys; c++) {
@@ -417,6 +437,97 @@ int sfr_solve_shared_device(sfr_context *ctx, sfr_matrix *A, int nsys,
     sfr_fill_stats(&ctx->h_sc[c], &st, ms);
     if (stats) stats[c] = st;
     if (st.status != SFR_OK && worst == SFR_OK) worst = st.status;
+  }
+  return worst;
+}
+
+/* Species batch, host arrays: the reference's per-species solverC loop
+ * (src/Combustion.c:76-92, 222-240) on one pattern.  BiCGSTAB systems run
+ * in groups of 2-4 through the batched kernels with each system's values
+ * interleaved per device entry; another solver id, storage CSR (diagonal
+ * inside al) or a single system are solved one after another. */
+int sfr_solve_batched(sfr_context *ctx, const sfr_system *sys, int nsys,
+                      const double *const *al, const double *const *ad,
+                      const double *const *au, const double *const *b,
+                      double *const *x, int solver, double tol, unsigned maxit,
+                      int newX, sfr_stats *stats)
+{
+  if (!ctx || !sys || !al || !ad || !b || !x || nsys < 1 || nsys > SFR_BATCH_MAX)
+    return SFR_BAD_ARG;
+  if (maxit == 0) return sfr_set_error(ctx, SFR_BAD_ARG, "maxit = 0");
+  TRY(hipSetDevice(ctx->device));
+  if (solver != SFR_PBICGSTAB || sys->storage == SFR_CSR || nsys == 1) {
+    int worst = SFR_OK;
+    for (int c = 0; c < nsys; c++) {
+      sfr_system sc = *sys;
+      sc.al = al[c];
+      sc.ad = ad[c];
+      sc.au = au ? au[c] : NULL;
+      sfr_stats st;
+      const int rc = sfr_solve(ctx, &sc, solver, tol, maxit, b[c], x[c], newX, &st);
+      if (rc == SFR_HIP_ERROR || rc == SFR_BAD_ARG || rc == SFR_NOT_SUPPORTED) return rc;
+      if (stats) stats[c] = st;
+      if (rc != SFR_OK && worst == SFR_OK) worst = rc;
+    }
+    return worst;
+  }
+  sfr_matrix *A = ctx->cached;
+  if (A && (A->h_ia != sys->ia || A->h_ja != sys->ja || A->n != sys->neq ||
+            A->nad != sys->nad || A->storage != sys->storage || A->unsym != sys->unsym)) {
+    sfr_matrix_destroy(ctx, A);
+    A = NULL;
+  }
+  if (!A) {
+    sfr_system s0 = *sys;
+    s0.al = al[0];
+    s0.ad = ad[0];
+    s0.au = au ? au[0] : NULL;
+    CHECK(sfr_matrix_create(ctx, &s0, SFR_FMT_AUTO, &A));
+    ctx->cached = A;
+  }
+  CHECK(sfr_nb_buffers(ctx, A->nnz, A->nraw));
+  const int64_t n = A->n, tot = 4 * n;
+  if (ctx->stage_n < 3 * tot) {
+    TRY(hipStreamSynchronize(ctx->stream));
+    hipFree(ctx->stage);
+    ctx->stage = NULL;
+    ctx->stage_n = 0;
+    TRY(hipMalloc((void **)&ctx->stage, sizeof(double) * (size_t)(3 * tot)));
+    ctx->stage_n = 3 * tot;
+  }
+  double *d_ad = ctx->stage, *d_b = ctx->stage + tot, *d_x = ctx->stage + 2 * tot;
+  const size_t bytes = sizeof(double) * (size_t)n;
+  const int csrc_unsym = (A->storage == SFR_CSRC || A->storage == SFR_CSRCCOO) && A->unsym;
+  hipStream_t s = ctx->stream;
+  /* groups of 2..4 systems, as even as possible */
+  const int ng = (nsys + 3) / 4, base = nsys / ng, rem = nsys % ng;
+  int worst = SFR_OK;
+  for (int gi = 0, c0 = 0; gi < ng; gi++) {
+    const int nb = base + (gi < rem ? 1 : 0);
+    for (int j = 0; j < nb; j++) {
+      const int c = c0 + j;
+      if (csrc_unsym) {
+        TRY(hipMemcpyAsync(ctx->nb_raw, al[c], sizeof(double) * (size_t)A->nad,
+                           hipMemcpyHostToDevice, s));
+        TRY(hipMemcpyAsync(ctx->nb_raw + A->nad, (au && au[c]) ? au[c] : al[c] + A->nad,
+                           sizeof(double) * (size_t)A->nad, hipMemcpyHostToDevice, s));
+      } else if (A->nraw > 0) {
+        TRY(hipMemcpyAsync(ctx->nb_raw, al[c], sizeof(double) * (size_t)A->nraw,
+                           hipMemcpyHostToDevice, s));
+      }
+      TRY(sfr_k_gather_nb(s, A->nnz, A->vmap, ctx->nb_raw, ctx->nb_val, nb, j));
+      TRY(hipMemcpyAsync(d_ad + j * n, ad[c], bytes, hipMemcpyHostToDevice, s));
+      TRY(hipMemcpyAsync(d_b + j * n, b[c], bytes, hipMemcpyHostToDevice, s));
+      if (!newX) TRY(hipMemcpyAsync(d_x + j * n, x[c], bytes, hipMemcpyHostToDevice, s));
+    }
+    const int rc = shared_run(ctx, A, nb, d_ad, d_b, d_x, ctx->nb_val, tol, maxit, newX,
+                              stats ? stats + c0 : NULL);
+    if (rc == SFR_HIP_ERROR || rc == SFR_BAD_ARG || rc == SFR_NOT_SUPPORTED) return rc;
+    if (rc != SFR_OK && worst == SFR_OK) worst = rc;
+    for (int j = 0; j < nb; j++)
+      TRY(hipMemcpyAsync(x[c0 + j], d_x + j * n, bytes, hipMemcpyDeviceToHost, s));
+    TRY(hipStreamSynchronize(s));
+    c0 += nb;
   }
   return worst;
 }

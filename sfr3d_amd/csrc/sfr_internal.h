@@ -73,6 +73,8 @@ typedef struct sfr_mat_view {
   const int64_t *exc_k;     /* entries whose offset is SFR_D16_ESC: their  */
   const int32_t *exc_col;   /*   slot index f*n+i and column               */
   int            nexc;
+  const double  *valnb;     /* batched kernels: per-system values          */
+                            /*   interleaved, valnb[k*nb + c], or NULL     */
 } sfr_mat_view;
 
 typedef struct sfr_geom { int rpt; int grid; } sfr_geom;
@@ -127,6 +129,9 @@ struct sfr_context {
   int          chunk;           /* iterations per captured chunk           */
   double      *gm_buf;          /* GMRES: basis, raw vectors, state        */
   int64_t      gm_n, gm_ld;
+  double      *nb_val;          /* species batch: interleaved values       */
+  double      *nb_raw;          /*   and one system's raw values           */
+  int64_t      nb_val_n, nb_raw_n;
   double       ru_delta;        /* MIXED: reliable-update residual drop    */
 };
 
@@ -176,6 +181,7 @@ int sfr_k_final(hipStream_t s, const sfr_mat_view *A, sfr_geom g,
 
 /* GMRES(SFR_GM_K), FP64 (reference src/Iters.c:587-790) */
 int sfr_gm_workspace(sfr_context *ctx, int64_t n);
+int sfr_nb_buffers(sfr_context *ctx, int64_t nnz, int64_t nraw);
 int sfr_k_gm_setup(hipStream_t s, const sfr_mat_view *A, sfr_geom g,
                    const double *b, double *x, int newX, double tol,
                    unsigned maxit, sfr_scalars *sc, double *part);
@@ -228,6 +234,9 @@ int sfr_k_bb_final(hipStream_t s, const sfr_mat_view *A, sfr_geom g, int nb,
                    const double *ad, const double *dinv, const double *b,
                    const double *x, sfr_scalars *sc, double *part);
 /* system-major src[c*n+i] -> interleaved dst[i*nb+c] (inv: 1/value, or NULL) */
+/* valnb[k*nb + c] = raw[vmap[k]] (0 where vmap[k] < 0) for k < nnz */
+int sfr_k_gather_nb(hipStream_t s, int64_t nnz, const int32_t *vmap,
+                    const double *raw, double *valnb, int nb, int c);
 int sfr_k_interleave(hipStream_t s, int64_t n, int nb, const double *src,
                      double *dst, double *inv);
 int sfr_k_deinterleave(hipStream_t s, int64_t n, int nb, const double *src,

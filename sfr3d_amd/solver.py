@@ -276,6 +276,28 @@ class Context:
         self._check(rc, allow=(L.MAXIT, L.BREAKDOWN))
         return xs, [st[c].as_dict() for c in range(nsys)]
 
+    def solve_batched(self, system, solver, tol, maxit, als, ads, bs,
+                      aus=None, xs=None, newX=True):
+        """Species batch (sfr_solve_batched): system's pattern, system c's
+        values als[c], ads[c] (aus[c]), right-hand side bs[c].
+        Returns (list of x, list of stats dicts)."""
+        nsys = len(bs)
+        f64 = lambda a: np.ascontiguousarray(a, np.float64)
+        als, ads, bs = [f64(a) for a in als], [f64(a) for a in ads], [f64(b) for b in bs]
+        aus = [f64(a) for a in aus] if aus is not None else None
+        xs = ([np.zeros(system.neq) for _ in range(nsys)] if xs is None
+              else [np.array(x, np.float64) for x in xs])
+        P = C.c_void_p * nsys
+        arr = lambda lst: C.cast(P(*[ptr(a) for a in lst]), C.c_void_p)
+        st = (SfrStats * nsys)()
+        s = system.c()
+        rc = lib.sfr_solve_batched(self.h, C.byref(s), nsys, arr(als), arr(ads),
+                                   arr(aus) if aus is not None else None,
+                                   arr(bs), arr(xs), solver, tol, maxit,
+                                   1 if newX else 0, st)
+        self._check(rc, allow=(L.MAXIT, L.BREAKDOWN))
+        return xs, [st[c].as_dict() for c in range(nsys)]
+
     def solve_adjoint(self, A, solver, tol, maxit, d_x, d_g, d_lam,
                       d_grad_al=None, d_grad_ad=None):
         """A^T lambda = g and the value gradients (sfr_solve_adjoint)."""

@@ -253,6 +253,64 @@ def momentum3d(n, seed=1, storage=CSRD, reorder=False):
                  {"b_all": np.stack(bs), "lA": lA})
 
 
+def species3d(n, nsys, seed=1, storage=CSRD, reorder=False):
+    """nsys transported scalars (SFR3D's species, src/Combustion.c:222-240)
+    on the 96^3-style periodic Taylor-Green box of momentum3d: the same
+    pattern and convective fluxes, FOUP, implicit Euler, but each species
+    with its own diffusivity nu_c = h*Sc_c/10 (Sc_c log-uniform in
+    [0.5, 2]), so its own off-diagonals and diagonal, Dirichlet walls at
+    phi = 0, source U[-1, 1] and old value U[0, 1].  Returns (system of
+    species 0, list of (al, ad, b) per species)."""
+    rng = np.random.default_rng(seed)
+    h = 2.0 * math.pi / n
+    dt, V, A = 0.5 * h, h ** 3, h * h
+    c = (np.arange(n) + 0.5) * h
+    z = c[:, None, None]
+    y = c[None, :, None]
+    x = c[None, None, :]
+    shp = (n, n, n)
+    numel = n ** 3
+    faces = [((x, y, z - h / 2), (0, 0, -1)), ((x, y, z + h / 2), (0, 0, 1)),
+             ((x, y - h / 2, z), (0, -1, 0)), ((x + h / 2, y, z), (1, 0, 0)),
+             ((x, y + h / 2, z), (0, 1, 0)), ((x - h / 2, y, z), (-1, 0, 0))]
+    k = np.arange(n)[:, None, None]
+    j = np.arange(n)[None, :, None]
+    i = np.arange(n)[None, None, :]
+    interior = [np.broadcast_to(k > 0, shp), np.broadcast_to(k < n - 1, shp),
+                np.broadcast_to(j > 0, shp), np.broadcast_to(i < n - 1, shp),
+                np.broadcast_to(j < n - 1, shp), np.broadcast_to(i > 0, shp)]
+    flux = []
+    for (fx, fy, fz), nrm in faces:
+        uf = _tg(fx, fy, fz)
+        flux.append(np.broadcast_to((uf[0] * nrm[0] + uf[1] * nrm[1] + uf[2] * nrm[2]) * A, shp))
+    nelcon, nviz = S.hex_adjacency(n, n, n)
+    num = S.reord(nelcon, nviz, reorder)
+    id_, neq = S.numeq_v1(num)
+    sysm, pat, out = None, None, []
+    for sp in range(nsys):
+        nu = h * float(np.exp(rng.uniform(np.log(0.5), np.log(2.0)))) / 10.0
+        D = nu * A / h
+        lA = np.zeros((numel, 7))
+        diag = np.full(shp, V / dt)
+        for f in range(6):
+            F = flux[f]
+            anb = D + np.maximum(-F, 0.0)
+            diag = diag + np.where(interior[f], D + np.maximum(F, 0.0),
+                                   2.0 * D + np.maximum(F, 0.0))
+            lA[:, f] = np.where(interior[f], -anb, 0.0).ravel()
+        lA[:, 6] = diag.ravel()
+        rhs = (V * rng.uniform(-1.0, 1.0, numel)
+               + V / dt * rng.uniform(0.0, 1.0, numel))
+        if sysm is None:
+            sysm, b0, pat = assemble_system(nelcon, nviz, num, id_, neq, storage,
+                                            True, lA, rhs)
+            out.append((sysm.al, sysm.ad, b0))
+        else:
+            al, ad, bc = S.assemble(pat, id_, nelcon, nviz, lA, rhs)
+            out.append((al, ad[:neq].copy(), bc[:neq].copy()))
+    return sysm, out
+
+
 # ---------------------------------------------------------------------------
 # Uneven batch: nsys independent all-Neumann 3-D Poisson systems, cube side
 # n_s = round(smin * (smax/smin)^u), u ~ U[0,1) (log-uniform), conductances

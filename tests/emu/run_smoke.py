@@ -1,7 +1,7 @@
 """TEST INFRASTRUCTURE -- runs the emulated library (argv[1]) on the small
 cases of argv[2] (written by tests/test_emulated_kernels.py with the
 oracle's solutions) in a process of its own: FP64 and mixed PCG, BiCGSTAB,
-BiCGSTAB(2), GMRES, MINRES,
+BiCGSTAB(2), GMRES, MINRES, the species batch,
 the shared-pattern batch, both adjoint paths, 16-bit column offsets with
 escaped far columns, and a tuned uneven batch.  Prints PASS/FAIL lines and
 exits non-zero on any failure."""
@@ -68,6 +68,19 @@ check(f"gmres mom rc {rc} cycles {st.outer}/{cyc[0]} it {st.iters}/{cyc[1]} rel 
 xs = np.zeros(3 * s.neq); sts = (St * 3)(); flat = np.ascontiguousarray(bb.reshape(-1))
 rc = lib.sfr_solve_shared_device(ctx, A, 3, None, flat.ctypes.data, xs.ctypes.data, 2, 1e-8, maxit, 1, C.cast(sts, vp))
 check(f"shared rc {rc} rel {rel(xs[:s.neq], xo):.1e}", rc == 0 and rel(xs[:s.neq], x) < 1e-12)
+lib.sfr_solve_batched.argtypes = [vp, vp, C.c_int, vp, vp, vp, vp, vp, C.c_int, C.c_double, C.c_uint, C.c_int, vp]
+P3 = vp * 3
+spal = [np.ascontiguousarray(z[f"sp{c}_al"]) for c in range(3)]
+spad = [np.ascontiguousarray(z[f"sp{c}_ad"]) for c in range(3)]
+spb = [np.ascontiguousarray(bb[c]) for c in range(3)]
+spx = [np.zeros(s.neq) for c in range(3)]
+sts = (St * 3)()
+rc = lib.sfr_solve_batched(ctx, C.byref(s), 3, C.cast(P3(*[a.ctypes.data for a in spal]), vp),
+                           C.cast(P3(*[a.ctypes.data for a in spad]), vp), None,
+                           C.cast(P3(*[a.ctypes.data for a in spb]), vp),
+                           C.cast(P3(*[a.ctypes.data for a in spx]), vp), 2, 1e-10, maxit, 1, C.cast(sts, vp))
+rs = max(rel(spx[c], z[f"sp{c}_xo"]) for c in range(3))
+check(f"species rc {rc} it {[sts[c].iters for c in range(3)]} rel {rs:.1e}", rc == 0 and rs < 1e-8)
 lam = np.zeros(s.neq); gal = np.zeros(len(z["mom_al"])); gad = np.zeros(s.neq); st = St()
 rc = lib.sfr_solve_adjoint(ctx, A, 2, 1e-8, maxit, x.ctypes.data, bb[0].ctypes.data, lam.ctypes.data, gal.ctypes.data, gad.ctypes.data, C.byref(st))
 check(f"adjoint bicg rc {rc} it {st.iters}", rc == 0 and st.iters > 0)

@@ -15,6 +15,7 @@ static inline void __sanitizer_finish_switch_fiber(void *, const void **, size_t
 #include <condition_variable>
 #include <cstdio>
 #include <cstdlib>
+#include <map>
 #include <mutex>
 #include <thread>
 #include <vector>
@@ -133,20 +134,62 @@ static double now_ms()
              std::chrono::steady_clock::now().time_since_epoch()).count();
 }
 
+// Device allocations, so that copies can be checked the way the runtime
+// checks them: a device-side range must lie inside one allocation.
+static std::map<uintptr_t, size_t> g_dev;
+
+static bool dev_range(const void *p, size_t n)
+{
+  const uintptr_t a = (uintptr_t)p;
+  auto it = g_dev.upper_bound(a);
+  if (it == g_dev.begin()) return false;
+  --it;
+  return a >= it->first && a + n <= it->first + it->second;
+}
+
+static hipError_t check_copy(void *d, const void *s, size_t n, hipMemcpyKind k)
+{
+  const bool dd = k == hipMemcpyHostToDevice || k == hipMemcpyDeviceToDevice;
+  const bool sd = k == hipMemcpyDeviceToHost || k == hipMemcpyDeviceToDevice;
+  if (!d || !s || (dd && !dev_range(d, n)) || (sd && !dev_range(s, n))) {
+    fprintf(stderr, "emu: invalid copy d=%p s=%p n=%zu kind=%d\n", d, s, n, (int)k);
+    return hipErrorInvalidValue;
+  }
+  return hipSuccess;
+}
+
 extern "C" {
 hipError_t hipMalloc(void **p, size_t n)
 {
   *p = malloc(n ? n : 1);
   memset(*p, 0x7f, n);
+  g_dev[(uintptr_t)*p] = n ? n : 1;
   return hipSuccess;
 }
-hipError_t hipFree(void *p) { free(p); return hipSuccess; }
-hipError_t hipHostMalloc(void **p, size_t n, unsigned int) { return hipMalloc(p, n); }
+hipError_t hipFree(void *p)
+{
+  if (p) g_dev.erase((uintptr_t)p);
+  free(p);
+  return hipSuccess;
+}
+hipError_t hipHostMalloc(void **p, size_t n, unsigned int)
+{
+  *p = malloc(n ? n : 1);
+  return hipSuccess;
+}
 hipError_t hipHostFree(void *p) { free(p); return hipSuccess; }
-hipError_t hipMemcpy(void *d, const void *s, size_t n, hipMemcpyKind) { memcpy(d, s, n); return hipSuccess; }
-hipError_t hipMemcpyAsync(void *d, const void *s, size_t n, hipMemcpyKind, hipStream_t)
+hipError_t hipMemcpy(void *d, const void *s, size_t n, hipMemcpyKind k)
+{
+  const hipError_t e = check_copy(d, s, n, k);
+  if (e != hipSuccess) return e;
+  memcpy(d, s, n);
+  return hipSuccess;
+}
+hipError_t hipMemcpyAsync(void *d, const void *s, size_t n, hipMemcpyKind k, hipStream_t)
 {
   if (emu::capturing) { fprintf(stderr, "emu: memcpy in capture\n"); abort(); }
+  const hipError_t e = check_copy(d, s, n, k);
+  if (e != hipSuccess) return e;
   memcpy(d, s, n);
   return hipSuccess;
 }

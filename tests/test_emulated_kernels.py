@@ -4,7 +4,7 @@ the resulting library is driven through the same C ABI as the GPU build.
 This checks kernel logic -- indexing, the scalar protocol between kernels,
 graph capture and replay, the fused mixed-precision iteration, GMRES's
 restart cycle, MINRES's ring buffers and parity-alternated scalars,
-BiCGSTAB(2)'s nine-kernel iteration, escaped 16-bit columns, the uneven batch, device assembly --
+BiCGSTAB(2)'s nine-kernel iteration, the species batch (per-system values), escaped 16-bit columns, the uneven batch, device assembly --
 against the oracle and
 the host assembler without a GPU.
 Set SFR_EMU_ASAN=1 to build it under AddressSanitizer (slower), which
@@ -45,6 +45,13 @@ def write_cases(path):
     put("mom", M.system, M.extra["b_all"], 2, 1000)
     xb, sb = oracle.pbicgstabl2(M.system, M.extra["b_all"][1], 1e-10, 1000)
     d["mom_b2_xo"], d["mom_b2_it"] = xb, np.array([sb["iters"]])
+    # species batch: three value sets on the momentum pattern
+    for c in range(3):
+        sc = System(M.system.neq, M.system.nad, M.system.ia, M.system.ja,
+                    M.system.al * (1.0 + 0.2 * c), M.system.ad * (1.0 + 0.2 * c) + 0.1 * c,
+                    M.system.storage, M.system.unsym)
+        d[f"sp{c}_al"], d[f"sp{c}_ad"] = sc.al, sc.ad
+        d[f"sp{c}_xo"] = oracle.pbicgstab(sc, M.extra["b_all"][c], 1e-10, 1000)[0]
     xg, sg = oracle.gmres(M.system, M.extra["b_all"][0], 1e-10, 1000)
     d["mom_gm_xo"], d["mom_gm_cyc"] = xg, np.array([sg["cycles"], sg["iters"]])
     B = systems.poisson3d(6, seed=11)
@@ -98,4 +105,4 @@ def test_kernels_under_cpu_emulation(tmp_path):
     print(res.stdout)
     assert res.returncode == 0, res.stdout + res.stderr[-3000:]
     assert "SMOKE OK" in res.stdout
-    assert res.stdout.count("PASS") == 13
+    assert res.stdout.count("PASS") == 14
