@@ -51,6 +51,12 @@ def main():
     ap.add_argument("--profile-kernels", action="store_true",
                     help="only launch the two SpMV kernels and a plain copy "
                          "of known size (for rocprofv3 counter runs)")
+    ap.add_argument("--only", choices=["solver_ids", "species", "uvw", "uneven",
+                                       "adjoint"],
+                    help="run one of the other workloads alone and print its "
+                         "JSON (for rocprofv3 runs of that workload)")
+    ap.add_argument("--solvers", default="PBICGSTAB,PBICGSTABL2,GMRES,PCG,MINRES",
+                    help="solver ids of the solver_ids workload (comma list)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is "
@@ -61,6 +67,18 @@ def main():
     import sfr3d_amd as S
     from sfr3d_amd import systems
     from sfr3d_amd._lib import lib
+
+    if args.only:
+        ctx = S.Context(0)
+        leg = {"solver_ids": lambda: bench_solvers(ctx, S, systems, cpu=not args.no_cpu,
+                                                   names=args.solvers.split(",")),
+               "species": lambda: bench_species(ctx, S, systems),
+               "uvw": lambda: bench_uvw(ctx, S, systems, lib),
+               "uneven": lambda: bench_uneven(ctx, S, systems),
+               "adjoint": lambda: bench_adjoint(ctx, S, systems)}[args.only]
+        print(json.dumps({args.only: leg()}), flush=True)
+        ctx.close()
+        return
 
     n = args.n
     t0 = time.time()
@@ -475,7 +493,7 @@ def bench_species(ctx, S, systems, reps=3):
     return out
 
 
-def bench_solvers(ctx, S, systems, cpu=True, reps=3):
+def bench_solvers(ctx, S, systems, cpu=True, reps=3, names=None):
     """Every solver id on the system it is meant for, device times:
     BiCGSTAB, BiCGSTAB(2) and GMRES(15) on the 96^3 upwind momentum system
     (u component), PCG and MINRES on the 96^3 variable-coefficient Poisson
@@ -490,6 +508,8 @@ def bench_solvers(ctx, S, systems, cpu=True, reps=3):
             ("GMRES", S.GMRES, M, M.extra["b_all"][0]),
             ("PCG", S.PCG, P, P.b),
             ("MINRES", S.MINRES, P, P.b)]
+    if names:
+        legs = [leg for leg in legs if leg[0] in names]
     mats = {}
     for name, sid, B, b in legs:
         key = id(B)

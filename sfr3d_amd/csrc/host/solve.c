@@ -62,17 +62,28 @@ static int gm_cycles_per_chunk(const sfr_context *ctx)
   return c < 1 ? 1 : c;
 }
 
-/* Capture `chunk` iterations of one loop body, or reuse a captured one. */
-static int get_graph(sfr_context *ctx, int kind, const void *k0,
-                     const void *k1, const void *k2, const sfr_mat_view *v,
-                     sfr_geom g, hipGraphExec_t *exec)
+/* Iterations per captured chunk of a loop kind: the full chunk, or the
+ * short one a solve starts with (see run_loop). */
+static int chunk_of(const sfr_context *ctx, int kind, int small)
 {
   /* the mixed loop alternates its direction buffers: an even chunk ends
-   * where it started */
-  const int chunk = kind == G_MX ? (ctx->chunk + 1) & ~1
-                  : kind == G_GM ? gm_cycles_per_chunk(ctx)
-                  : kind == G_MR ? mr_chunk(ctx)
-                  : kind == G_B2 ? b2_chunk(ctx) : ctx->chunk;
+   * where it started; MINRES's rings repeat every 6 iterations */
+  switch (kind) {
+  case G_MX: return small ? 2 : (ctx->chunk + 1) & ~1;
+  case G_GM: return gm_cycles_per_chunk(ctx);
+  case G_MR: return small ? 6 : mr_chunk(ctx);
+  case G_B2: return small ? 1 : b2_chunk(ctx);
+  default:   return small ? (ctx->chunk < 2 ? ctx->chunk : 2) : ctx->chunk;
+  }
+}
+
+/* Capture chunk_of(kind, small) iterations of one loop body, or reuse a
+ * captured one. */
+static int get_graph(sfr_context *ctx, int kind, const void *k0,
+                     const void *k1, const void *k2, const sfr_mat_view *v,
+                     sfr_geom g, int small, hipGraphExec_t *exec)
+{
+  const int chunk = chunk_of(ctx, kind, small);
   for (int i = 0; i < ctx->ngraph; i++) {
     sfr_graph *e = &ctx->graphs[i];
     if (e->kind == kind && e->chunk == chunk && e->key[0] == k0 &&
@@ -176,10 +187,60 @@ static int run_loop_n(sfr_context *ctx, hipGraphExec_t ex, int which,
   return SFR_OK;
 }
 
-static int run_loop(sfr_context *ctx, hipGraphExec_t ex, int which,
-                    unsigned max_chunks, int nsys)
+/* The loop of a solve: short chunks (exs, cs iterations each) for its first
+ * `ramp` iterations, then full ones (exb, cb), two in flight, at most
+ * max_iters iterations issued.  Most solves of a SIMPLE step converge in a
+ * handful of iterations: starting short bounds the early-exit launches
+ * issued past convergence to two short chunks, while a long solve soon
+ * runs on full chunks. */
+static int run_loop_ramp(sfr_context *ctx, hipGraphExec_t exs, unsigned cs,
+                         hipGraphExec_t exb, unsigned cb, unsigned ramp,
+                         int which, unsigned max_iters, int nsys)
 {
-  return run_loop_n(ctx, ex, which, max_chunks, nsys, 2);
+  hipStream_t s = ctx->stream;
+  unsigned launched = 0, seen = 0, iters = 0;
+  for (;;) {
+    while (launched - seen < 2 && iters < max_iters) {
+      const int slot = (int)(launched & 1);
+      const int small = iters < ramp;
+      TRY(hipGraphLaunch(small ? exs : exb, s));
+      TRY(hipMemcpyAsync(ctx->h_poll + SFR_MAXSYS * slot, ctx->d_sc,
+                         sizeof(sfr_scalars) * (size_t)nsys, hipMemcpyDeviceToHost, s));
+      TRY(hipEventRecord(ctx->ev_poll[slot], s));
+      iters += small ? cs : cb;
+      launched++;
+    }
+    if (seen == launched) break;                 /* iteration budget spent */
+    const int slot = (int)(seen & 1);
+    TRY(hipEventSynchronize(ctx->ev_poll[slot]));
+    seen++;
+    const sfr_scalars *p = ctx->h_poll + SFR_MAXSYS * slot;
+    int finished = 1;
+    for (int c = 0; c < nsys; c++)
+      if (!(p[c].done || (which && p[c].ru))) finished = 0;
+    if (finished) break;
+  }
+  return SFR_OK;
+}
+
+/* The short and the full chunk of a loop kind (captured before a solve
+ * enqueues anything) and the loop over them for up to maxit iterations. */
+typedef struct { hipGraphExec_t s, b; unsigned cs, cb; } chunks;
+
+static int get_chunks(sfr_context *ctx, int kind, const void *k0, const void *k1,
+                      const void *k2, const sfr_mat_view *v, sfr_geom g, chunks *c)
+{
+  CHECK(get_graph(ctx, kind, k0, k1, k2, v, g, 1, &c->s));
+  CHECK(get_graph(ctx, kind, k0, k1, k2, v, g, 0, &c->b));
+  c->cs = (unsigned)chunk_of(ctx, kind, 1);
+  c->cb = (unsigned)chunk_of(ctx, kind, 0);
+  return SFR_OK;
+}
+
+static int run_chunks(sfr_context *ctx, const chunks *c, int which, unsigned maxit,
+                      int nsys)
+{
+  return run_loop_ramp(ctx, c->s, c->cs, c->b, c->cb, c->cb, which, maxit + 2 * c->cb, nsys);
 }
 
 void sfr_fill_stats(const sfr_scalars *sc, sfr_stats *st, float ms)
@@ -221,11 +282,6 @@ static int finish(sfr_context *ctx, const sfr_mat_view *v, sfr_geom g,
   return tmp.status;
 }
 
-static unsigned chunk_budget(const sfr_context *ctx, unsigned maxit)
-{
-  const unsigned c = (unsigned)ctx->chunk;
-  return maxit / c + 2;
-}
 
 static int solve_pcg64(sfr_context *ctx, sfr_matrix *A, double tol,
                        unsigned maxit, const double *d_b, double *d_x, int newX,
@@ -234,12 +290,12 @@ static int solve_pcg64(sfr_context *ctx, sfr_matrix *A, double tol,
   CHECK(sfr_workspace(ctx, A->n));
   sfr_mat_view v = sfr_view(A);
   sfr_geom g = {A->rpt, 0};
-  hipGraphExec_t ex;
-  CHECK(get_graph(ctx, G_PCG, A, d_x, NULL, &v, g, &ex));
+  chunks ch;
+  CHECK(get_chunks(ctx, G_PCG, A, d_x, NULL, &v, g, &ch));
   TRY(hipEventRecord(ctx->ev_start, ctx->stream));
   TRY(sfr_k_pcg_setup(ctx->stream, &v, g, d_b, d_x, ctx->w64[0], ctx->w64[1], newX, tol,
                       maxit, ctx->d_sc, ctx->part));
-  CHECK(run_loop(ctx, ex, 0, chunk_budget(ctx, maxit), 1));
+  CHECK(run_chunks(ctx, &ch, 0, maxit, 1));
   return finish(ctx, &v, g, d_b, d_x, st);
 }
 
@@ -250,13 +306,13 @@ static int solve_bicg64(sfr_context *ctx, sfr_matrix *A, double tol,
   CHECK(sfr_workspace(ctx, A->n));
   sfr_mat_view v = sfr_view(A);
   sfr_geom g = {A->rpt, 0};
-  hipGraphExec_t ex;
-  CHECK(get_graph(ctx, G_BICG, A, d_x, NULL, &v, g, &ex));
+  chunks ch;
+  CHECK(get_chunks(ctx, G_BICG, A, d_x, NULL, &v, g, &ch));
   double *const *w = ctx->w64;
   TRY(hipEventRecord(ctx->ev_start, ctx->stream));
   TRY(sfr_k_bicg_setup(ctx->stream, &v, g, d_b, d_x, w[0], w[1], w[2], w[3], newX, tol,
                        maxit, ctx->d_sc, ctx->part));
-  CHECK(run_loop(ctx, ex, 0, chunk_budget(ctx, maxit), 1));
+  CHECK(run_chunks(ctx, &ch, 0, maxit, 1));
   return finish(ctx, &v, g, d_b, d_x, st);
 }
 
@@ -274,7 +330,7 @@ static int solve_gmres64(sfr_context *ctx, sfr_matrix *A, double tol,
   sfr_mat_view v = sfr_view(A);
   sfr_geom g = {A->rpt, 0};
   hipGraphExec_t ex;
-  CHECK(get_graph(ctx, G_GM, A, d_x, d_b, &v, g, &ex));
+  CHECK(get_graph(ctx, G_GM, A, d_x, d_b, &v, g, 0, &ex));
   TRY(hipEventRecord(ctx->ev_start, ctx->stream));
   TRY(sfr_k_gm_setup(ctx->stream, &v, g, d_b, d_x, newX, tol, maxit, ctx->d_sc,
                      ctx->part));
@@ -303,12 +359,12 @@ static int solve_bicgl2_64(sfr_context *ctx, sfr_matrix *A, double tol,
   CHECK(sfr_workspace(ctx, A->n));
   sfr_mat_view v = sfr_view(A);
   sfr_geom g = {A->rpt, 0};
-  hipGraphExec_t ex;
-  CHECK(get_graph(ctx, G_B2, A, d_x, NULL, &v, g, &ex));
+  chunks ch;
+  CHECK(get_chunks(ctx, G_B2, A, d_x, NULL, &v, g, &ch));
   TRY(hipEventRecord(ctx->ev_start, ctx->stream));
   TRY(sfr_k_b2_setup(ctx->stream, &v, g, d_b, d_x, ctx->w64, newX, tol, maxit,
                      ctx->d_sc, ctx->part));
-  CHECK(run_loop(ctx, ex, 0, maxit / (unsigned)b2_chunk(ctx) + 2, 1));
+  CHECK(run_chunks(ctx, &ch, 0, maxit, 1));
   return finish(ctx, &v, g, d_b, d_x, st);
 }
 
@@ -321,13 +377,13 @@ static int solve_minres64(sfr_context *ctx, sfr_matrix *A, double tol,
   CHECK(sfr_workspace(ctx, A->n));
   sfr_mat_view v = sfr_view(A);
   sfr_geom g = {A->rpt, 0};
-  hipGraphExec_t ex;
-  CHECK(get_graph(ctx, G_MR, A, d_x, NULL, &v, g, &ex));
+  chunks ch;
+  CHECK(get_chunks(ctx, G_MR, A, d_x, NULL, &v, g, &ch));
   MR_RINGS(ctx);
   TRY(hipEventRecord(ctx->ev_start, ctx->stream));
   TRY(sfr_k_mr_setup(ctx->stream, &v, g, d_b, d_x, V_, W_, Z_, newX, tol, maxit,
                      ctx->d_sc, ctx->part));
-  CHECK(run_loop(ctx, ex, 0, maxit / (unsigned)mr_chunk(ctx) + 2, 1));
+  CHECK(run_chunks(ctx, &ch, 0, maxit, 1));
   return finish(ctx, &v, g, d_b, d_x, st);
 }
 
@@ -339,8 +395,8 @@ static int solve_pcg_mixed(sfr_context *ctx, sfr_matrix *A, double tol,
   CHECK(sfr_matrix_prepare_mixed(ctx, A));
   sfr_mat_view v = sfr_view(A);
   sfr_geom g = {A->rpt, 0};
-  hipGraphExec_t ex;
-  CHECK(get_graph(ctx, G_MX, A, NULL, NULL, &v, g, &ex));
+  chunks ch;
+  CHECK(get_chunks(ctx, G_MX, A, NULL, NULL, &v, g, &ch));
   float *const *w32 = ctx->w32;
   hipStream_t s = ctx->stream;
   TRY(hipEventRecord(ctx->ev_start, s));
@@ -353,7 +409,7 @@ static int solve_pcg_mixed(sfr_context *ctx, sfr_matrix *A, double tol,
   /* every fold is triggered by a residual drop of delta (or the stop test):
    * bound the number of folds generously */
   for (int folds = 0; folds < 100000; folds++) {
-    CHECK(run_loop(ctx, ex, 1, chunk_budget(ctx, maxit), 1));
+    CHECK(run_chunks(ctx, &ch, 1, maxit, 1));
     TRY(hipMemcpyAsync(ctx->h_poll, ctx->d_sc, sizeof(sfr_scalars), hipMemcpyDeviceToHost, s));
     TRY(hipStreamSynchronize(s));
     if (ctx->h_poll[0].done != SFR_DONE_RUN) break;
@@ -407,9 +463,9 @@ static int shared_run(sfr_context *ctx, sfr_matrix *A, int nsys,
   sfr_geom g = {A->rpt, 0};
   double *const *w = ctx->w64;
   hipStream_t s = ctx->stream;
-  hipGraphExec_t ex;
-  CHECK(get_graph(ctx, valnb ? G_BBV : G_BB, A, w[0], (const void *)(intptr_t)nsys, &v, g,
-                  &ex));
+  chunks ch;
+  CHECK(get_chunks(ctx, valnb ? G_BBV : G_BB, A, w[0], (const void *)(intptr_t)nsys, &v, g,
+                   &ch));
   TRY(hipEventRecord(ctx->ev_start, s));
   if (d_ad) TRY(sfr_k_interleave(s, n, nsys, d_ad, w[8], w[9]));
   else for (int c = 0; c < nsys; c++) {
@@ -422,7 +478,7 @@ static int shared_run(sfr_context *ctx, sfr_matrix *A, int nsys,
   if (!newX) TRY(sfr_k_interleave(s, n, nsys, d_x, w[0], NULL));
   TRY(sfr_k_bb_setup(s, &v, g, nsys, w[8], w[9], w[7], w[0], w[1], w[2], w[3], w[4], newX,
                      tol, maxit, ctx->d_sc, ctx->part));
-  CHECK(run_loop(ctx, ex, 0, chunk_budget(ctx, maxit), nsys));
+  CHECK(run_chunks(ctx, &ch, 0, maxit, nsys));
   TRY(sfr_k_bb_final(s, &v, g, nsys, w[8], w[9], w[7], w[0], ctx->d_sc, ctx->part));
   TRY(sfr_k_deinterleave(s, n, nsys, w[0], d_x));
   TRY(hipEventRecord(ctx->ev_stop, s));
