@@ -280,9 +280,11 @@ void sfr_assembly_destroy(sfr_context *ctx, sfr_assembly *plan);
  * the updated direction, direction and correction updates, p.q). */
 int sfr_kernel_time(sfr_context *ctx, sfr_matrix *A, int which, int reps,
                     double *ms_per_launch);
-/* Tuning knobs of a context: "chunk" = iterations per captured graph chunk
- * (1..256; GMRES captures chunk/8 restart cycles, at least one); "ru_delta" = MIXED: drop of the FP32 residual (relative to its
- * largest value since the last fold) that triggers an FP64 reliable update. */
+/* Tuning knobs of a context: "chunk" = iterations per full captured graph
+ * chunk (1..256; solves start on short chunks; GMRES captures chunk/16
+ * restart cycles, at least one); "ru_delta" = MIXED: drop of the FP32
+ * residual (relative to its largest value since the last fold) that
+ * triggers an FP64 reliable update. */
 int sfr_set_option(sfr_context *ctx, const char *name, double value);
 
 /* Drop-in twin of solverC (reference include/Solv.h:59-70), same prototype
