@@ -80,3 +80,13 @@ def test_batch_entry_points_refuse_bad_arguments():
     assert lib.sfr_batch_info(None, 0, None, None, None, None) == _lib.BAD_ARG
     assert lib.sfr_batch_buckets(None, None, 0) == 0
     lib.sfr_batch_destroy(None, None)
+
+
+def test_solve_entry_points_refuse_bad_arguments():
+    lib = _lib.lib
+    assert lib.sfr_solve_batched(None, None, 3, None, None, None, None, None,
+                                 _lib.PBICGSTAB, 1e-8, 100, 1, None) == _lib.BAD_ARG
+    assert lib.sfr_solve_shared(None, None, 3, None, None, None, _lib.PBICGSTAB,
+                                1e-8, 100, 1, None) == _lib.BAD_ARG
+    assert lib.sfr_solve_device(None, None, _lib.GMRES, 0, 1e-8, 10, None, None, 1,
+                                None) == _lib.BAD_ARG
