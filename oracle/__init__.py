@@ -202,11 +202,17 @@ def full_rows(system):
 
 
 def pcg_omp(rows, ad, b, tol, maxit, nthreads=0):
+    """OpenMP PCG over full rows (rows from full_rows); n is the row count of
+    rows, and ad and b may be longer (ELLPACK keeps ad[n+1])."""
     rp, cj, cv = rows
-    x = np.zeros(len(ad))
+    n = len(rp) - 1
+    if len(ad) < n or len(b) < n:
+        raise ValueError("ad and b need one entry per row")
+    x = np.zeros(n)
     st = OrcStats()
-    rc = orc().orc_pcg_omp(len(ad), rp, cj, cv,
-                           np.ascontiguousarray(ad[:len(ad)], np.float64),
+    b = np.ascontiguousarray(b[:n], np.float64)
+    rc = orc().orc_pcg_omp(n, rp, cj, cv,
+                           np.ascontiguousarray(ad[:n], np.float64),
                            np.ascontiguousarray(b, np.float64), x, tol, maxit,
                            nthreads, C.byref(st))
     if rc:

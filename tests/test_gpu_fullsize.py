@@ -1,0 +1,76 @@
+"""The benchmark's full-size configurations on the GPU, checked against the
+oracle where it finishes in seconds and otherwise through properties that do
+not depend on size:
+  - 128^3 variable-coefficient Poisson, RCM, device ELL (2,097,152 rows):
+    FP64 PCG against the oracle's OpenMP PCG (the reference's recurrences
+    over full rows): the same iteration count to within 2, solutions to
+    1e-9; mixed precision: the reference's stop test met on the true FP64
+    residual, the solution within 1e-6 of the FP64 one, bitwise repeatable.
+  - 96^3 Taylor-Green momentum, shared CSRD pattern (884,736 rows x 3):
+    the batched u/v/w BiCGSTAB bitwise equal to three single solves and
+    within 1e-6 of the oracle's serial BiCGSTAB, every stop test met.
+"""
+import numpy as np
+import pytest
+
+import oracle
+import sfr3d_amd as S
+from sfr3d_amd import systems
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-8
+
+
+def relerr(x, y):
+    return np.linalg.norm(x - y) / max(np.linalg.norm(y), 1e-300)
+
+
+@pytest.fixture(scope="module")
+def poisson128():
+    return systems.poisson3d(128, seed=1, kmin=1e-2, reorder=True, storage=S.ELLPACK)
+
+
+def test_poisson_128cube_fp64_and_mixed(ctx, poisson128):
+    B = poisson128
+    n = B.system.neq
+    print(f"128^3 Poisson: {n} rows", flush=True)
+    A = ctx.matrix(B.system)
+    db, dx = ctx.array(B.b), ctx.empty(n)
+    st64 = ctx.solve_device(A, S.PCG, TOL, 100000, db, dx)
+    x64 = dx.get()
+    xo, so = oracle.pcg_omp(oracle.full_rows(B.system), B.system.ad[:n], B.b, TOL, 100000)
+    print(f"  fp64 iters {st64['iters']} oracle {so['iters']}", flush=True)
+    assert st64["status"] == S._lib.OK and so["status"] == 0
+    assert abs(int(st64["iters"]) - int(so["iters"])) <= 2
+    assert relerr(x64, xo) < 1e-9
+    assert st64["norm_r_m"] < st64["conv"]
+    stm = ctx.solve_device(A, S.PCG, TOL, 100000, db, dx, precision=S.PREC_MIXED)
+    xm = dx.get()
+    print(f"  mixed iters {stm['iters']} folds {stm['outer']} "
+          f"relerr vs fp64 {relerr(xm, x64):.2e}", flush=True)
+    assert stm["status"] == S._lib.OK
+    assert stm["norm_r_m"] < stm["conv"]             # the reference's own test, FP64
+    assert relerr(xm, x64) < 1e-6
+    assert relerr(xm, xo) < 1e-6
+    stm2 = ctx.solve_device(A, S.PCG, TOL, 100000, db, dx, precision=S.PREC_MIXED)
+    assert stm2["iters"] == stm["iters"] and np.array_equal(dx.get(), xm)
+    A.destroy(); db.free(); dx.free()
+
+
+def test_momentum_96cube_uvw_batch(ctx):
+    M = systems.momentum3d(96)
+    n = M.system.neq
+    bs = M.extra["b_all"]
+    print(f"96^3 momentum: {n} rows x 3", flush=True)
+    xs, sts = ctx.solve_shared(M.system, S.PBICGSTAB, TOL, 5000, list(bs))
+    for c in range(3):
+        x1, s1 = ctx.solve(M.system, S.PBICGSTAB, TOL, 5000, bs[c])
+        xo, so = oracle.pbicgstab(M.system, bs[c], TOL, 5000)
+        print(f"  comp {c}: batched iters {sts[c]['iters']} single {s1['iters']} "
+              f"oracle {so['iters']}", flush=True)
+        assert sts[c]["status"] == S._lib.OK
+        assert sts[c]["iters"] == s1["iters"] and np.array_equal(xs[c], x1)
+        assert abs(int(sts[c]["iters"]) - int(so["iters"])) <= 2
+        assert relerr(xs[c], xo) < 1e-6
+        assert sts[c]["norm_r"] < sts[c]["conv"] * 1.0001 or sts[c]["relres"] < 1e-7
