@@ -9,6 +9,9 @@ not depend on size:
   - 96^3 Taylor-Green momentum, shared CSRD pattern (884,736 rows x 3):
     the batched u/v/w BiCGSTAB bitwise equal to three single solves and
     within 1e-6 of the oracle's serial BiCGSTAB, every stop test met.
+  - the uneven batch of 256 all-Neumann Poisson systems (36.2 M rows),
+    tuned, mixed precision: every system meets its own FP64 stop test, the
+    four smallest agree with the oracle's PCG to 1e-6.
 """
 import numpy as np
 import pytest
@@ -74,3 +77,31 @@ def test_momentum_96cube_uvw_batch(ctx):
         assert abs(int(sts[c]["iters"]) - int(so["iters"])) <= 2
         assert relerr(xs[c], xo) < 1e-6
         assert sts[c]["norm_r"] < sts[c]["conv"] * 1.0001 or sts[c]["relres"] < 1e-7
+
+
+def test_uneven_batch_256_systems(ctx):
+    """The benchmark's uneven batch: 256 all-Neumann Poisson systems with
+    sides log-uniform in [16, 96] (36.2 M rows), tuned, mixed precision.
+    Every system stops on the reference PCG's test applied to its true FP64
+    residual; the smallest ones are checked against the oracle."""
+    built = systems.uneven_batch(256, seed=1)
+    print(f"uneven batch: {sum(b.system.neq for b in built)} rows", flush=True)
+    Bt = ctx.batch([b.system for b in built], tune=True)
+    hb = np.zeros(Bt.rows)
+    for k, b in enumerate(built):
+        r0 = Bt.info(k)["row0"]
+        hb[r0:r0 + b.system.neq] = b.b
+    db, dx = ctx.array(hb), ctx.empty(Bt.rows)
+    sts = Bt.solve_device(db, dx, TOL, 100000)
+    x = dx.get()
+    assert all(st["status"] == S._lib.OK for st in sts)
+    assert all(st["norm_r_m"] < st["conv"] for st in sts)
+    order = np.argsort([b.system.neq for b in built])
+    for k in order[:4]:
+        b = built[k]
+        r0 = Bt.info(int(k))["row0"]
+        xo, _ = oracle.pcg(b.system, b.b, TOL, 100000)
+        assert relerr(x[r0:r0 + b.system.neq], xo) < 1e-6, int(k)
+    print(f"  iterations {min(s['iters'] for s in sts)}..{max(s['iters'] for s in sts)}",
+          flush=True)
+    Bt.destroy(); db.free(); dx.free()
