@@ -192,13 +192,19 @@ static int run_loop_n(sfr_context *ctx, hipGraphExec_t ex, int which,
  * max_iters iterations issued.  Most solves of a SIMPLE step converge in a
  * handful of iterations: starting short bounds the early-exit launches
  * issued past convergence to two short chunks, while a long solve soon
- * runs on full chunks. */
+ * runs on full chunks.  `last` (may be NULL) receives the first system's
+ * scalars as the last chunk seen left them. */
 static int run_loop_ramp(sfr_context *ctx, hipGraphExec_t exs, unsigned cs,
                          hipGraphExec_t exb, unsigned cb, unsigned ramp,
-                         int which, unsigned max_iters, int nsys)
+                         int which, unsigned max_iters, int nsys,
+                         sfr_scalars *last)
 {
   hipStream_t s = ctx->stream;
   unsigned launched = 0, seen = 0, iters = 0;
+  if (last) {
+    memset(last, 0, sizeof *last);
+    last->done = SFR_DONE_RUN;
+  }
   for (;;) {
     while (launched - seen < 2 && iters < max_iters) {
       const int slot = (int)(launched & 1);
@@ -215,6 +221,7 @@ static int run_loop_ramp(sfr_context *ctx, hipGraphExec_t exs, unsigned cs,
     TRY(hipEventSynchronize(ctx->ev_poll[slot]));
     seen++;
     const sfr_scalars *p = ctx->h_poll + SFR_MAXSYS * slot;
+    if (last) *last = p[0];
     int finished = 1;
     for (int c = 0; c < nsys; c++)
       if (!(p[c].done || (which && p[c].ru))) finished = 0;
@@ -240,7 +247,8 @@ static int get_chunks(sfr_context *ctx, int kind, const void *k0, const void *k1
 static int run_chunks(sfr_context *ctx, const chunks *c, int which, unsigned maxit,
                       int nsys)
 {
-  return run_loop_ramp(ctx, c->s, c->cs, c->b, c->cb, c->cb, which, maxit + 2 * c->cb, nsys);
+  return run_loop_ramp(ctx, c->s, c->cs, c->b, c->cb, c->cb, which, maxit + 2 * c->cb, nsys,
+                       NULL);
 }
 
 void sfr_fill_stats(const sfr_scalars *sc, sfr_stats *st, float ms)
@@ -409,12 +417,18 @@ static int solve_pcg_mixed(sfr_context *ctx, sfr_matrix *A, double tol,
   /* every fold is triggered by a residual drop of delta (or the stop test):
    * bound the number of folds generously */
   for (int folds = 0; folds < 100000; folds++) {
-    CHECK(run_chunks(ctx, &ch, 1, maxit, 1));
-    TRY(hipMemcpyAsync(ctx->h_poll, ctx->d_sc, sizeof(sfr_scalars), hipMemcpyDeviceToHost, s));
-    TRY(hipStreamSynchronize(s));
-    if (ctx->h_poll[0].done != SFR_DONE_RUN) break;
-    if (!ctx->h_poll[0].ru) break;        /* loop stopped with no request */
-    const float *p_old = ctx->h_poll[0].pbuf ? w32[4] : w32[2];
+    /* only the first stretch starts on short chunks: one that follows a
+     * fold runs about log(delta) / log(rate) iterations */
+    sfr_scalars last;
+    CHECK(run_loop_ramp(ctx, ch.s, ch.cs, ch.b, ch.cb, folds ? 0 : ch.cb, 1,
+                        maxit + 2 * ch.cb, 1, &last));
+    if (last.done != SFR_DONE_RUN) break;
+    if (!last.ru) break;                  /* loop stopped with no request */
+    /* Once MA or MB requests a fold, every later MA / MB returns at once
+     * and writes nothing but run4 / run5, so the chunk that saw the request
+     * already holds the final pbuf.  The fold kernels queue behind the
+     * chunk still in flight without the host waiting for it. */
+    const float *p_old = last.pbuf ? w32[4] : w32[2];
     TRY(sfr_k_mx_fold(s, &v, g, d_b, d_x, w32[0], w32[1], p_old, w32[4], 0, 0,
                       ctx->d_sc, ctx->part));
   }
