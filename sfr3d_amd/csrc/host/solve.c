@@ -90,7 +90,13 @@ static int get_graph(sfr_context *ctx, int kind, const void *k0,
         e->key[1] == k1 && e->key[2] == k2 &&
         e->key[3] == (const void *)ctx->w64[0] && e->key[4] == (const void *)(intptr_t)g.rpt &&
         e->key[5] == (const void *)ctx->gm_buf) {
-      *exec = e->exec;
+      /* least recently used first: a hit moves to the end, so the graph a
+       * caller has just been handed is never the one a second lookup for
+       * the same solve evicts */
+      const sfr_graph hit = *e;
+      memmove(e, e + 1, sizeof(sfr_graph) * (size_t)(ctx->ngraph - 1 - i));
+      ctx->graphs[ctx->ngraph - 1] = hit;
+      *exec = hit.exec;
       return SFR_OK;
     }
   }

@@ -3,8 +3,8 @@ cases of argv[2] (written by tests/test_emulated_kernels.py with the
 oracle's solutions) in a process of its own: FP64 and mixed PCG, BiCGSTAB,
 BiCGSTAB(2), GMRES, MINRES, the species batch,
 the shared-pattern batch, both adjoint paths, 16-bit column offsets with
-escaped far columns, and a tuned uneven batch.  Prints PASS/FAIL lines and
-exits non-zero on any failure."""
+escaped far columns, a tuned uneven batch and re-tuned rows per thread.
+Prints PASS/FAIL lines and exits non-zero on any failure."""
 import ctypes as C, sys, os, numpy as np
 lib = C.CDLL(sys.argv[1])
 z = np.load(sys.argv[2])
@@ -103,6 +103,22 @@ rc2 = lib.sfr_batch_solve(ctx, Bt, 1e-8, 1000, C.cast(P(b.ctypes.data, b.ctypes.
                           C.cast(P(xs[0].ctypes.data, xs[1].ctypes.data), vp), 1, C.cast(sts, vp))
 check(f"batch rc {rc}/{rc2} rel {rel(xs[1], z['pois_xo']):.1e}", rc == 0 and rc2 == 0 and rel(xs[1], z['pois_xo']) < 1e-6)
 lib.sfr_batch_destroy(ctx, Bt)
+# rows per thread re-tuned between solves (graphs captured again): every
+# solve still matches the oracle
+s, maxit = mk("pois"); A = vp(); assert lib.sfr_matrix_create(ctx, C.byref(s), 0, C.byref(A)) == 0
+lib.sfr_matrix_tune.argtypes = [vp, vp, C.c_int, C.c_int]
+b = np.ascontiguousarray(z["pois_b"]); xo = z["pois_xo"]
+ok, its = True, []
+for rpt in (8, 1, 8):
+    ok &= lib.sfr_matrix_tune(ctx, A, rpt, 4) == 0
+    for solver, prec in ((1, 0), (1, 1)):
+        x = np.zeros(s.neq); st = St()
+        rc = lib.sfr_solve_device(ctx, A, solver, prec, 1e-8, maxit, b.ctypes.data,
+                                  x.ctypes.data, 1, C.byref(st))
+        ok &= rc == 0 and rel(x, xo) < 1e-6
+        its.append(st.iters)
+check(f"retune it {its}", ok)
+lib.sfr_matrix_destroy(ctx, A)
 # device assembly vs host assembly + set_values: bitwise the same solve
 class Pat(C.Structure):
     _fields_ = [("neq", C.c_int32), ("nad", C.c_int32), ("nia", C.c_int32),

@@ -105,4 +105,4 @@ def test_kernels_under_cpu_emulation(tmp_path):
     print(res.stdout)
     assert res.returncode == 0, res.stdout + res.stderr[-3000:]
     assert "SMOKE OK" in res.stdout
-    assert res.stdout.count("PASS") == 14
+    assert res.stdout.count("PASS") == 15

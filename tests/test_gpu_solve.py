@@ -139,6 +139,32 @@ def test_deterministic(ctx):
     assert s1["iters"] == s2["iters"] and np.array_equal(x1, x2)
 
 
+def test_retune_and_graph_cache_churn():
+    """Rows per thread re-tuned and the chunk length changed between solves
+    of several solver ids: many more loop graphs are captured than the
+    context's cache holds (16), so entries are evicted, least recently used
+    first, and captured again.  Every solve matches the oracle."""
+    B = systems.poisson3d(10, seed=4, storage=S.ELLPACK)
+    ref = {S.PCG: oracle.pcg(B.system, B.b, TOL, 5000)[0],
+           S.PBICGSTAB: oracle.pbicgstab(B.system, B.b, TOL, 5000)[0],
+           S.MINRES: oracle.minres(B.system, B.b, TOL, 5000)[0]}
+    cases = ((S.PCG, S.PREC_FP64), (S.PCG, S.PREC_MIXED),
+             (S.PBICGSTAB, S.PREC_FP64), (S.MINRES, S.PREC_FP64))
+    with S.Context(0) as c2:
+        A = c2.matrix(B.system)
+        db, dx = c2.array(B.b), c2.empty(B.system.neq)
+        for chunk in (16, 4, 8, 16):
+            c2.set_option("chunk", chunk)
+            for rpt in (8, 2, 1):
+                print(f"churn chunk={chunk} rpt={rpt}", flush=True)
+                A.tune(rpt)
+                for solver, prec in cases:
+                    st = c2.solve_device(A, solver, TOL, 5000, db, dx, precision=prec)
+                    assert st["status"] == S._lib.OK, (chunk, rpt, solver, prec)
+                    assert relerr(dx.get(), ref[solver]) < XTOL, (chunk, rpt, solver, prec)
+        A.destroy(); db.free(); dx.free()
+
+
 def test_ell_16bit_offsets_with_escaped_far_columns(ctx):
     """SFR3D's ELLPACK points a slot facing a cell without an equation at
     column 0 (src/EllPack.c:54-60), and with the reference's assembly
