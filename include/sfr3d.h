@@ -124,7 +124,12 @@ int  sfr_matrix_set_values(sfr_context *ctx, sfr_matrix *A, const double *al,
                            const double *ad, const double *au);
 void sfr_matrix_destroy(sfr_context *ctx, sfr_matrix *A);
 /* launch shape of the solve kernels: rows per thread (1, 2, 4, 8) and, for
- * the CSR format, lanes per row (1, 2, 4, 8) */
+ * the CSR format, lanes per row (1, 2, 4, 8).  rpt = 0 measures instead:
+ * short FP64 PCG runs at each rows-per-thread choice pick the FP64 PCG
+ * loop's (the other loops keep theirs).  It runs solves on the context's
+ * stream, allocates two temporary n-vectors and takes tens of
+ * milliseconds at 2 M rows.  The choice changes the order of the device
+ * sums, so solutions can differ in the last bits from untuned ones. */
 int  sfr_matrix_tune(sfr_context *ctx, sfr_matrix *A, int rpt, int lanes);
 /* neq, device format, ELL slots, stored nnz of the device format */
 int  sfr_matrix_info(const sfr_matrix *A, int32_t *neq, int *format,

@@ -657,12 +657,14 @@ int sfr_matrix_prepare_mixed(sfr_context *ctx, sfr_matrix *A)
 int sfr_matrix_tune(sfr_context *ctx, sfr_matrix *A, int rpt, int lanes)
 {
   if (!ctx || !A) return SFR_BAD_ARG;
-  if (rpt != 1 && rpt != 2 && rpt != 4 && rpt != 8)
+  if (rpt != 0 && rpt != 1 && rpt != 2 && rpt != 4 && rpt != 8)
     return sfr_set_error(ctx, SFR_BAD_ARG, "rows per thread %d", rpt);
   if (lanes != 1 && lanes != 2 && lanes != 4 && lanes != 8)
     return sfr_set_error(ctx, SFR_BAD_ARG, "lanes per row %d", lanes);
-  A->rpt = rpt;
   if (A->fmt == SFR_FMT_CSR) A->lanes = lanes;
+  if (rpt == 0) return sfr_tune_pcg(ctx, A);
+  A->rpt = rpt;
+  A->rpt_pcg = 0;
   return SFR_OK;
 }
 

@@ -303,7 +303,7 @@ static int solve_pcg64(sfr_context *ctx, sfr_matrix *A, double tol,
 {
   CHECK(sfr_workspace(ctx, A->n));
   sfr_mat_view v = sfr_view(A);
-  sfr_geom g = {A->rpt, 0};
+  sfr_geom g = {A->rpt_pcg ? A->rpt_pcg : A->rpt, 0};
   chunks ch;
   CHECK(get_chunks(ctx, G_PCG, A, d_x, NULL, &v, g, &ch));
   TRY(hipEventRecord(ctx->ev_start, ctx->stream));
@@ -311,6 +311,58 @@ static int solve_pcg64(sfr_context *ctx, sfr_matrix *A, double tol,
                       maxit, ctx->d_sc, ctx->part));
   CHECK(run_chunks(ctx, &ch, 0, maxit, 1));
   return finish(ctx, &v, g, d_b, d_x, st);
+}
+
+/* Rows per thread of the FP64 PCG loop by measurement: 64 iterations of
+ * PCG (tol 0, so the iteration count is fixed) on a fixed pseudo-random
+ * right-hand side at each of 1, 2, 4 and 8 rows per thread, untimed once
+ * (graph capture) and timed twice; the fastest time per iteration is kept.
+ * Which one wins depends on the matrix size and on the caches (the 128^3
+ * Poisson matrix prefers 2, the 96^3 one the default 4). */
+int sfr_tune_pcg(sfr_context *ctx, sfr_matrix *A)
+{
+  const int64_t n = A->n;
+  double *h = (double *)malloc(sizeof(double) * (size_t)(n > 0 ? n : 1));
+  if (!h) return sfr_set_error(ctx, SFR_BAD_ARG, "tune: no host memory");
+  uint64_t s = 0x9e3779b97f4a7c15ull;
+  for (int64_t i = 0; i < n; i++) {
+    s = s * 6364136223846793005ull + 1442695040888963407ull;
+    h[i] = (double)(s >> 11) * 0x1.0p-53 - 0.5;
+  }
+  double *d_b = NULL, *d_x = NULL;
+  int rc = SFR_OK;
+  hipError_t e = hipSetDevice(ctx->device);
+  if (e == hipSuccess) e = hipMalloc((void **)&d_b, sizeof(double) * (size_t)n);
+  if (e == hipSuccess) e = hipMalloc((void **)&d_x, sizeof(double) * (size_t)n);
+  if (e == hipSuccess)
+    e = hipMemcpy(d_b, h, sizeof(double) * (size_t)n, hipMemcpyHostToDevice);
+  free(h);
+  if (e != hipSuccess)
+    rc = sfr_set_error(ctx, SFR_HIP_ERROR, "tune: %s", hipGetErrorString(e));
+  const int keep = A->rpt_pcg;
+  int best = 0;
+  double best_t = 0.0;
+  for (int rpt = 1; rpt <= 8 && rc == SFR_OK; rpt *= 2) {
+    A->rpt_pcg = rpt;
+    double t = 0.0;
+    for (int rep = 0; rep < 3 && rc == SFR_OK; rep++) {
+      sfr_stats st;
+      const int r = solve_pcg64(ctx, A, 0.0, 64, d_b, d_x, 1, &st);
+      if (r != SFR_OK && r != SFR_MAXIT && r != SFR_BREAKDOWN) rc = r;
+      else if (rep > 0 && st.iters > 0) {
+        const double ti = st.time_s / st.iters;
+        if (t == 0.0 || ti < t) t = ti;
+      }
+    }
+    if (rc == SFR_OK && t > 0.0 && (best == 0 || t < best_t)) {
+      best = rpt;
+      best_t = t;
+    }
+  }
+  A->rpt_pcg = (rc == SFR_OK && best) ? best : keep;
+  if (d_b) hipFree(d_b);
+  if (d_x) hipFree(d_x);
+  return rc;
 }
 
 static int solve_bicg64(sfr_context *ctx, sfr_matrix *A, double tol,
@@ -703,7 +755,7 @@ int sfr_kernel_time(sfr_context *ctx, sfr_matrix *A, int which, int reps,
   CHECK(sfr_workspace(ctx, A->n));
   if (which == 1) CHECK(sfr_matrix_prepare_mixed(ctx, A));
   sfr_mat_view v = sfr_view(A);
-  sfr_geom g = {A->rpt, 0};
+  sfr_geom g = {(which == 0 && A->rpt_pcg) ? A->rpt_pcg : A->rpt, 0};
   hipStream_t s = ctx->stream;
   float *const *w32 = ctx->w32;
   if (which == 0) {

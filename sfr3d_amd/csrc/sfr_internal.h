@@ -89,6 +89,7 @@ struct sfr_matrix {
   int      diag_in_al;   /* storage CSR: diagonal inside al                */
   int      have32;       /* FP32 scaled copy is current                    */
   int      rpt;          /* tuned rows per thread                          */
+  int      rpt_pcg;      /* FP64 PCG loop's rows per thread (0: rpt)       */
   int32_t *col, *rowptr, *vmap, *dmap;
   int16_t *col16;        /* ELL: col - row, SFR_D16_ESC for the few that   */
   int64_t *exc_k;        /*   do not fit 16 bits, listed in exc_k/exc_col  */
@@ -178,6 +179,9 @@ int sfr_k_pcg_norm(hipStream_t s, const sfr_mat_view *A, sfr_geom g,
 int sfr_k_final(hipStream_t s, const sfr_mat_view *A, sfr_geom g,
                 const double *b, const double *x, sfr_scalars *sc,
                 double *part);
+
+/* measured rows per thread of the FP64 PCG loop -> A->rpt_pcg */
+int sfr_tune_pcg(sfr_context *ctx, sfr_matrix *A);
 
 /* GMRES(SFR_GM_K), FP64 (reference src/Iters.c:587-790) */
 int sfr_gm_workspace(sfr_context *ctx, int64_t n);

@@ -165,6 +165,37 @@ def test_retune_and_graph_cache_churn():
         A.destroy(); db.free(); dx.free()
 
 
+def test_measured_pcg_tuning(ctx):
+    """sfr_matrix_tune with rpt = 0 times short FP64 PCG runs at 1, 2, 4 and
+    8 rows per thread and keeps the fastest for the FP64 PCG loop; solves
+    afterwards (FP64 and mixed, ELL and CSR) still match the oracle, and an
+    explicit rows-per-thread choice afterwards still applies."""
+    B = systems.poisson3d(24, seed=5, storage=S.CSRD)
+    xo, so = oracle.pcg(B.system, B.b, TOL, 5000)
+    for fmt in (S.FMT_ELL, S.FMT_CSR):
+        print(f"measured tuning fmt={fmt}", flush=True)
+        A = ctx.matrix(B.system, fmt)
+        A.tune(0)
+        db, dx = ctx.array(B.b), ctx.empty(B.system.neq)
+        for prec in (S.PREC_FP64, S.PREC_MIXED):
+            st = ctx.solve_device(A, S.PCG, TOL, 5000, db, dx, precision=prec)
+            assert st["status"] == S._lib.OK, (fmt, prec)
+            assert relerr(dx.get(), xo) < XTOL, (fmt, prec)
+        st = ctx.solve_device(A, S.PCG, TOL, 5000, db, dx)
+        assert abs(int(st["iters"]) - int(so["iters"])) <= 2
+        x_tuned = dx.get()
+        A.tune(8)
+        st8 = ctx.solve_device(A, S.PCG, TOL, 5000, db, dx)
+        assert st8["status"] == S._lib.OK and relerr(dx.get(), x_tuned) < 1e-9
+        A.destroy(); db.free(); dx.free()
+    with pytest.raises(S.SolverError):
+        A = ctx.matrix(B.system)
+        try:
+            A.tune(3)
+        finally:
+            A.destroy()
+
+
 def test_ell_16bit_offsets_with_escaped_far_columns(ctx):
     """SFR3D's ELLPACK points a slot facing a cell without an equation at
     column 0 (src/EllPack.c:54-60), and with the reference's assembly
