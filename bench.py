@@ -59,8 +59,9 @@ def main():
                     help="solver ids of the solver_ids workload (comma list)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is "
-              "not supported (see DESIGN.md for how the solve would shard)",
+        print(f"bench.py: the solver runs on one GPU; --gpus {args.gpus} is "
+              "not supported (a multi-GPU solve would split the rows into "
+              "blocks with a halo exchange per SpMV and all-reduced dots)",
               file=sys.stderr)
         sys.exit(2)
 

@@ -90,3 +90,20 @@ def test_solve_entry_points_refuse_bad_arguments():
                                 1e-8, 100, 1, None) == _lib.BAD_ARG
     assert lib.sfr_solve_device(None, None, _lib.GMRES, 0, 1e-8, 10, None, None, 1,
                                 None) == _lib.BAD_ARG
+
+
+def test_matrix_tune_refuses_bad_arguments():
+    lib = _lib.lib
+    assert lib.sfr_matrix_tune(None, None, 0, 4) == _lib.BAD_ARG
+
+
+def test_bench_refuses_several_gpus():
+    """The benchmark runs on one GPU and says so for --gpus > 1, before it
+    touches a device."""
+    import subprocess
+    import sys
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    res = subprocess.run([sys.executable, os.path.join(root, "bench.py"), "--gpus", "2"],
+                         capture_output=True, text=True, timeout=120, cwd=root)
+    assert res.returncode == 2
+    assert "one GPU" in res.stderr and not res.stdout

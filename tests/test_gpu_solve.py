@@ -2,7 +2,7 @@
 pbicgstab restated in C, itself checked bit-exact against the reference's own
 compiled solverC in test_oracle.py).
 
-Bar (north star): solutions agree with the oracle's to a relative 1e-6 at the
+Bar: solutions agree with the oracle's to a relative 1e-6 at the
 reference tolerance 1e-8; iteration counts agree to within 2 (device dot
 products are summed in a different, fixed order, so the last bits differ).
 """
