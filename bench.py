@@ -51,6 +51,9 @@ def main():
     ap.add_argument("--profile-kernels", action="store_true",
                     help="only launch the two SpMV kernels and a plain copy "
                          "of known size (for rocprofv3 counter runs)")
+    ap.add_argument("--rpt", type=int, default=0, choices=[0, 1, 2, 4, 8],
+                    help="with --profile-kernels: rows per thread of the "
+                         "kernels (0: the library's default)")
     ap.add_argument("--only", choices=["solver_ids", "species", "uvw", "uneven",
                                        "adjoint"],
                     help="run one of the other workloads alone and print its "
@@ -96,6 +99,8 @@ def main():
     dx = ctx.empty(neq)
 
     if args.profile_kernels:
+        if args.rpt:
+            A.tune(args.rpt)
         profile_kernels(ctx, A, lib)
         return
 
