@@ -179,6 +179,8 @@ def main():
                  "relerr_vs_untuned": float(np.linalg.norm(x64t - x64) /
                                             np.linalg.norm(x64)),
                  "tuning_s": round(tune_s, 3),
+                 "rows_per_thread": A.info()["rows_per_thread_pcg"],
+                 "rows_per_thread_default": info["rows_per_thread"],
                  "speedup_mixed_vs_fp64_tuned": round(t64t / tmx, 3)}
 
     copy_gbs = None if args.no_extra else hbm_copy_ceiling(ctx)

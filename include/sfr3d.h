@@ -138,6 +138,9 @@ int  sfr_matrix_info(const sfr_matrix *A, int32_t *neq, int *format,
  * ELL's columns are kept as 16-bit offsets from the row (every |j - i| <
  * 32768, e.g. banded after RCM), else 4 */
 int  sfr_matrix_index_bytes(const sfr_matrix *A);
+/* rows per thread of the solve kernels and of the FP64 PCG loop (the
+ * latter differs after sfr_matrix_tune with rpt = 0) */
+int  sfr_matrix_rows_per_thread(const sfr_matrix *A, int *rpt, int *rpt_pcg);
 
 /* ---- solves ---------------------------------------------------------------
  * sfr_solve: host arrays in and out, the drop-in for solverC's PCG,

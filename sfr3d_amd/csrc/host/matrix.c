@@ -668,6 +668,14 @@ int sfr_matrix_tune(sfr_context *ctx, sfr_matrix *A, int rpt, int lanes)
   return SFR_OK;
 }
 
+int sfr_matrix_rows_per_thread(const sfr_matrix *A, int *rpt, int *rpt_pcg)
+{
+  if (!A) return SFR_BAD_ARG;
+  if (rpt) *rpt = A->rpt;
+  if (rpt_pcg) *rpt_pcg = A->rpt_pcg ? A->rpt_pcg : A->rpt;
+  return SFR_OK;
+}
+
 int sfr_matrix_index_bytes(const sfr_matrix *A)
 {
   if (!A) return 0;

@@ -93,9 +93,12 @@ class DeviceMatrix:
         n, f, s, nnz = C.c_int32(), C.c_int(), C.c_int(), C.c_int64()
         lib.sfr_matrix_info(self.h, C.byref(n), C.byref(f), C.byref(s),
                             C.byref(nnz))
+        rpt, rpt_pcg = C.c_int(), C.c_int()
+        lib.sfr_matrix_rows_per_thread(self.h, C.byref(rpt), C.byref(rpt_pcg))
         return {"neq": n.value, "format": f.value, "slots": s.value,
                 "nnz": nnz.value,
-                "index_bytes": int(lib.sfr_matrix_index_bytes(self.h))}
+                "index_bytes": int(lib.sfr_matrix_index_bytes(self.h)),
+                "rows_per_thread": rpt.value, "rows_per_thread_pcg": rpt_pcg.value}
 
     def destroy(self):
         if self.h:

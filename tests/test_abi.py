@@ -95,6 +95,7 @@ def test_solve_entry_points_refuse_bad_arguments():
 def test_matrix_tune_refuses_bad_arguments():
     lib = _lib.lib
     assert lib.sfr_matrix_tune(None, None, 0, 4) == _lib.BAD_ARG
+    assert lib.sfr_matrix_rows_per_thread(None, None, None) == _lib.BAD_ARG
 
 
 def test_bench_refuses_several_gpus():

@@ -176,6 +176,7 @@ def test_measured_pcg_tuning(ctx):
         print(f"measured tuning fmt={fmt}", flush=True)
         A = ctx.matrix(B.system, fmt)
         A.tune(0)
+        assert A.info()["rows_per_thread_pcg"] in (1, 2, 4, 8)
         db, dx = ctx.array(B.b), ctx.empty(B.system.neq)
         for prec in (S.PREC_FP64, S.PREC_MIXED):
             st = ctx.solve_device(A, S.PCG, TOL, 5000, db, dx, precision=prec)
@@ -185,6 +186,7 @@ def test_measured_pcg_tuning(ctx):
         assert abs(int(st["iters"]) - int(so["iters"])) <= 2
         x_tuned = dx.get()
         A.tune(8)
+        assert A.info()["rows_per_thread"] == A.info()["rows_per_thread_pcg"] == 8
         st8 = ctx.solve_device(A, S.PCG, TOL, 5000, db, dx)
         assert st8["status"] == S._lib.OK and relerr(dx.get(), x_tuned) < 1e-9
         A.destroy(); db.free(); dx.free()
