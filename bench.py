@@ -26,11 +26,11 @@ import numpy as np  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
 TOL = 1e-8
-# HBM bytes per launch of k_mx_a on the 128^3 workload, from two rocprofv3
-# counter runs of `bench.py --profile-kernels` (FETCH_SIZE x 2.000 and
-# WRITE_SIZE x 1.000, the factors a 256 MiB copy gives): 115.2 MB read +
-# 25.2 MB written
-K_MX_A_TRAFFIC_128 = 140.4e6
+# HBM bytes per launch of k_mx_a on the 128^3 workload (2 rows per thread,
+# 1,024 workgroups), from two rocprofv3 counter runs of
+# `bench.py --profile-kernels` (FETCH_SIZE x 2.000 and WRITE_SIZE x 1.000,
+# the factors a 256 MiB copy gives): 105.5 MB read + 25.2 MB written
+K_MX_A_TRAFFIC_128 = 130.7e6
 
 
 def log(*a):

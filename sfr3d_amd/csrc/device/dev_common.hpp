@@ -464,15 +464,17 @@ __host__ inline void with_op32(const sfr_mat_view *A, F &&f)
 }
 
 // Workgroups for n rows at rpt rows per thread and `lanes` lanes per row,
-// a multiple of 8 (XCD remap) and at most SFR_GRIDCAP (kernels grid-stride
-// beyond that), so a consumer never reduces more than SFR_GRIDCAP partials.
-static inline int grid_for(int64_t n, int rpt, int lanes)
+// a multiple of 8 (XCD remap) and at most SFR_GRIDCAP, or `cap` when that
+// is smaller (kernels grid-stride beyond it), so a consumer never reduces
+// more than SFR_GRIDCAP partials.
+static inline int grid_for(int64_t n, int rpt, int lanes, int cap = 0)
 {
   const int64_t rows = (int64_t)rpt * (SFR_BLOCK / lanes);
+  const int64_t c = (cap > 0 && cap < SFR_GRIDCAP) ? ((cap + 7) / 8) * 8 : SFR_GRIDCAP;
   int64_t g = (n + rows - 1) / rows;
   g = ((g + 7) / 8) * 8;
   if (g < 8) g = 8;
-  if (g > SFR_GRIDCAP) g = SFR_GRIDCAP;
+  if (g > c) g = c;
   return (int)g;
 }
 

@@ -342,8 +342,8 @@ extern "C" int sfr_k_mx_fold(hipStream_t s, const sfr_mat_view *A,
                              float *p_new, int init, int zero_x,
                              sfr_scalars *sc, double *part)
 {
-  const sfr_geom gv = {g0.rpt, grid_for(A->n, g0.rpt, 1)};
-  const sfr_geom gm = {g0.rpt, grid_for(A->n, g0.rpt, op_lanes(A))};
+  const sfr_geom gv = {g0.rpt, grid_for(A->n, g0.rpt, 1, g0.cap)};
+  const sfr_geom gm = {g0.rpt, grid_for(A->n, g0.rpt, op_lanes(A), g0.cap)};
   double *rc = REGION(part, 2);
   if (!init) SFR_LAUNCH_V(k_mx_fold, gv, s, A->n, y, A->dscale, x);
   with_op64(A, [&](auto op) {
@@ -376,8 +376,8 @@ extern "C" int sfr_k_mx_inner_iter(hipStream_t s, const sfr_mat_view *A,
                                    float *const pb[2], float *q, int par,
                                    sfr_scalars *sc, double *part)
 {
-  const sfr_geom gv = {g0.rpt, grid_for(A->n, g0.rpt, 1)};
-  const sfr_geom gm = {g0.rpt, grid_for(A->n, g0.rpt, op_lanes(A))};
+  const sfr_geom gv = {g0.rpt, grid_for(A->n, g0.rpt, 1, g0.cap)};
+  const sfr_geom gm = {g0.rpt, grid_for(A->n, g0.rpt, op_lanes(A), g0.cap)};
   double *ra = REGION(part, 0), *rb = REGION(part, 1);
   const float *po = pb[par];
   float *pn = pb[par ^ 1];
@@ -398,8 +398,8 @@ extern "C" int sfr_k_mx_a_only(hipStream_t s, const sfr_mat_view *A,
                                float *pn, float *q, float *y, sfr_scalars *sc,
                                double *part)
 {
-  const sfr_geom gv = {g0.rpt, grid_for(A->n, g0.rpt, 1)};
-  const sfr_geom gm = {g0.rpt, grid_for(A->n, g0.rpt, op_lanes(A))};
+  const sfr_geom gv = {g0.rpt, grid_for(A->n, g0.rpt, 1, g0.cap)};
+  const sfr_geom gm = {g0.rpt, grid_for(A->n, g0.rpt, op_lanes(A), g0.cap)};
   with_op32(A, [&](auto op) {
     using Op = decltype(op);
     SFR_LAUNCH_RPT(k_mx_a, Op, gm, s, op, r, po, pn, q, y,

@@ -665,6 +665,7 @@ int sfr_matrix_tune(sfr_context *ctx, sfr_matrix *A, int rpt, int lanes)
   if (rpt == 0) return sfr_tune_pcg(ctx, A);
   A->rpt = rpt;
   A->rpt_pcg = 0;
+  A->rpt_set = 1;
   return SFR_OK;
 }
 

@@ -77,7 +77,9 @@ typedef struct sfr_mat_view {
                             /*   interleaved, valnb[k*nb + c], or NULL     */
 } sfr_mat_view;
 
-typedef struct sfr_geom { int rpt; int grid; } sfr_geom;
+/* launch geometry: rows per thread, workgroups (derived), and a cap on the
+ * workgroups below SFR_GRIDCAP (0: SFR_GRIDCAP) */
+typedef struct sfr_geom { int rpt; int grid; int cap; } sfr_geom;
 
 struct sfr_matrix {
   int32_t  n;
@@ -90,6 +92,7 @@ struct sfr_matrix {
   int      have32;       /* FP32 scaled copy is current                    */
   int      rpt;          /* tuned rows per thread                          */
   int      rpt_pcg;      /* FP64 PCG loop's rows per thread (0: rpt)       */
+  int      rpt_set;      /* rpt was chosen by the caller (sfr_matrix_tune) */
   int32_t *col, *rowptr, *vmap, *dmap;
   int16_t *col16;        /* ELL: col - row, SFR_D16_ESC for the few that   */
   int64_t *exc_k;        /*   do not fit 16 bits, listed in exc_k/exc_col  */
