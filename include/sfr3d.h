@@ -124,7 +124,9 @@ int  sfr_matrix_set_values(sfr_context *ctx, sfr_matrix *A, const double *al,
                            const double *ad, const double *au);
 void sfr_matrix_destroy(sfr_context *ctx, sfr_matrix *A);
 /* launch shape of the solve kernels: rows per thread (1, 2, 4, 8) and, for
- * the CSR format, lanes per row (1, 2, 4, 8).  rpt = 0 measures instead:
+ * the CSR format, lanes per row (1, 2, 4, 8).  An explicit rpt applies to
+ * every loop; without one, the mixed loop of a matrix of 2^21 rows or more
+ * runs 2 rows per thread on at most 1,024 workgroups.  rpt = 0 measures:
  * short FP64 PCG runs at each rows-per-thread choice pick the FP64 PCG
  * loop's (the other loops keep theirs).  It runs solves on the context's
  * stream, allocates two temporary n-vectors and takes tens of
