@@ -144,6 +144,19 @@ int  sfr_matrix_index_bytes(const sfr_matrix *A);
  * latter differs after sfr_matrix_tune with rpt = 0) */
 int  sfr_matrix_rows_per_thread(const sfr_matrix *A, int *rpt, int *rpt_pcg);
 
+/* The host-pointer entry points (sfr_solve, sfr_solverC, sfr_solve_shared,
+ * sfr_solve_batched) keep up to 4 device matrices per context, one per
+ * pattern, recognised by the identity of ia/ja with neq, nad, storage and
+ * unsym; a later call on a cached pattern uploads values only, and the least
+ * recently used pattern is dropped for a fifth.  SFR3D alternates at least
+ * the velocity and the pressure patterns every SIMPLE step
+ * (src/Simple.c:2983-3136).  A caller that frees a pattern and builds
+ * another at the same addresses must call sfr_matrix_cache_clear first.
+ * sfr_cache_info: matrices held, matrices created and lookups that hit. */
+int  sfr_cache_info(const sfr_context *ctx, int *entries, int64_t *created,
+                    int64_t *hits);
+void sfr_matrix_cache_clear(sfr_context *ctx);
+
 /* ---- solves ---------------------------------------------------------------
  * sfr_solve: host arrays in and out, the drop-in for solverC's PCG,
  * PBICGSTAB, PBICGSTABL2, GMRES and MINRES branches (reference
@@ -299,7 +312,12 @@ int sfr_set_option(sfr_context *ctx, const char *name, double value);
 
 /* Drop-in twin of solverC (reference include/Solv.h:59-70), same prototype
  * with the reference's opaque Memoria/Interface/BufferOmp pointers passed
- * through untouched.  Non-convergence and bad ids end the process with the
+ * through untouched.  Storage CSR is refused (exit with a message): the
+ * reference's solverC does not solve a CSR system as stored (ad aliases al,
+ * src/Datastruct.c:85; matVecCsrSym drops the lower triangle,
+ * src/HccaBlas.c:3315-3325), so no faithful answer exists; sfr_solve takes
+ * CSR and solves the stored operator.  Non-convergence and bad ids end the
+ * process with the
  * reference's messages and exit codes (src/Cg.c:195-200,
  * src/Bicgstab.c:225-231, 1362-1366, src/Iters.c:770-774, src/Solv.c:496;
  * MINRES only warns, src/Iters.c:540-544); with fLog one line in the
@@ -311,6 +329,9 @@ void sfr_solverC(void *m, int nEq, int nEqNov, int nAd, int nAdR,
                  double tol, unsigned int maxIt, short storage, short solver,
                  FILE *fSolvLog, bool fLog, bool newX, bool unSym,
                  const char *name);
+/* The context sfr_solverC solves on (device $SFR_DEVICE, default 0),
+ * created on first use; for options and sfr_cache_info. */
+int sfr_solverC_context(sfr_context **ctx);
 
 #ifdef __cplusplus
 }

@@ -138,6 +138,10 @@ _sig("sfr_batch_destroy", None, vp, vp)
 _sig("sfr_solverC", None, vp, C.c_int, C.c_int, C.c_int, C.c_int, vp, vp, vp,
      vp, vp, vp, vp, vp, vp, C.c_double, C.c_uint, C.c_short, C.c_short, vp,
      C.c_bool, C.c_bool, C.c_bool, C.c_char_p)
+_sig("sfr_solverC_context", C.c_int, C.POINTER(vp))
+_sig("sfr_cache_info", C.c_int, vp, C.POINTER(C.c_int), C.POINTER(C.c_int64),
+     C.POINTER(C.c_int64))
+_sig("sfr_matrix_cache_clear", None, vp)
 # host set-up
 _sig("sfr_mesh_quad_adj", C.c_int, C.c_int, C.c_int, i32p, i16p)
 _sig("sfr_mesh_hex_adj", C.c_int, C.c_int, C.c_int, C.c_int, i32p, i16p)

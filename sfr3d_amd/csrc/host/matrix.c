@@ -97,11 +97,28 @@ fail:
   return rc;
 }
 
+/* Loop graphs are destroyed only with the context stream drained: a graph
+ * must not go while a launch of it may still be in flight. */
 static void drop_graphs(sfr_context *ctx)
 {
+  if (ctx->ngraph && ctx->stream) hipStreamSynchronize(ctx->stream);
   for (int i = 0; i < ctx->ngraph; i++)
     if (ctx->graphs[i].exec) hipGraphExecDestroy(ctx->graphs[i].exec);
   ctx->ngraph = 0;
+}
+
+void sfr_drop_graphs_keyed(sfr_context *ctx, int slot, const void *key)
+{
+  int w = 0, synced = 0;
+  for (int i = 0; i < ctx->ngraph; i++) {
+    if (ctx->graphs[i].key[slot] == key) {
+      if (!synced) { hipStreamSynchronize(ctx->stream); synced = 1; }
+      hipGraphExecDestroy(ctx->graphs[i].exec);
+    } else {
+      ctx->graphs[w++] = ctx->graphs[i];
+    }
+  }
+  ctx->ngraph = w;
 }
 
 void sfr_finalize(sfr_context *ctx)
@@ -110,7 +127,7 @@ void sfr_finalize(sfr_context *ctx)
   hipSetDevice(ctx->device);
   if (ctx->stream) hipStreamSynchronize(ctx->stream);
   drop_graphs(ctx);
-  if (ctx->cached) sfr_matrix_destroy(ctx, ctx->cached);
+  sfr_mcache_clear(ctx);
   for (int i = 0; i < 10; i++) hipFree(ctx->w64[i]);
   for (int i = 0; i < 6; i++) hipFree(ctx->w32[i]);
   hipFree(ctx->stage);
@@ -334,14 +351,10 @@ void sfr_matrix_destroy(sfr_context *ctx, sfr_matrix *A)
   if (!A) return;
   if (ctx) {
     hipStreamSynchronize(ctx->stream);
-    if (ctx->cached == A) ctx->cached = NULL;
+    for (int i = 0; i < SFR_MCACHE; i++)
+      if (ctx->mcache[i].A == A) ctx->mcache[i].A = NULL;
     /* graphs captured on this matrix are stale */
-    int w = 0;
-    for (int i = 0; i < ctx->ngraph; i++) {
-      if (ctx->graphs[i].key[0] == (const void *)A) hipGraphExecDestroy(ctx->graphs[i].exec);
-      else ctx->graphs[w++] = ctx->graphs[i];
-    }
-    ctx->ngraph = w;
+    sfr_drop_graphs_keyed(ctx, 0, A);
   }
   if (A->T) sfr_matrix_destroy(ctx, A->T);
   hipFree(A->col); hipFree(A->col16); hipFree(A->exc_k); hipFree(A->exc_col);
@@ -661,7 +674,12 @@ int sfr_matrix_tune(sfr_context *ctx, sfr_matrix *A, int rpt, int lanes)
     return sfr_set_error(ctx, SFR_BAD_ARG, "rows per thread %d", rpt);
   if (lanes != 1 && lanes != 2 && lanes != 4 && lanes != 8)
     return sfr_set_error(ctx, SFR_BAD_ARG, "lanes per row %d", lanes);
-  if (A->fmt == SFR_FMT_CSR) A->lanes = lanes;
+  if (A->fmt == SFR_FMT_CSR && A->lanes != lanes) {
+    A->lanes = lanes;
+    /* the loop graphs of A were captured with the old lanes per row (the
+     * graph key holds them too; this frees the stale ones at once) */
+    sfr_drop_graphs_keyed(ctx, 0, A);
+  }
   if (rpt == 0) return sfr_tune_pcg(ctx, A);
   A->rpt = rpt;
   A->rpt_pcg = 0;

@@ -89,7 +89,8 @@ static int get_graph(sfr_context *ctx, int kind, const void *k0,
     if (e->kind == kind && e->chunk == chunk && e->key[0] == k0 &&
         e->key[1] == k1 && e->key[2] == k2 &&
         e->key[3] == (const void *)ctx->w64[0] && e->key[4] == (const void *)(intptr_t)g.rpt &&
-        e->key[5] == (const void *)ctx->gm_buf && e->key[6] == (const void *)(intptr_t)g.cap) {
+        e->key[5] == (const void *)ctx->gm_buf && e->key[6] == (const void *)(intptr_t)g.cap &&
+        e->key[7] == (const void *)(intptr_t)v->lanes) {
       /* least recently used first: a hit moves to the end, so the graph a
        * caller has just been handed is never the one a second lookup for
        * the same solve evicts */
@@ -101,6 +102,8 @@ static int get_graph(sfr_context *ctx, int kind, const void *k0,
     }
   }
   if (ctx->ngraph == (int)(sizeof ctx->graphs / sizeof ctx->graphs[0])) {
+    /* a graph is destroyed only with no launch of it in flight */
+    TRY(hipStreamSynchronize(ctx->stream));
     hipGraphExecDestroy(ctx->graphs[0].exec);
     memmove(&ctx->graphs[0], &ctx->graphs[1], sizeof(sfr_graph) * (size_t)(ctx->ngraph - 1));
     ctx->ngraph--;
@@ -158,6 +161,7 @@ static int get_graph(sfr_context *ctx, int kind, const void *k0,
   e->key[4] = (const void *)(intptr_t)g.rpt;
   e->key[5] = (const void *)ctx->gm_buf;
   e->key[6] = (const void *)(intptr_t)g.cap;
+  e->key[7] = (const void *)(intptr_t)v->lanes;
   *exec = ex;
   return SFR_OK;
 }
@@ -361,6 +365,9 @@ int sfr_tune_pcg(sfr_context *ctx, sfr_matrix *A)
     }
   }
   A->rpt_pcg = (rc == SFR_OK && best) ? best : keep;
+  /* the trial graphs hold the temporary x: drop them before it goes, so
+   * they neither occupy the caller's cache slots nor outlive their buffer */
+  if (d_x) sfr_drop_graphs_keyed(ctx, 1, d_x);
   if (d_b) hipFree(d_b);
   if (d_x) hipFree(d_x);
   return rc;
@@ -511,6 +518,69 @@ static int solve_pcg_mixed(sfr_context *ctx, sfr_matrix *A, double tol,
   return finish(ctx, &v, g, d_b, d_x, st);
 }
 
+/* The device matrix of the host-pointer paths for sys's pattern (the
+ * identity of ia/ja plus the sizes and storage), from the context's small
+ * LRU set.  A hit uploads the values al/ad/au (upload = 1) or leaves them
+ * (upload = 0, the caller brings its own); a miss creates the matrix from
+ * them, evicting the least recently used entry. */
+static int cached_matrix(sfr_context *ctx, const sfr_system *sys, const double *al,
+                         const double *ad, const double *au, int upload,
+                         sfr_matrix **out)
+{
+  const uint64_t tick = ++ctx->mtick;
+  for (int i = 0; i < SFR_MCACHE; i++) {
+    sfr_matrix *A = ctx->mcache[i].A;
+    if (A && A->h_ia == sys->ia && A->h_ja == sys->ja && A->n == sys->neq &&
+        A->nad == sys->nad && A->storage == sys->storage && A->unsym == sys->unsym) {
+      ctx->mcache[i].used = tick;
+      ctx->mhits++;
+      if (upload) CHECK(sfr_matrix_set_values(ctx, A, al, ad, au));
+      *out = A;
+      return SFR_OK;
+    }
+  }
+  int slot = 0;
+  for (int i = 0; i < SFR_MCACHE; i++) {
+    if (!ctx->mcache[i].A) { slot = i; break; }
+    if (ctx->mcache[i].used < ctx->mcache[slot].used) slot = i;
+  }
+  if (ctx->mcache[slot].A) sfr_matrix_destroy(ctx, ctx->mcache[slot].A);
+  sfr_system s0 = *sys;
+  s0.al = al;
+  s0.ad = ad;
+  s0.au = au;
+  sfr_matrix *A = NULL;
+  CHECK(sfr_matrix_create(ctx, &s0, SFR_FMT_AUTO, &A));
+  ctx->mcache[slot].A = A;
+  ctx->mcache[slot].used = tick;
+  ctx->mcreated++;
+  *out = A;
+  return SFR_OK;
+}
+
+void sfr_mcache_clear(sfr_context *ctx)
+{
+  for (int i = 0; i < SFR_MCACHE; i++)
+    if (ctx->mcache[i].A) sfr_matrix_destroy(ctx, ctx->mcache[i].A);
+}
+
+void sfr_matrix_cache_clear(sfr_context *ctx)
+{
+  if (ctx) sfr_mcache_clear(ctx);
+}
+
+int sfr_cache_info(const sfr_context *ctx, int *entries, int64_t *created,
+                   int64_t *hits)
+{
+  if (!ctx) return SFR_BAD_ARG;
+  int e = 0;
+  for (int i = 0; i < SFR_MCACHE; i++) e += ctx->mcache[i].A != NULL;
+  if (entries) *entries = e;
+  if (created) *created = ctx->mcreated;
+  if (hits) *hits = ctx->mhits;
+  return SFR_OK;
+}
+
 /* Shared-pattern batch: interleave the system-major inputs, run the batched
  * BiCGSTAB, de-interleave x.  Workspace (interleaved, nsys*n each):
  * w0 x, w1 r, w2 r0, w3 p, w4 z, w5 v, w6 t, w7 b, w8 ad, w9 dinv. */
@@ -617,20 +687,10 @@ int sfr_solve_batched(sfr_context *ctx, const sfr_system *sys, int nsys,
     }
     return worst;
   }
-  sfr_matrix *A = ctx->cached;
-  if (A && (A->h_ia != sys->ia || A->h_ja != sys->ja || A->n != sys->neq ||
-            A->nad != sys->nad || A->storage != sys->storage || A->unsym != sys->unsym)) {
-    sfr_matrix_destroy(ctx, A);
-    A = NULL;
-  }
-  if (!A) {
-    sfr_system s0 = *sys;
-    s0.al = al[0];
-    s0.ad = ad[0];
-    s0.au = au ? au[0] : NULL;
-    CHECK(sfr_matrix_create(ctx, &s0, SFR_FMT_AUTO, &A));
-    ctx->cached = A;
-  }
+  /* each group uploads its own values below: a cached pattern is taken as
+   * it is */
+  sfr_matrix *A = NULL;
+  CHECK(cached_matrix(ctx, sys, al[0], ad[0], au ? au[0] : NULL, 0, &A));
   CHECK(sfr_nb_buffers(ctx, A->nnz, A->nraw));
   const int64_t n = A->n, tot = 4 * n;
   if (ctx->stage_n < 3 * tot) {
@@ -686,20 +746,8 @@ int sfr_solve_shared(sfr_context *ctx, const sfr_system *sys, int nsys,
 {
   if (!ctx || !sys || !b || !x || nsys < 1 || nsys > 4) return SFR_BAD_ARG;
   TRY(hipSetDevice(ctx->device));
-  sfr_matrix *A = ctx->cached;
-  if (A && (A->h_ia != sys->ia || A->h_ja != sys->ja || A->n != sys->neq ||
-            A->nad != sys->nad || A->storage != sys->storage || A->unsym != sys->unsym)) {
-    sfr_matrix_destroy(ctx, A);
-    A = NULL;
-  }
-  if (A) {
-    CHECK(sfr_matrix_set_values(ctx, A, sys->al, ad ? ad[0] : sys->ad, sys->au));
-  } else {
-    sfr_system s0 = *sys;
-    if (ad) s0.ad = ad[0];
-    CHECK(sfr_matrix_create(ctx, &s0, SFR_FMT_AUTO, &A));
-    ctx->cached = A;
-  }
+  sfr_matrix *A = NULL;
+  CHECK(cached_matrix(ctx, sys, sys->al, ad ? ad[0] : sys->ad, sys->au, 1, &A));
   const int64_t n = A->n, tot = n * nsys;
   if (ctx->stage_n < 3 * tot) {
     TRY(hipStreamSynchronize(ctx->stream));
@@ -838,27 +886,17 @@ int sfr_solve_adjoint(sfr_context *ctx, sfr_matrix *A, int solver, double tol,
   return rc;
 }
 
-/* Host-pointer solve: the pattern is uploaded once and recognised again by
- * the identity of ia/ja (SFR3D re-assembles values on a fixed pattern). */
+/* Host-pointer solve: each pattern is uploaded once and recognised again by
+ * the identity of ia/ja (SFR3D re-assembles values on fixed patterns and
+ * alternates several of them per SIMPLE step). */
 int sfr_solve(sfr_context *ctx, const sfr_system *sys, int solver, double tol,
               unsigned maxit, const double *b, double *x, int newX,
               sfr_stats *st)
 {
   if (!ctx || !sys || !b || !x) return SFR_BAD_ARG;
   TRY(hipSetDevice(ctx->device));
-  sfr_matrix *A = ctx->cached;
-  if (A && (A->h_ia != sys->ia || A->h_ja != sys->ja || A->n != sys->neq ||
-            A->nad != sys->nad || A->storage != sys->storage ||
-            A->unsym != sys->unsym)) {
-    sfr_matrix_destroy(ctx, A);
-    A = NULL;
-  }
-  if (A) {
-    CHECK(sfr_matrix_set_values(ctx, A, sys->al, sys->ad, sys->au));
-  } else {
-    CHECK(sfr_matrix_create(ctx, sys, SFR_FMT_AUTO, &A));
-    ctx->cached = A;
-  }
+  sfr_matrix *A = NULL;
+  CHECK(cached_matrix(ctx, sys, sys->al, sys->ad, sys->au, 1, &A));
   CHECK(sfr_workspace(ctx, A->n));
   double *d_b = ctx->w64[8], *d_x = ctx->w64[9];
   const size_t bytes = sizeof(double) * (size_t)A->n;
@@ -875,6 +913,21 @@ int sfr_solve(sfr_context *ctx, const sfr_system *sys, int solver, double tol,
 /* ---- solverC drop-in -------------------------------------------------------- */
 
 static sfr_context *g_ctx;
+
+int sfr_solverC_context(sfr_context **ctx)
+{
+  if (!ctx) return SFR_BAD_ARG;
+  if (!g_ctx) {
+    const char *dev = getenv("SFR_DEVICE");
+    const int rc = sfr_init(&g_ctx, dev ? atoi(dev) : 0);
+    if (rc != SFR_OK) {
+      *ctx = NULL;
+      return rc;
+    }
+  }
+  *ctx = g_ctx;
+  return SFR_OK;
+}
 
 void sfr_solverC(void *m, int nEq, int nEqNov, int nAd, int nAdR, int *ia,
                  int *ja, double *al, double *ad, double *au, double *b,
@@ -900,9 +953,23 @@ void sfr_solverC(void *m, int nEq, int nEqNov, int nAd, int nAdR, int *ia,
             " are not supported\n", nAdR, nEqNov, nEq);
     exit(EXIT_FAILURE);
   }
-  if (!g_ctx) {
-    const char *dev = getenv("SFR_DEVICE");
-    const int rc = sfr_init(&g_ctx, dev ? atoi(dev) : 0);
+  if (storage == SFR_CSR) {
+    /* The reference's storage CSR does not solve the assembled system:
+     * dataStruct aliases ad to al (src/Datastruct.c:85), so Jacobi is built
+     * from the first neq values of al, and matVecCsrSym zeroes y[i] after
+     * earlier rows have scattered into it (src/HccaBlas.c:3315-3325), which
+     * drops the lower triangle.  This build computes the operator the
+     * storage represents (sfr_solve accepts it), so its answer would differ
+     * from solverC's: the drop-in refuses rather than return it silently. */
+    fprintf(stderr, "sfr_solverC: storage CSR (%d) is refused: the reference's solverC "
+            "builds Jacobi from al (ad aliases al) and its symmetric product drops the "
+            "lower triangle; use CSRD, CSRC or ELLPACK, or sfr_solve for the exact "
+            "operator\n", storage);
+    exit(EXIT_FAILURE);
+  }
+  {
+    sfr_context *c = NULL;
+    const int rc = sfr_solverC_context(&c);
     if (rc != SFR_OK) {
       fprintf(stderr, "sfr_solverC: %s\n", sfr_status_string(rc));
       exit(EXIT_FAILURE);

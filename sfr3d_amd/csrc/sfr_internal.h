@@ -114,6 +114,18 @@ typedef struct sfr_graph {
   int            kind, chunk;
 } sfr_graph;
 
+/* Host-pointer paths (sfr_solve, sfr_solverC, sfr_solve_shared,
+ * sfr_solve_batched): device matrices kept per pattern, recognised by the
+ * identity of ia/ja and the sizes, least recently used evicted.  SFR3D
+ * alternates at least two patterns per SIMPLE step (velocity and pressure,
+ * plus energy / turbulence / species), each re-assembled on a fixed
+ * structure. */
+#define SFR_MCACHE 4
+typedef struct sfr_mcache {
+  sfr_matrix *A;
+  uint64_t    used;             /* tick of the last lookup that hit       */
+} sfr_mcache;
+
 struct sfr_context {
   int          device, ncu;
   hipStream_t  stream;
@@ -129,7 +141,9 @@ struct sfr_context {
   float       *w32[6];
   sfr_graph    graphs[16];
   int          ngraph;
-  sfr_matrix  *cached;          /* pattern cache of the host-pointer path  */
+  sfr_mcache   mcache[SFR_MCACHE]; /* patterns of the host-pointer paths */
+  uint64_t     mtick;             /* lookups so far (LRU clock)          */
+  int64_t      mcreated, mhits;   /* matrices created / lookups that hit */
   int          chunk;           /* iterations per captured chunk           */
   double      *gm_buf;          /* GMRES: basis, raw vectors, state        */
   int64_t      gm_n, gm_ld;
@@ -140,6 +154,10 @@ struct sfr_context {
 };
 
 int sfr_set_error(sfr_context *ctx, int code, const char *fmt, ...);
+/* destroy every cached host-path matrix (context teardown) */
+void sfr_mcache_clear(sfr_context *ctx);
+/* destroy the loop graphs whose key slot `slot` equals `key` */
+void sfr_drop_graphs_keyed(sfr_context *ctx, int slot, const void *key);
 /* solve statistics from a scalar block (ms = device time of the solve) */
 void sfr_fill_stats(const sfr_scalars *sc, sfr_stats *st, float ms);
 int  sfr_matrix_prepare_mixed(sfr_context *ctx, sfr_matrix *A);

@@ -216,6 +216,13 @@ class Context:
             lib.sfr_finalize(self.h)
             self.h = None
 
+    def cache_info(self):
+        """sfr_cache_info: (matrices held, matrices created, lookups that hit)
+        of the host-pointer paths' pattern cache."""
+        e, c, h = C.c_int(), C.c_int64(), C.c_int64()
+        self._check(lib.sfr_cache_info(self.h, C.byref(e), C.byref(c), C.byref(h)))
+        return e.value, c.value, h.value
+
     def set_option(self, name, value):
         """sfr_set_option: "chunk" (iterations per captured graph chunk) or
         "ru_delta" (mixed-precision reliable-update threshold)."""

@@ -85,24 +85,84 @@ def write_cases(path):
     np.savez(path, **d)
 
 
-@pytest.mark.timeout(900)
-def test_kernels_under_cpu_emulation(tmp_path):
+ASAN = os.environ.get("SFR_EMU_ASAN") == "1"
+
+
+@pytest.fixture(scope="module")
+def emu_lib():
     if not os.path.exists("/opt/rocm/include/hip/hip_runtime_api.h"):
         pytest.skip("ROCm headers absent")
     import build_emu
-    lib = build_emu.build(asan=os.environ.get("SFR_EMU_ASAN") == "1")
-    cases = str(tmp_path / "cases.npz")
-    write_cases(cases)
+    return build_emu.build(asan=ASAN)
+
+
+def _env():
     env = dict(os.environ)
-    if os.environ.get("SFR_EMU_ASAN") == "1":
+    if ASAN:
         asan = subprocess.run(["gcc", "-print-file-name=libasan.so"],
                               capture_output=True, text=True).stdout.strip()
         env["LD_PRELOAD"] = asan
         env["ASAN_OPTIONS"] = "detect_leaks=0"
+    return env
+
+
+@pytest.mark.timeout(900)
+def test_kernels_under_cpu_emulation(emu_lib, tmp_path):
+    cases = str(tmp_path / "cases.npz")
+    write_cases(cases)
     res = subprocess.run([sys.executable, os.path.join(HERE, "emu", "run_smoke.py"),
-                          lib, cases], capture_output=True, text=True, env=env,
+                          emu_lib, cases], capture_output=True, text=True, env=_env(),
                          timeout=850)
     print(res.stdout)
     assert res.returncode == 0, res.stdout + res.stderr[-3000:]
     assert "SMOKE OK" in res.stdout
     assert res.stdout.count("PASS") == 15
+
+
+@pytest.fixture(scope="module")
+def dropin_cases(tmp_path_factory):
+    from tests import dropin_cases as D
+    path = str(tmp_path_factory.mktemp("dropin") / "dropin.npz")
+    D.write(path)
+    return path
+
+
+def _dropin(lib, cases, mode):
+    return subprocess.run([sys.executable, os.path.join(HERE, "emu", "run_dropin.py"),
+                           lib, cases, mode], capture_output=True, text=True,
+                          env=_env(), timeout=600)
+
+
+@pytest.mark.timeout(700)
+def test_dropin_alternating_patterns_emulated(emu_lib, dropin_cases):
+    """sfr_solverC as SFR3D's SIMPLE step calls it: velocity, pressure and a
+    third pattern in turn with new values each round, then two more
+    patterns: every x and every field of every log line against the
+    oracle, and the pattern cache against an LRU model (no pattern is
+    uploaded twice while it stays among the four most recent)."""
+    res = _dropin(emu_lib, dropin_cases, "alternate")
+    print(res.stdout)
+    assert res.returncode == 0, res.stdout + res.stderr[-3000:]
+    assert "DROPIN OK" in res.stdout and "FAIL" not in res.stdout
+    assert res.stdout.count("PASS") == 26
+
+
+@pytest.mark.timeout(300)
+@pytest.mark.parametrize("mode,code,out,err", [
+    ("maxit_pcg", 1, "name : sistEqTest\nPCG: *** WARNING: no convergende reached !\n"
+                     "MAXIT = 3 \n", ""),
+    ("maxit_bicg", 254, "name : sistEqTest\n PBICGSTAB *** WARNING: no convergende "
+                        "reached !\nMAXIT = 3 \n", ""),
+    ("csr", 1, "", "storage CSR (1) is refused"),
+    ("badid", 1, "", "Opcao 9 e invalida!!"),
+])
+def test_dropin_exit_paths_emulated(emu_lib, dropin_cases, mode, code, out, err):
+    """The drop-in ends the process as the reference's solverC does: at maxIt
+    PCG prints its warning and exits with EXIT_FAILURE (src/Cg.c:195-200),
+    BiCGSTAB with EXIT_SOLVER = -2 (src/Bicgstab.c:225-231); a bad solver id
+    exits as ERRO_OP does (src/Solv.c:496); storage CSR is refused."""
+    res = _dropin(emu_lib, dropin_cases, mode)
+    assert res.returncode == code, res.stdout + res.stderr[-2000:]
+    assert "NOT REACHED" not in res.stdout
+    assert res.stdout == out
+    assert err in res.stderr
