@@ -19,6 +19,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 ORACLE_LIB = os.path.join(HERE, "liboracle.so")
 REF_LIB = os.path.join(HERE, "_ref", "libsfr_ref.so")
+REF_OMP_LIB = os.path.join(HERE, "_ref", "libsfr_ref_omp.so")
 
 i32p = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
 i64p = np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS")
@@ -106,6 +107,50 @@ def ref():
                                    f64p, f64p, f64p, f64p]
         _ref = lib
     return _ref
+
+
+_ref_omp = None
+
+
+def ref_omp_available():
+    return os.path.exists(REF_OMP_LIB)
+
+
+def ref_omp():
+    """The reference's own C built with -fopenmp (its OpenMP solvers)."""
+    global _ref_omp
+    if _ref_omp is None:
+        if not ref_omp_available():
+            raise ImportError(f"{REF_OMP_LIB} missing: run `make -C oracle ref`")
+        lib = C.CDLL(REF_OMP_LIB)
+        lib.ref_init.argtypes = [C.c_long]
+        lib.ref_solve_omp.argtypes = [C.c_int, C.c_int, i32p, i32p, f64p, f64p,
+                                      f64p, f64p, C.c_double, C.c_uint,
+                                      C.c_short, C.c_short, C.c_int, C.c_int,
+                                      f64p, C.c_char_p, C.c_int]
+        lib.ref_init(1 << 30)
+        _ref_omp = lib
+    return _ref_omp
+
+
+def ref_solve_omp(system, b, tol, maxit, solver, nthreads):
+    """The reference's solverC with its OpenMP solvers (pcgOmp for solver 1,
+    pbicgstabOmp for 2) on nthreads threads.  Returns (x, stats) with the
+    iterations, xKx, ||x|| and time the reference logs."""
+    lib = ref_omp()
+    x = np.zeros(system.neq)
+    out = np.zeros(4)
+    buf = C.create_string_buffer(512)
+    ad = system.ad if system.ad is not None else system.al
+    rc = lib.ref_solve_omp(system.neq, system.nad, system.ia, system.ja,
+                           _al_contig(system), np.ascontiguousarray(ad, np.float64),
+                           np.ascontiguousarray(b, np.float64), x, tol, maxit,
+                           system.storage, solver, 1 if system.unsym else 0,
+                           nthreads, out, buf, 512)
+    if rc:
+        raise RuntimeError(f"ref_solve_omp = {rc}")
+    return x, {"iters": int(out[0]), "xKx": out[1], "norm_x": out[2],
+               "time_s": out[3], "line": buf.value.decode()}
 
 
 # ---- convenience wrappers ---------------------------------------------------

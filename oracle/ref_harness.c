@@ -7,7 +7,8 @@
  * functions unchanged:
  *   reord          src/Reord.c:18          numEqV1/V2  src/Numeq.c:70,109
  *   dataStruct     src/Datastruct.c:28     assbly      src/Assbly.c:43
- *   solverC        src/Solv.c:64 (PCG / PBICGSTAB, serial, log line captured)
+ *   solverC        src/Solv.c:64 (PCG / PBICGSTAB, serial, log line captured;
+ *                  with -fopenmp also its OpenMP solvers, ref_solve_omp)
  *   genrcm         src/Rcm.c:56            matvecs     src/HccaBlas.c
  * Used only to pin the oracle and to generate golden vectors.
  */
@@ -188,6 +189,69 @@ int ref_solve(int neq, int nad, int *ia, int *ja, double *al, double *ad,
   out[0] = it; out[1] = xkx; out[2] = nx; out[3] = tm_;
   return 0;
 }
+
+#ifdef _OPENMP
+#include <omp.h>
+#include <Csr.h>
+/* solverC with the reference's OpenMP solvers (pcgOmp, src/Cg.c:490-715;
+ * pbicgstabOmp, src/Bicgstab.c:554-803) on `nthreads` threads: the process
+ * globals set as the reference's openMpSet and ReadFile do for the solver
+ * (ompVar.fSolver, nThreadsSolver, src/ReadFile.c:5570-5600), the rows
+ * split among the threads by the reference's partitionCsrByNonzeros
+ * (src/Csr.c:1827, called from pMatrixSolverOmp), and the scatter buffer
+ * thY of nthreads * neq doubles.  out and line as ref_solve. */
+int ref_solve_omp(int neq, int nad, int *ia, int *ja, double *al, double *ad,
+                  double *b, double *x, double tol, unsigned maxit,
+                  short storage, short solver, int unsym, int nthreads,
+                  double *out, char *line, int linelen)
+{
+  /* computeEffectiveWork (src/Csr.c) opens a parallel region of the default
+   * size and writes one entry per thread: the default is set to nthreads
+   * and the arrays sized for whichever is larger */
+  const int nmax_th = omp_get_max_threads() > nthreads ? omp_get_max_threads() : nthreads;
+  omp_set_num_threads(nthreads);
+  BufferOmp bo;
+  bo.thBegin = calloc((size_t)nmax_th, sizeof(INT));
+  bo.thEnd = calloc((size_t)nmax_th, sizeof(INT));
+  bo.thSize = calloc((size_t)nmax_th, sizeof(INT));
+  bo.thHeight = calloc((size_t)nmax_th, sizeof(INT));
+  bo.thY = calloc((size_t)nthreads * (size_t)neq, sizeof(DOUBLE));
+  FILE *f = tmpfile();
+  int rc = 0;
+  if (!bo.thBegin || !bo.thEnd || !bo.thSize || !bo.thHeight || !bo.thY || !f) {
+    rc = -1;
+    goto out;
+  }
+  partitionCsrByNonzeros(ia, ja, neq, nthreads, bo.thBegin, bo.thEnd, bo.thSize,
+                         bo.thHeight, storage);
+  ompVar.fSolver = true;
+  ompVar.nThreadsSolver = (unsigned short)nthreads;
+  solverC(&g_mem, neq, neq, nad, 0, ia, ja, al, ad, NULL, b, x, NULL, &bo,
+          tol, maxit, storage, solver, f, true, true, unsym != 0, "ref");
+  ompVar.fSolver = false;
+  ompVar.nThreadsSolver = 1;
+  fflush(f);
+  rewind(f);
+  char buf[512] = {0};
+  if (!fgets(buf, sizeof buf, f)) { rc = -2; goto out; }
+  if (line && linelen > 0) { strncpy(line, buf, (size_t)linelen - 1); line[linelen - 1] = 0; }
+  double t, xkx, nx, tm_;
+  int it;
+  /* the OpenMP solvers' own log lines (src/Cg.c:706-713,
+   * src/Bicgstab.c:794-801) */
+  const int ok = solver == PCG
+      ? sscanf(buf, "PCGOMP: tol %lf iteration %d xKx %lf norma(x*x) %lf time %lf",
+               &t, &it, &xkx, &nx, &tm_)
+      : sscanf(buf, "PBICGSTABOMP: tol %lf iteration %d xKx %lf norma(x*x) %lf time %lf",
+               &t, &it, &xkx, &nx, &tm_);
+  if (ok != 5) { rc = -3; goto out; }
+  out[0] = it; out[1] = xkx; out[2] = nx; out[3] = tm_;
+out:
+  if (f) fclose(f);
+  free(bo.thBegin); free(bo.thEnd); free(bo.thSize); free(bo.thHeight); free(bo.thY);
+  return rc;
+}
+#endif
 
 /* y = A x with the reference's serial matvec for the storage. */
 int ref_matvec(short storage, int unsym, int neq, int *ia, int *ja,

@@ -149,3 +149,25 @@ def test_pcg_omp_maxit_and_fixed_iterations():
         xs, ss = oracle.pcg(B.system, B.b, 0.0, k)
         assert so["status"] == 1 and ss["status"] == 1
         assert np.linalg.norm(xo - xs) / np.linalg.norm(xs) <= 1e-12
+
+
+@pytest.mark.ref
+@pytest.mark.parametrize("threads", [1, 4])
+def test_reference_openmp_solvers_match_serial_oracle(threads):
+    """The reference's own OpenMP solvers (pcgOmp, pbicgstabOmp through
+    solverC, oracle/_ref/libsfr_ref_omp.so -- the CPU baseline) against the
+    serial oracle: identical iteration counts, x to 1e-12, bit for bit on
+    one thread (its row partition, scatter buffer and reductions then
+    follow the serial order)."""
+    if not oracle.ref_omp_available():
+        pytest.skip("oracle/_ref/libsfr_ref_omp.so not built (reference tree absent)")
+    for B, solver, f in ((Y.poisson3d(14, seed=3, storage=3), 1, oracle.pcg),
+                         (Y.poisson3d(14, seed=3, storage=2), 1, oracle.pcg),
+                         (Y.momentum3d(12), 2, oracle.pbicgstab)):
+        xs, ss = f(B.system, B.b, 1e-10, 5000)
+        xr, sr = oracle.ref_solve_omp(B.system, B.b, 1e-10, 5000, solver, threads)
+        assert sr["iters"] == ss["iters"]
+        assert np.linalg.norm(xr - xs) / np.linalg.norm(xs) <= 1e-12
+        assert abs(sr["xKx"] - ss["xKx"]) <= 1e-11 * abs(ss["xKx"])
+        if threads == 1:
+            assert np.array_equal(xr, xs)
