@@ -174,6 +174,25 @@ int sfr_solve_device(sfr_context *ctx, sfr_matrix *A, int solver,
                      const double *d_b, double *d_x, int newX,
                      sfr_stats *stats);
 
+/* Allocation-free solve: sfr_solve_device for SFR_PCG or SFR_PBICGSTAB in
+ * FP64 that allocates nothing -- no hipMalloc, hipFree, graph or other
+ * allocating runtime call.  No counterpart in the reference: it is for
+ * callers that cannot allocate (code around HIP graph capture, tight
+ * serving loops, harnesses that forbid allocation).  The caller passes the
+ * resident matrix, d_b and d_x; the solve's scalars, partial sums and work
+ * vectors live in the library's static device scratch, 64 MiB that each GPU
+ * the library runs on holds until the process ends.  Limit: 2,708,640 rows
+ * for PCG, 1,354,304 for PBICGSTAB; a larger system is refused with
+ * SFR_BAD_ARG.  The kernels are those of sfr_solve_device, launched directly
+ * instead of from captured graphs, so x, the iteration count and the
+ * statistics are bitwise those of sfr_solve_device.  Every call on one GPU
+ * uses that GPU's one scratch and finds the previous call's contents there:
+ * calls that overlap on one GPU (other contexts, other threads) race, and
+ * the caller must serialise them. */
+int sfr_solve_device_noalloc(sfr_context *ctx, sfr_matrix *A, int solver,
+                             double tol, unsigned maxit, const double *d_b,
+                             double *d_x, int newX, sfr_stats *stats);
+
 /* Shared-pattern batch (SFR3D's u/v/w momentum solves, reference
  * src/Simple.c:2738-2750, 2980-3036): nsys (1..4) systems with the same
  * ia/ja and off-diagonal values, each with its own diagonal, right-hand side

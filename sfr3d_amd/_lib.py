@@ -106,6 +106,8 @@ _sig("sfr_solve", C.c_int, vp, C.POINTER(SfrSystem), C.c_int, C.c_double,
      C.c_uint, vp, vp, C.c_int, C.POINTER(SfrStats))
 _sig("sfr_solve_device", C.c_int, vp, vp, C.c_int, C.c_int, C.c_double,
      C.c_uint, vp, vp, C.c_int, C.POINTER(SfrStats))
+_sig("sfr_solve_device_noalloc", C.c_int, vp, vp, C.c_int, C.c_double, C.c_uint,
+     vp, vp, C.c_int, C.POINTER(SfrStats))
 _sig("sfr_solve_batched", C.c_int, vp, C.POINTER(SfrSystem), C.c_int, vp, vp, vp,
      vp, vp, C.c_int, C.c_double, C.c_uint, C.c_int, C.POINTER(SfrStats))
 _sig("sfr_solve_shared", C.c_int, vp, C.POINTER(SfrSystem), C.c_int, vp, vp,

@@ -89,6 +89,7 @@ int sfr_init(sfr_context **out, int device)
   INIT_TRY(hipEventCreate(&ctx->ev_stop));
   INIT_TRY(hipEventCreateWithFlags(&ctx->ev_poll[0], hipEventDisableTiming));
   INIT_TRY(hipEventCreateWithFlags(&ctx->ev_poll[1], hipEventDisableTiming));
+  INIT_TRY(sfr_scratch_address(device, &ctx->scratch));
 #undef INIT_TRY
   *out = ctx;
   return SFR_OK;

@@ -205,12 +205,38 @@ static int run_loop_n(sfr_context *ctx, hipGraphExec_t ex, int which,
  * issued past convergence to two short chunks, while a long solve soon
  * runs on full chunks.  `last` (may be NULL) receives the first system's
  * scalars as the last chunk seen left them. */
+/* The loop body launched without graphs (the allocation-free solve): the
+ * iteration kernels enqueued directly, `iters` iterations per chunk, with
+ * the scalars, partials and work vectors wherever the caller keeps them. */
+typedef struct direct_loop {
+  int           solver;         /* SFR_PCG or SFR_PBICGSTAB                */
+  sfr_mat_view  v;
+  sfr_geom      g;
+  double       *x, *w[6];
+  sfr_scalars  *sc;
+  double       *part;
+} direct_loop;
+
+static int direct_chunk(sfr_context *ctx, const direct_loop *d, unsigned iters)
+{
+  for (unsigned it = 0; it < iters; it++) {
+    if (d->solver == SFR_PCG)
+      TRY(sfr_k_pcg_iter(ctx->stream, &d->v, d->g, d->x, d->w[0], d->w[1], d->w[2], d->sc,
+                         d->part));
+    else
+      TRY(sfr_k_bicg_iter(ctx->stream, &d->v, d->g, d->x, d->w[0], d->w[1], d->w[2], d->w[3],
+                          d->w[4], d->w[5], d->sc, d->part));
+  }
+  return SFR_OK;
+}
+
 static int run_loop_ramp(sfr_context *ctx, hipGraphExec_t exs, unsigned cs,
                          hipGraphExec_t exb, unsigned cb, unsigned ramp,
                          int which, unsigned max_iters, int nsys,
-                         sfr_scalars *last)
+                         sfr_scalars *last, const direct_loop *dl)
 {
   hipStream_t s = ctx->stream;
+  const sfr_scalars *dsc = dl ? dl->sc : ctx->d_sc;
   unsigned launched = 0, seen = 0, iters = 0;
   if (last) {
     memset(last, 0, sizeof *last);
@@ -220,8 +246,9 @@ static int run_loop_ramp(sfr_context *ctx, hipGraphExec_t exs, unsigned cs,
     while (launched - seen < 2 && iters < max_iters) {
       const int slot = (int)(launched & 1);
       const int small = iters < ramp;
-      TRY(hipGraphLaunch(small ? exs : exb, s));
-      TRY(hipMemcpyAsync(ctx->h_poll + SFR_MAXSYS * slot, ctx->d_sc,
+      if (dl) CHECK(direct_chunk(ctx, dl, small ? cs : cb));
+      else TRY(hipGraphLaunch(small ? exs : exb, s));
+      TRY(hipMemcpyAsync(ctx->h_poll + SFR_MAXSYS * slot, dsc,
                          sizeof(sfr_scalars) * (size_t)nsys, hipMemcpyDeviceToHost, s));
       TRY(hipEventRecord(ctx->ev_poll[slot], s));
       iters += small ? cs : cb;
@@ -259,7 +286,7 @@ static int run_chunks(sfr_context *ctx, const chunks *c, int which, unsigned max
                       int nsys)
 {
   return run_loop_ramp(ctx, c->s, c->cs, c->b, c->cb, c->cb, which, maxit + 2 * c->cb, nsys,
-                       NULL);
+                       NULL, NULL);
 }
 
 void sfr_fill_stats(const sfr_scalars *sc, sfr_stats *st, float ms)
@@ -285,12 +312,15 @@ void sfr_fill_stats(const sfr_scalars *sc, sfr_stats *st, float ms)
   }
 }
 
-static int finish(sfr_context *ctx, const sfr_mat_view *v, sfr_geom g,
-                  const double *d_b, const double *d_x, sfr_stats *st)
+/* The final norms of a solve whose scalars and partials are at sc / part,
+ * then its statistics. */
+static int finish_at(sfr_context *ctx, const sfr_mat_view *v, sfr_geom g,
+                     const double *d_b, const double *d_x, sfr_scalars *sc,
+                     double *part, sfr_stats *st)
 {
-  TRY(sfr_k_final(ctx->stream, v, g, d_b, d_x, ctx->d_sc, ctx->part));
+  TRY(sfr_k_final(ctx->stream, v, g, d_b, d_x, sc, part));
   TRY(hipEventRecord(ctx->ev_stop, ctx->stream));
-  TRY(hipMemcpyAsync(ctx->h_sc, ctx->d_sc, sizeof(sfr_scalars), hipMemcpyDeviceToHost,
+  TRY(hipMemcpyAsync(ctx->h_sc, sc, sizeof(sfr_scalars), hipMemcpyDeviceToHost,
                      ctx->stream));
   TRY(hipStreamSynchronize(ctx->stream));
   float ms = 0.0f;
@@ -299,6 +329,12 @@ static int finish(sfr_context *ctx, const sfr_mat_view *v, sfr_geom g,
   sfr_fill_stats(ctx->h_sc, &tmp, ms);
   if (st) *st = tmp;
   return tmp.status;
+}
+
+static int finish(sfr_context *ctx, const sfr_mat_view *v, sfr_geom g,
+                  const double *d_b, const double *d_x, sfr_stats *st)
+{
+  return finish_at(ctx, v, g, d_b, d_x, ctx->d_sc, ctx->part, st);
 }
 
 
@@ -504,7 +540,7 @@ static int solve_pcg_mixed(sfr_context *ctx, sfr_matrix *A, double tol,
      * fold runs about log(delta) / log(rate) iterations */
     sfr_scalars last;
     CHECK(run_loop_ramp(ctx, ch.s, ch.cs, ch.b, ch.cb, folds ? 0 : ch.cb, 1,
-                        maxit + 2 * ch.cb, 1, &last));
+                        maxit + 2 * ch.cb, 1, &last, NULL));
     if (last.done != SFR_DONE_RUN) break;
     if (!last.ru) break;                  /* loop stopped with no request */
     /* Once MA or MB requests a fold, every later MA / MB returns at once
@@ -809,6 +845,59 @@ int sfr_solve_device(sfr_context *ctx, sfr_matrix *A, int solver,
     return solve_minres64(ctx, A, tol, maxit, d_b, d_x, newX, st);
   }
   return sfr_set_error(ctx, SFR_BAD_ARG, "solver id %d", solver);
+}
+
+/* Work vectors of the allocation-free solve: PCG r, p, q; BiCGSTAB r, r0,
+ * p, z, v, t; each padded to 256 bytes in the static scratch. */
+static int noalloc_vectors(int solver)
+{
+  return solver == SFR_PCG ? 3 : 6;
+}
+
+int sfr_solve_device_noalloc(sfr_context *ctx, sfr_matrix *A, int solver,
+                             double tol, unsigned maxit, const double *d_b,
+                             double *d_x, int newX, sfr_stats *st)
+{
+  if (!ctx || !A || !d_b || !d_x) return SFR_BAD_ARG;
+  if (solver != SFR_PCG && solver != SFR_PBICGSTAB)
+    return sfr_set_error(ctx, SFR_NOT_SUPPORTED,
+                         "allocation-free solve: solver %d (PCG and PBICGSTAB only)", solver);
+  if (maxit == 0) return sfr_set_error(ctx, SFR_BAD_ARG, "maxit = 0");
+  if (!ctx->scratch) return sfr_set_error(ctx, SFR_HIP_ERROR, "static scratch unavailable");
+  const int nvec = noalloc_vectors(solver);
+  const int64_t ld = ((int64_t)A->n + 31) & ~(int64_t)31;
+  const int64_t room = (int64_t)((SFR_SCRATCH_BYTES - SFR_SCRATCH_VEC) / sizeof(double));
+  if (nvec * ld > room)
+    return sfr_set_error(ctx, SFR_BAD_ARG,
+                         "allocation-free solve: %d rows exceed the %zu MiB static scratch "
+                         "(at most %lld rows for solver %d)", (int)A->n,
+                         SFR_SCRATCH_BYTES >> 20, (long long)((room / nvec) & ~(int64_t)31),
+                         solver);
+  TRY(hipSetDevice(ctx->device));
+  unsigned char *base = (unsigned char *)ctx->scratch;
+  direct_loop d;
+  memset(&d, 0, sizeof d);
+  d.solver = solver;
+  d.v = sfr_view(A);
+  d.g = (sfr_geom){(solver == SFR_PCG && A->rpt_pcg) ? A->rpt_pcg : A->rpt, 0, 0};
+  d.x = d_x;
+  d.sc = (sfr_scalars *)base;
+  d.part = (double *)(base + SFR_SCRATCH_PART);
+  for (int k = 0; k < nvec; k++) d.w[k] = (double *)(base + SFR_SCRATCH_VEC) + k * ld;
+  hipStream_t s = ctx->stream;
+  /* the scratch keeps the previous call's scalars: start from zero */
+  TRY(sfr_k_fill(s, (int64_t)((sizeof(sfr_scalars) + 7) / 8), 0.0, (double *)d.sc));
+  TRY(hipEventRecord(ctx->ev_start, s));
+  if (solver == SFR_PCG)
+    TRY(sfr_k_pcg_setup(s, &d.v, d.g, d_b, d_x, d.w[0], d.w[1], newX, tol, maxit, d.sc,
+                        d.part));
+  else
+    TRY(sfr_k_bicg_setup(s, &d.v, d.g, d_b, d_x, d.w[0], d.w[1], d.w[2], d.w[3], newX, tol,
+                         maxit, d.sc, d.part));
+  const int kind = solver == SFR_PCG ? G_PCG : G_BICG;
+  const unsigned cs = (unsigned)chunk_of(ctx, kind, 1), cb = (unsigned)chunk_of(ctx, kind, 0);
+  CHECK(run_loop_ramp(ctx, NULL, cs, NULL, cb, cb, 0, maxit + 2 * cb, 1, NULL, &d));
+  return finish_at(ctx, &d.v, d.g, d_b, d_x, d.sc, d.part, st);
 }
 
 /* Average duration of one launch of a loop kernel, timed with HIP events on

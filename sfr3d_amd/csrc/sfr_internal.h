@@ -25,6 +25,13 @@ extern "C" {
 #define SFR_D16_MAXEXC 256       /* most exceptions a 16-bit ELL may have   */
 #define SFR_GM_K    15           /* GMRES Krylov dimension (src/Solv.c:82)  */
 #define SFR_GM_STATE 512         /* doubles of GMRES state: h, c, s, e      */
+/* Static device scratch of the allocation-free solve (device/scratch.hip):
+ * scalars at 0, the four partial-sum regions of a solve (4 x 4 x SFR_MAXG
+ * doubles) from SFR_SCRATCH_PART, work vectors from SFR_SCRATCH_VEC, each
+ * on a 256-byte boundary. */
+#define SFR_SCRATCH_BYTES ((size_t)64 << 20)
+#define SFR_SCRATCH_PART  ((size_t)4096)
+#define SFR_SCRATCH_VEC   (SFR_SCRATCH_PART + (size_t)16 * SFR_MAXG * sizeof(double))
 
 /* Scalars of one Krylov solve, resident on the device.  The last workgroup
  * of each reducing kernel writes them; later kernels read them. */
@@ -151,6 +158,7 @@ struct sfr_context {
   double      *nb_raw;          /*   and one system's raw values           */
   int64_t      nb_val_n, nb_raw_n;
   double       ru_delta;        /* MIXED: reliable-update residual drop    */
+  void        *scratch;         /* device address of the static scratch    */
 };
 
 int sfr_set_error(sfr_context *ctx, int code, const char *fmt, ...);
@@ -165,6 +173,10 @@ int  sfr_matrix_prepare_mixed(sfr_context *ctx, sfr_matrix *A);
 int  sfr_matrix_refresh(sfr_context *ctx, sfr_matrix *A);
 sfr_mat_view sfr_view(const sfr_matrix *A);
 sfr_geom     sfr_geom_for(const sfr_context *ctx, int64_t n, int rpt);
+
+/* device address of the static scratch on `device` (the current device),
+ * looked up once per device; returns a hipError_t */
+int sfr_scratch_address(int device, void **ptr);
 
 /* ---- launchers (the .hip files); each enqueues on `s`, returns a hipError_t */
 /* matrix values */

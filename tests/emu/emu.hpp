@@ -31,6 +31,12 @@ using std::fabs;
 #define __shared__ static
 #define __forceinline__ inline
 #define __launch_bounds__(...)
+#ifndef HIP_SYMBOL
+#define HIP_SYMBOL(X) X
+#endif
+#ifndef __align__
+#define __align__(n) __attribute__((aligned(n)))
+#endif
 
 namespace emu {
 struct Idx { unsigned x, y, z; };

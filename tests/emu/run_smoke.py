@@ -154,6 +154,19 @@ check(f"assembly rc {rc}/{rc2} b {np.array_equal(db, b2)} it {st1.iters}/{st2.it
       rc == 0 and rc2 == 0 and np.array_equal(db, b2) and np.array_equal(x1, x2))
 lib.sfr_assembly_destroy(ctx, plan)
 lib.sfr_matrix_destroy(ctx, A1); lib.sfr_matrix_destroy(ctx, A2)
+# allocation-free solve (static scratch, no graphs): bitwise the graph path,
+# twice in a row, the second on a smaller system
+lib.sfr_solve_device_noalloc.argtypes = [vp, vp, C.c_int, C.c_double, C.c_uint, vp, vp, C.c_int, vp]
+for name, solver in (("mom", 2), ("pois", 1)):
+    s, maxit = mk(name); A = vp(); assert lib.sfr_matrix_create(ctx, C.byref(s), 0, C.byref(A)) == 0
+    b = np.ascontiguousarray(z[f"{name}_b"][0] if name == "mom" else z[f"{name}_b"])
+    x1, x2 = np.zeros(s.neq), np.zeros(s.neq); st1, st2 = St(), St()
+    rc1 = lib.sfr_solve_device_noalloc(ctx, A, solver, 1e-8, maxit, b.ctypes.data, x1.ctypes.data, 1, C.byref(st1))
+    rc2 = lib.sfr_solve_device(ctx, A, solver, 0, 1e-8, maxit, b.ctypes.data, x2.ctypes.data, 1, C.byref(st2))
+    check(f"noalloc {name} rc {rc1}/{rc2} it {st1.iters}/{st2.iters}",
+          rc1 == 0 and rc2 == 0 and np.array_equal(x1, x2) and st1.iters == st2.iters
+          and st1.xKx == st2.xKx)
+    lib.sfr_matrix_destroy(ctx, A)
 lib.sfr_finalize(ctx)
 print("SMOKE", "FAILED" if fails else "OK", flush=True)
 os._exit(1 if fails else 0)

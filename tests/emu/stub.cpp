@@ -20,6 +20,7 @@ static inline void __sanitizer_finish_switch_fiber(void *, const void **, size_t
 #include <thread>
 #include <vector>
 #include "emu.hpp"
+#include "sfr_internal.h"
 
 namespace emu {
 thread_local Idx tIdx, bIdx;
@@ -170,6 +171,14 @@ hipError_t hipFree(void *p)
 {
   if (p) g_dev.erase((uintptr_t)p);
   free(p);
+  return hipSuccess;
+}
+// A __device__ variable is host memory here; the library's one such
+// variable is its static scratch, registered as a device range of its size.
+hipError_t hipGetSymbolAddress(void **p, const void *sym)
+{
+  *p = (void *)sym;
+  g_dev[(uintptr_t)sym] = SFR_SCRATCH_BYTES;
   return hipSuccess;
 }
 hipError_t hipHostMalloc(void **p, size_t n, unsigned int)

@@ -116,7 +116,7 @@ def test_kernels_under_cpu_emulation(emu_lib, tmp_path):
     print(res.stdout)
     assert res.returncode == 0, res.stdout + res.stderr[-3000:]
     assert "SMOKE OK" in res.stdout
-    assert res.stdout.count("PASS") == 15
+    assert res.stdout.count("PASS") == 17
 
 
 @pytest.fixture(scope="module")
