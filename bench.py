@@ -351,58 +351,85 @@ def bench_assembly(ctx, S, A, B, db, reps=5):
 
 
 def bench_uvw(ctx, S, systems, lib, reps=3):
-    """u/v/w momentum (96^3 Taylor-Green upwind, shared CSRD pattern):
-    one batched BiCGSTAB launch stream vs three sequential device solves."""
+    """u/v/w momentum (96^3 Taylor-Green upwind, shared CSRD pattern, the
+    components' own diagonals ad (1 + 0.05 c) as the reference's ad + c*n):
+    one batched BiCGSTAB launch stream in FP64 and in mixed precision
+    (FP32 iterations, FP64 reliable updates) vs three sequential FP64
+    device solves."""
     from sfr3d_amd._lib import SfrStats
+    from sfr3d_amd.solver import System
     M = systems.momentum3d(96)
-    n = M.system.neq
-    A = ctx.matrix(M.system)
+    s0 = M.system
+    n = s0.neq
+    ads = [s0.ad[:n] * (1.0 + 0.05 * c) for c in range(3)]
+    mats = [ctx.matrix(System(n, s0.nad, s0.ia, s0.ja, s0.al, ads[c], s0.storage, s0.unsym))
+            for c in range(3)]
     bs = M.extra["b_all"]
     db = ctx.array(np.ascontiguousarray(bs.reshape(-1)))
+    dad = ctx.array(np.ascontiguousarray(np.concatenate(ads)))
     dx = ctx.empty(3 * n)
     db1 = [ctx.array(bs[c]) for c in range(3)]
     dx1 = ctx.empty(n)
     st = (SfrStats * 3)()
 
-    def batched():
-        rc = lib.sfr_solve_shared_device(ctx.h, A.h, 3, None, db.ptr, dx.ptr,
+    def batched(prec):
+        ctx.set_option("precision", prec)
+        rc = lib.sfr_solve_shared_device(ctx.h, mats[0].h, 3, dad.ptr, db.ptr, dx.ptr,
                                          S.PBICGSTAB, TOL, 5000, 1, st)
+        ctx.set_option("precision", S.PREC_FP64)
         assert rc == 0, rc
 
     def sequential():
         out = []
         for c in range(3):
-            s1 = ctx.solve_device(A, S.PBICGSTAB, TOL, 5000, db1[c], dx1)
+            s1 = ctx.solve_device(mats[c], S.PBICGSTAB, TOL, 5000, db1[c], dx1)
             out.append((s1, dx1.get()))
         return out
 
-    batched()
-    sequential()
-    t = time.perf_counter()
-    for _ in range(reps):
-        batched()
-    lib.sfr_synchronize(ctx.h)
-    tb = (time.perf_counter() - t) / reps
+    def timed(f, *a):
+        f(*a)
+        lib.sfr_synchronize(ctx.h)
+        t = time.perf_counter()
+        for _ in range(reps):
+            r = f(*a)
+        lib.sfr_synchronize(ctx.h)
+        return (time.perf_counter() - t) / reps, r
+
+    tb, _ = timed(batched, S.PREC_FP64)
     xb = dx.get().reshape(3, n)
-    t = time.perf_counter()
-    for _ in range(reps):
-        seq = sequential()
-    ts = (time.perf_counter() - t) / reps
+    its = [st[c].iters for c in range(3)]
+    relres = [st[c].relres for c in range(3)]
+    tm, _ = timed(batched, S.PREC_MIXED)
+    xm = dx.get().reshape(3, n)
+    its_m = [st[c].iters for c in range(3)]
+    outer_m = [st[c].outer for c in range(3)]
+    relres_m = [st[c].relres for c in range(3)]
+    ts, seq = timed(sequential)
     rel = max(float(np.linalg.norm(xb[c] - seq[c][1]) / np.linalg.norm(seq[c][1]))
               for c in range(3))
-    its = [st[c].iters for c in range(3)]
+    rel_m = max(float(np.linalg.norm(xm[c] - xb[c]) / np.linalg.norm(xb[c]))
+                for c in range(3))
     log(f"[bench] u/v/w 96^3 BiCGSTAB: batched {tb * 1e3:.2f} ms, 3 sequential "
-        f"{ts * 1e3:.2f} ms ({ts / tb:.2f}x), iters {its}, max relerr {rel:.1e}")
-    A.destroy(); db.free(); dx.free(); dx1.free()
+        f"{ts * 1e3:.2f} ms ({ts / tb:.2f}x), iters {its}, max relerr {rel:.1e}; "
+        f"mixed batched {tm * 1e3:.2f} ms, iters {its_m} folds {outer_m}, "
+        f"relerr vs FP64 {rel_m:.1e}")
+    for m in mats:
+        m.destroy()
+    db.free(); dad.free(); dx.free(); dx1.free()
     for d in db1:
         d.free()
     return {"workload": "96^3 Taylor-Green upwind momentum, u/v/w, shared "
-                        "CSRD pattern, Jacobi-BiCGSTAB to tol 1e-8",
+                        "CSRD pattern, own diagonals ad (1 + 0.05 c), "
+                        "Jacobi-BiCGSTAB to tol 1e-8",
             "neq": n, "batched_ms": round(tb * 1e3, 3),
             "sequential_ms": round(ts * 1e3, 3),
             "speedup": round(ts / tb, 3), "iters": its,
-            "relres": [st[c].relres for c in range(3)],
-            "max_relerr_batched_vs_sequential": rel}
+            "relres": relres,
+            "max_relerr_batched_vs_sequential": rel,
+            "mixed": {"batched_ms": round(tm * 1e3, 3), "iters": its_m,
+                      "folds": outer_m, "relres": relres_m,
+                      "max_relerr_vs_fp64_batched": rel_m,
+                      "speedup_vs_fp64_batched": round(tb / tm, 3)}}
 
 
 def bench_uneven(ctx, S, systems, reps=3):

@@ -160,12 +160,17 @@ void sfr_matrix_cache_clear(sfr_context *ctx);
 /* ---- solves ---------------------------------------------------------------
  * sfr_solve: host arrays in and out, the drop-in for solverC's PCG,
  * PBICGSTAB, PBICGSTABL2, GMRES and MINRES branches (reference
- * src/Solv.c:93-445).  BiCGSTAB(2), GMRES and MINRES run in FP64 only
- * (SFR_PREC_MIXED returns SFR_NOT_SUPPORTED).
+ * src/Solv.c:93-445).
  * sfr_solve_device: b and x already on the device (d_b, d_x: neq doubles).
- * precision SFR_PREC_MIXED runs FP32 inner solves under an FP64 refinement
- * loop whose residual, update and stopping test are FP64 and use the same
- * test as the FP64 solver. */
+ * precision SFR_PREC_MIXED (PCG and PBICGSTAB) runs the Krylov iterations
+ * in FP32 -- vectors and matrix values, dot products accumulated in FP64 --
+ * and folds their correction into the FP64 solution with reliable updates:
+ * the residual, the solution update and the stopping test are FP64, the
+ * test is the FP64 solver's own.  BiCGSTAB(2), GMRES and MINRES run in FP64
+ * only (SFR_PREC_MIXED returns SFR_NOT_SUPPORTED).  The host-pointer and
+ * batch entry points (sfr_solve, sfr_solverC, sfr_solve_shared*,
+ * sfr_solve_batched) take the precision from the context option
+ * "precision" (default SFR_PREC_FP64), for PCG and PBICGSTAB. */
 int sfr_solve(sfr_context *ctx, const sfr_system *sys, int solver, double tol,
               unsigned maxit, const double *b, double *x, int newX,
               sfr_stats *stats);
@@ -198,7 +203,8 @@ int sfr_solve_device_noalloc(sfr_context *ctx, sfr_matrix *A, int solver,
  * ia/ja and off-diagonal values, each with its own diagonal, right-hand side
  * and solution, solved by one set of batched launches in which every SpMV
  * reads the shared pattern once.  Each system keeps its own scalars and stops
- * on its own test; stats[c] holds system c's results.  Solver: PBICGSTAB.
+ * on its own test; stats[c] holds system c's results.  Solver: PBICGSTAB (in FP32
+ * with FP64 reliable updates under the context option "precision").
  * Host form: ad[c] (NULL: sys->ad for every system), b[c], x[c] are arrays of
  * neq doubles.  Device form: d_ad (or NULL), d_b, d_x hold nsys*neq doubles,
  * system c at offset c*neq (the reference's b + c*neq layout). */
@@ -219,7 +225,8 @@ int sfr_solve_shared_device(sfr_context *ctx, sfr_matrix *A, int nsys,
  * b[c] and x[c].  sys supplies neq, nad, ia, ja, storage and unsym; its
  * value pointers are ignored.  With solver PBICGSTAB the systems run in
  * groups of 2-4 through one set of batched launches per group, every SpMV
- * reading the pattern once for the group and each system's own values;
+ * reading the pattern once for the group and each system's own values
+ * (in FP32 with FP64 reliable updates under the "precision" option);
  * other solver ids and storage CSR solve the systems one after another.
  * Each system keeps its own stop test; stats[c] holds system c's results.
  * (The reference skips a species whose ||b|| < SZERO before calling the
@@ -322,11 +329,14 @@ void sfr_assembly_destroy(sfr_context *ctx, sfr_assembly *plan);
  * the updated direction, direction and correction updates, p.q). */
 int sfr_kernel_time(sfr_context *ctx, sfr_matrix *A, int which, int reps,
                     double *ms_per_launch);
-/* Tuning knobs of a context: "chunk" = iterations per full captured graph
+/* Options of a context: "chunk" = iterations per full captured graph
  * chunk (1..256; solves start on short chunks; GMRES captures chunk/16
- * restart cycles, at least one); "ru_delta" = MIXED: drop of the FP32
- * residual (relative to its largest value since the last fold) that
- * triggers an FP64 reliable update. */
+ * restart cycles, at least one); "precision" = SFR_PREC_FP64 or
+ * SFR_PREC_MIXED for the host-pointer and batch entry points (PCG and
+ * PBICGSTAB; the other solver ids stay FP64); "ru_delta" / "ru_delta_bicg"
+ * = MIXED PCG / BiCGSTAB: drop of the FP32 residual norm (relative to its
+ * largest value since the last fold) that triggers an FP64 reliable update
+ * (defaults 0.1 and 1e-3). */
 int sfr_set_option(sfr_context *ctx, const char *name, double value);
 
 /* Drop-in twin of solverC (reference include/Solv.h:59-70), same prototype

@@ -75,6 +75,7 @@ int sfr_init(sfr_context **out, int device)
   ctx->device = device;
   ctx->chunk = 16;
   ctx->ru_delta = 0.1;
+  ctx->ru_delta_b = 1e-3;
   if (hipSetDevice(device) != hipSuccess) { free(ctx); return SFR_NO_DEVICE; }
   int rc = SFR_OK;
 #define INIT_TRY(call) do { if ((call) != hipSuccess) { rc = SFR_HIP_ERROR; goto fail; } } while (0)
@@ -135,6 +136,8 @@ void sfr_finalize(sfr_context *ctx)
   hipFree(ctx->gm_buf);
   hipFree(ctx->nb_val);
   hipFree(ctx->nb_raw);
+  hipFree(ctx->nb_val32);
+  hipFree(ctx->mb32);
   hipFree(ctx->d_sc);
   hipFree(ctx->part);
   if (ctx->h_sc) hipHostFree(ctx->h_sc);
@@ -231,12 +234,16 @@ int sfr_nb_buffers(sfr_context *ctx, int64_t nnz, int64_t nraw)
   drop_graphs(ctx);
   hipFree(ctx->nb_val);
   hipFree(ctx->nb_raw);
+  hipFree(ctx->nb_val32);
   ctx->nb_val = ctx->nb_raw = NULL;
-  ctx->nb_val_n = ctx->nb_raw_n = 0;
+  ctx->nb_val32 = NULL;
+  ctx->nb_val_n = ctx->nb_raw_n = ctx->nb_val32_n = 0;
   TRY(hipMalloc((void **)&ctx->nb_val, sizeof(double) * (size_t)nv));
   TRY(hipMalloc((void **)&ctx->nb_raw, sizeof(double) * (size_t)nr));
+  TRY(hipMalloc((void **)&ctx->nb_val32, sizeof(float) * (size_t)nv));
   ctx->nb_val_n = nv;
   ctx->nb_raw_n = nr;
+  ctx->nb_val32_n = nv;
   return SFR_OK;
 }
 
@@ -360,7 +367,7 @@ void sfr_matrix_destroy(sfr_context *ctx, sfr_matrix *A)
   if (A->T) sfr_matrix_destroy(ctx, A->T);
   hipFree(A->col); hipFree(A->col16); hipFree(A->exc_k); hipFree(A->exc_col);
   hipFree(A->rowptr); hipFree(A->vmap); hipFree(A->val);
-  hipFree(A->dscale); hipFree(A->val32); hipFree(A->grad_rc);
+  hipFree(A->dscale); hipFree(A->val32); hipFree(A->val32r); hipFree(A->grad_rc);
   if (!A->borrowed) {
     hipFree(A->dmap); hipFree(A->raw); hipFree(A->raw_ad); hipFree(A->ad);
     hipFree(A->dinv);
@@ -654,6 +661,32 @@ int sfr_matrix_refresh(sfr_context *ctx, sfr_matrix *A)
   if (A->T) TRY(sfr_k_gather(ctx->stream, A->T->nnz, A->T->vmap, A->raw, A->T->val));
   TRY(sfr_k_diag(ctx->stream, A->n, A->diag_in_al ? A->dmap : NULL, A->raw, A->raw_ad, A->ad, A->dinv));
   A->have32 = 0;
+  A->have32r = 0;
+  return SFR_OK;
+}
+
+/* FP32 copy of the device-format values, unscaled (mixed BiCGSTAB) */
+int sfr_matrix_prepare_f32(sfr_context *ctx, sfr_matrix *A)
+{
+  if (A->have32r) return SFR_OK;
+  if (!A->val32r) TRY(hipMalloc((void **)&A->val32r, sizeof(float) * (size_t)(A->nnz + 1)));
+  TRY(sfr_k_to_f32(ctx->stream, A->nnz, A->val, A->val32r));
+  A->have32r = 1;
+  return SFR_OK;
+}
+
+/* Mixed BiCGSTAB buffers: 9 FP32 vectors of up to n entries each (grow-only;
+ * captured graphs hold the old pointers, so they go when it grows). */
+int sfr_mb_buffers(sfr_context *ctx, int64_t n)
+{
+  if (n <= ctx->mb32_n) return SFR_OK;
+  TRY(hipStreamSynchronize(ctx->stream));
+  drop_graphs(ctx);
+  hipFree(ctx->mb32);
+  ctx->mb32 = NULL;
+  ctx->mb32_n = 0;
+  TRY(hipMalloc((void **)&ctx->mb32, sizeof(float) * 9 * (size_t)n));
+  ctx->mb32_n = n;
   return SFR_OK;
 }
 

@@ -31,7 +31,17 @@ int sfr_matrix_prepare_mixed(sfr_context *ctx, sfr_matrix *A);
 int sfr_matrix_prepare_adjoint(sfr_context *ctx, sfr_matrix *A);
 
 enum { G_PCG = 1, G_BICG = 2, G_MX = 3, G_BB = 4, G_GM = 5, G_MR = 6, G_B2 = 7,
-       G_BBV = 8 };
+       G_BBV = 8, G_MB = 9 };
+
+/* mixed BiCGSTAB state of nb systems in ctx->mb32: w[0..6] = y, r, r0, p,
+ * z, v, t, then the FP32 diagonal and inverse (nb*n floats each) */
+static void mb_split(sfr_context *ctx, int64_t nbn, float *w[7], float **ad32,
+                     float **dinv32)
+{
+  for (int k = 0; k < 7; k++) w[k] = ctx->mb32 + (size_t)k * (size_t)nbn;
+  *ad32 = ctx->mb32 + (size_t)7 * (size_t)nbn;
+  *dinv32 = ctx->mb32 + (size_t)8 * (size_t)nbn;
+}
 
 /* BiCGSTAB(2) iterations (two BiCG steps and a GCR(2) step, nine kernels)
  * per captured chunk */
@@ -136,6 +146,14 @@ static int get_graph(sfr_context *ctx, int kind, const void *k0,
     else if (kind == G_MR) {
       MR_RINGS(ctx);
       lrc = sfr_k_mr_iter(s, v, g, (double *)k1, V_, W_, Z_, it % 6, ctx->d_sc, ctx->part);
+    }
+    else if (kind == G_MB) {  /* k1: the FP32 values, k2: the batch size */
+      const int nb = (int)(intptr_t)k2;
+      float *w[7], *ad32, *dinv32;
+      mb_split(ctx, (int64_t)nb * v->n, w, &ad32, &dinv32);
+      lrc = sfr_k_mb_iter(s, v, g, nb, v->valnb ? NULL : (const float *)k1,
+                          v->valnb ? (const float *)k1 : NULL, w, ad32, dinv32, ctx->d_sc,
+                          ctx->part);
     }
     else                    /* G_BB, G_BBV: k2 carries the batch size */
       lrc = sfr_k_bb_iter(s, v, g, (int)(intptr_t)k2, w[8], w[9], w[0], w[1], w[2], w[3],
@@ -259,11 +277,15 @@ static int run_loop_ramp(sfr_context *ctx, hipGraphExec_t exs, unsigned cs,
     TRY(hipEventSynchronize(ctx->ev_poll[slot]));
     seen++;
     const sfr_scalars *p = ctx->h_poll + SFR_MAXSYS * slot;
-    if (last) *last = p[0];
-    int finished = 1;
-    for (int c = 0; c < nsys; c++)
+    if (last) memcpy(last, p, sizeof(sfr_scalars) * (size_t)nsys);
+    /* which = 0: every system done; 1: every system done or asking for a
+     * fold; 2: every system done, or any running one asking for a fold */
+    int finished = 1, asks = 0;
+    for (int c = 0; c < nsys; c++) {
       if (!(p[c].done || (which && p[c].ru))) finished = 0;
-    if (finished) break;
+      if (!p[c].done && p[c].ru) asks = 1;
+    }
+    if (finished || (which == 2 && asks)) break;
   }
   return SFR_OK;
 }
@@ -497,6 +519,83 @@ static int solve_minres64(sfr_context *ctx, sfr_matrix *A, double tol,
   return finish(ctx, &v, g, d_b, d_x, st);
 }
 
+/* Mixed-precision BiCGSTAB of nb systems on A's pattern (device kernels in
+ * bicgstab_mixed.hip): ad, b, x FP64 and interleaved per row (nb = 1:
+ * plain), valnb = per-system FP64 values interleaved per entry or NULL for
+ * A's own.  The loop runs captured chunks of the five FP32 kernels; after
+ * each chunk the host looks at the systems' fold requests and enqueues the
+ * fold kernels, which serve every system that asked, while the others go
+ * on in the next chunk.  mb_plan captures the chunks before the solve
+ * enqueues anything; mb_loop leaves the systems' scalars in d_sc. */
+typedef struct mb_plan {
+  sfr_mat_view v;
+  sfr_geom     g;
+  const float *vals32;
+  chunks       ch;
+} mb_plan;
+
+static int mb_plan_make(sfr_context *ctx, sfr_matrix *A, int nb, const double *valnb,
+                        mb_plan *pl)
+{
+  CHECK(sfr_mb_buffers(ctx, A->n * nb));
+  pl->v = sfr_view(A);
+  pl->v.valnb = valnb;
+  pl->g = (sfr_geom){A->rpt, 0, 0};
+  if (!valnb && !A->val32r)
+    TRY(hipMalloc((void **)&A->val32r, sizeof(float) * (size_t)(A->nnz + 1)));
+  pl->vals32 = valnb ? ctx->nb_val32 : A->val32r;
+  return get_chunks(ctx, G_MB, A, pl->vals32, (const void *)(intptr_t)nb, &pl->v, pl->g,
+                    &pl->ch);
+}
+
+static int mb_loop(sfr_context *ctx, sfr_matrix *A, const mb_plan *pl, int nb,
+                   const double *ad, const double *b, double *x, double tol,
+                   unsigned maxit, int newX)
+{
+  hipStream_t s = ctx->stream;
+  if (pl->v.valnb)
+    TRY(sfr_k_to_f32(s, A->nnz * nb, pl->v.valnb, ctx->nb_val32));
+  else
+    CHECK(sfr_matrix_prepare_f32(ctx, A));
+  const chunks *ch = &pl->ch;
+  float *w[7], *ad32, *dinv32;
+  mb_split(ctx, A->n * nb, w, &ad32, &dinv32);
+  TRY(sfr_k_mb_start(s, &pl->v, pl->g, nb, ad, b, x, ad32, dinv32, newX, tol, maxit,
+                     ctx->ru_delta_b, ctx->d_sc, ctx->part));
+  TRY(sfr_k_mb_fold(s, &pl->v, pl->g, nb, ad, b, x, w, ad32, dinv32, 1, newX, ctx->d_sc,
+                    ctx->part));
+  /* every fold is triggered by a residual drop of delta or a stop test:
+   * bound the number of stretches generously */
+  for (int folds = 0; folds < 100000; folds++) {
+    sfr_scalars last[SFR_MAXSYS];
+    CHECK(run_loop_ramp(ctx, ch->s, ch->cs, ch->b, ch->cb, folds ? 0 : ch->cb, 2,
+                        maxit + 2 * ch->cb, nb, last, NULL));
+    int asks = 0, running = 0;
+    for (int c = 0; c < nb; c++) {
+      if (last[c].done == SFR_DONE_RUN) running = 1;
+      if (last[c].done == SFR_DONE_RUN && last[c].ru) asks = 1;
+    }
+    if (!running || !asks) break;
+    /* the fold kernels queue behind the chunk still in flight and serve
+     * every system whose request is set when they run */
+    TRY(sfr_k_mb_fold(s, &pl->v, pl->g, nb, ad, b, x, w, ad32, dinv32, 0, 0, ctx->d_sc,
+                      ctx->part));
+  }
+  return SFR_OK;
+}
+
+static int solve_bicg_mixed(sfr_context *ctx, sfr_matrix *A, double tol,
+                            unsigned maxit, const double *d_b, double *d_x,
+                            int newX, sfr_stats *st)
+{
+  CHECK(sfr_workspace(ctx, A->n));
+  mb_plan pl;
+  CHECK(mb_plan_make(ctx, A, 1, NULL, &pl));
+  TRY(hipEventRecord(ctx->ev_start, ctx->stream));
+  CHECK(mb_loop(ctx, A, &pl, 1, A->ad, d_b, d_x, tol, maxit, newX));
+  return finish(ctx, &pl.v, pl.g, d_b, d_x, st);
+}
+
 /* Launch geometry of the mixed loop.  From 2^21 rows up it runs 2 rows per
  * thread with the grid capped at 1,024 workgroups, so that each workgroup
  * makes several passes.  The rows in flight then span a fraction of the
@@ -617,6 +716,14 @@ int sfr_cache_info(const sfr_context *ctx, int *entries, int64_t *created,
   return SFR_OK;
 }
 
+/* The arithmetic the context's "precision" option asks for, where the
+ * solver has a mixed-precision variant (PCG, PBICGSTAB); FP64 otherwise. */
+static int ctx_precision(const sfr_context *ctx, int solver)
+{
+  return (ctx->precision == SFR_PREC_MIXED && (solver == SFR_PCG || solver == SFR_PBICGSTAB))
+             ? SFR_PREC_MIXED : SFR_PREC_FP64;
+}
+
 /* Shared-pattern batch: interleave the system-major inputs, run the batched
  * BiCGSTAB, de-interleave x.  Workspace (interleaved, nsys*n each):
  * w0 x, w1 r, w2 r0, w3 p, w4 z, w5 v, w6 t, w7 b, w8 ad, w9 dinv. */
@@ -627,7 +734,7 @@ int sfr_cache_info(const sfr_context *ctx, int *entries, int64_t *created,
 static int shared_run(sfr_context *ctx, sfr_matrix *A, int nsys,
                       const double *d_ad, const double *d_b, double *d_x,
                       const double *valnb, double tol, unsigned maxit,
-                      int newX, sfr_stats *stats);
+                      int newX, int precision, sfr_stats *stats);
 
 int sfr_solve_shared_device(sfr_context *ctx, sfr_matrix *A, int nsys,
                             const double *d_ad, const double *d_b, double *d_x,
@@ -638,19 +745,21 @@ int sfr_solve_shared_device(sfr_context *ctx, sfr_matrix *A, int nsys,
   if (nsys == 1) {
     /* one system: the single-system path, with its own diagonal */
     if (d_ad) return sfr_set_error(ctx, SFR_NOT_SUPPORTED, "nsys = 1 with a separate diagonal");
-    return sfr_solve_device(ctx, A, solver, SFR_PREC_FP64, tol, maxit, d_b, d_x, newX, stats);
+    return sfr_solve_device(ctx, A, solver, ctx_precision(ctx, solver), tol, maxit, d_b, d_x,
+                            newX, stats);
   }
   if (solver != SFR_PBICGSTAB)
     return sfr_set_error(ctx, SFR_NOT_SUPPORTED, "shared-pattern batch: solver %d", solver);
   if (maxit == 0) return sfr_set_error(ctx, SFR_BAD_ARG, "maxit = 0");
   TRY(hipSetDevice(ctx->device));
-  return shared_run(ctx, A, nsys, d_ad, d_b, d_x, NULL, tol, maxit, newX, stats);
+  return shared_run(ctx, A, nsys, d_ad, d_b, d_x, NULL, tol, maxit, newX,
+                    ctx_precision(ctx, solver), stats);
 }
 
 static int shared_run(sfr_context *ctx, sfr_matrix *A, int nsys,
                       const double *d_ad, const double *d_b, double *d_x,
                       const double *valnb, double tol, unsigned maxit,
-                      int newX, sfr_stats *stats)
+                      int newX, int precision, sfr_stats *stats)
 {
   const int64_t n = A->n, tot = n * nsys;
   CHECK(sfr_workspace(ctx, tot));
@@ -659,9 +768,14 @@ static int shared_run(sfr_context *ctx, sfr_matrix *A, int nsys,
   sfr_geom g = {A->rpt, 0, 0};
   double *const *w = ctx->w64;
   hipStream_t s = ctx->stream;
+  const int mixed = precision == SFR_PREC_MIXED;
   chunks ch;
-  CHECK(get_chunks(ctx, valnb ? G_BBV : G_BB, A, w[0], (const void *)(intptr_t)nsys, &v, g,
-                   &ch));
+  mb_plan pl;
+  if (mixed)
+    CHECK(mb_plan_make(ctx, A, nsys, valnb, &pl));
+  else
+    CHECK(get_chunks(ctx, valnb ? G_BBV : G_BB, A, w[0], (const void *)(intptr_t)nsys, &v, g,
+                     &ch));
   TRY(hipEventRecord(ctx->ev_start, s));
   if (d_ad) TRY(sfr_k_interleave(s, n, nsys, d_ad, w[8], w[9]));
   else for (int c = 0; c < nsys; c++) {
@@ -672,9 +786,13 @@ static int shared_run(sfr_context *ctx, sfr_matrix *A, int nsys,
   if (!d_ad) TRY(sfr_k_interleave(s, n, nsys, w[1], w[8], w[9]));
   TRY(sfr_k_interleave(s, n, nsys, d_b, w[7], NULL));
   if (!newX) TRY(sfr_k_interleave(s, n, nsys, d_x, w[0], NULL));
-  TRY(sfr_k_bb_setup(s, &v, g, nsys, w[8], w[9], w[7], w[0], w[1], w[2], w[3], w[4], newX,
-                     tol, maxit, ctx->d_sc, ctx->part));
-  CHECK(run_chunks(ctx, &ch, 0, maxit, nsys));
+  if (mixed) {
+    CHECK(mb_loop(ctx, A, &pl, nsys, w[8], w[7], w[0], tol, maxit, newX));
+  } else {
+    TRY(sfr_k_bb_setup(s, &v, g, nsys, w[8], w[9], w[7], w[0], w[1], w[2], w[3], w[4], newX,
+                       tol, maxit, ctx->d_sc, ctx->part));
+    CHECK(run_chunks(ctx, &ch, 0, maxit, nsys));
+  }
   TRY(sfr_k_bb_final(s, &v, g, nsys, w[8], w[9], w[7], w[0], ctx->d_sc, ctx->part));
   TRY(sfr_k_deinterleave(s, n, nsys, w[0], d_x));
   TRY(hipEventRecord(ctx->ev_stop, s));
@@ -763,7 +881,7 @@ int sfr_solve_batched(sfr_context *ctx, const sfr_system *sys, int nsys,
       if (!newX) TRY(hipMemcpyAsync(d_x + j * n, x[c], bytes, hipMemcpyHostToDevice, s));
     }
     const int rc = shared_run(ctx, A, nb, d_ad, d_b, d_x, ctx->nb_val, tol, maxit, newX,
-                              stats ? stats + c0 : NULL);
+                              ctx_precision(ctx, solver), stats ? stats + c0 : NULL);
     if (rc == SFR_HIP_ERROR || rc == SFR_BAD_ARG || rc == SFR_NOT_SUPPORTED) return rc;
     if (rc != SFR_OK && worst == SFR_OK) worst = rc;
     for (int j = 0; j < nb; j++)
@@ -802,7 +920,8 @@ int sfr_solve_shared(sfr_context *ctx, const sfr_system *sys, int nsys,
   }
   int rc;
   if (nsys == 1)
-    rc = sfr_solve_device(ctx, A, solver, SFR_PREC_FP64, tol, maxit, d_b, d_x, newX, stats);
+    rc = sfr_solve_device(ctx, A, solver, ctx_precision(ctx, solver), tol, maxit, d_b, d_x,
+                          newX, stats);
   else
     rc = sfr_solve_shared_device(ctx, A, nsys, ad ? d_ad : NULL, d_b, d_x, solver, tol, maxit,
                                  newX, stats);
@@ -824,11 +943,10 @@ int sfr_solve_device(sfr_context *ctx, sfr_matrix *A, int solver,
     return precision == SFR_PREC_MIXED
                ? solve_pcg_mixed(ctx, A, tol, maxit, d_b, d_x, newX, st)
                : solve_pcg64(ctx, A, tol, maxit, d_b, d_x, newX, st);
-  if (solver == SFR_PBICGSTAB) {
-    if (precision == SFR_PREC_MIXED)
-      return sfr_set_error(ctx, SFR_NOT_SUPPORTED, "mixed BiCGSTAB");
-    return solve_bicg64(ctx, A, tol, maxit, d_b, d_x, newX, st);
-  }
+  if (solver == SFR_PBICGSTAB)
+    return precision == SFR_PREC_MIXED
+               ? solve_bicg_mixed(ctx, A, tol, maxit, d_b, d_x, newX, st)
+               : solve_bicg64(ctx, A, tol, maxit, d_b, d_x, newX, st);
   if (solver == SFR_PBICGSTABL2) {
     if (precision == SFR_PREC_MIXED)
       return sfr_set_error(ctx, SFR_NOT_SUPPORTED, "mixed BiCGSTAB(2)");
@@ -991,8 +1109,8 @@ int sfr_solve(sfr_context *ctx, const sfr_system *sys, int solver, double tol,
   const size_t bytes = sizeof(double) * (size_t)A->n;
   TRY(hipMemcpyAsync(d_b, b, bytes, hipMemcpyHostToDevice, ctx->stream));
   if (!newX) TRY(hipMemcpyAsync(d_x, x, bytes, hipMemcpyHostToDevice, ctx->stream));
-  const int rc = sfr_solve_device(ctx, A, solver, SFR_PREC_FP64, tol, maxit, d_b, d_x,
-                                  newX, st);
+  const int rc = sfr_solve_device(ctx, A, solver, ctx_precision(ctx, solver), tol, maxit,
+                                  d_b, d_x, newX, st);
   if (rc == SFR_HIP_ERROR || rc == SFR_BAD_ARG || rc == SFR_NOT_SUPPORTED) return rc;
   TRY(hipMemcpyAsync(x, d_x, bytes, hipMemcpyDeviceToHost, ctx->stream));
   TRY(hipStreamSynchronize(ctx->stream));
@@ -1127,9 +1245,21 @@ int sfr_set_option(sfr_context *ctx, const char *name, double value)
     ctx->chunk = (int)value;
     return SFR_OK;
   }
+  if (!strcmp(name, "precision")) {
+    if (value != SFR_PREC_FP64 && value != SFR_PREC_MIXED)
+      return sfr_set_error(ctx, SFR_BAD_ARG, "precision %g", value);
+    ctx->precision = (int)value;
+    return SFR_OK;
+  }
   if (!strcmp(name, "ru_delta")) {
     if (!(value > 0 && value < 1)) return sfr_set_error(ctx, SFR_BAD_ARG, "ru_delta %g", value);
     ctx->ru_delta = value;
+    return SFR_OK;
+  }
+  if (!strcmp(name, "ru_delta_bicg")) {
+    if (!(value > 0 && value < 1))
+      return sfr_set_error(ctx, SFR_BAD_ARG, "ru_delta_bicg %g", value);
+    ctx->ru_delta_b = value;
     return SFR_OK;
   }
   return sfr_set_error(ctx, SFR_BAD_ARG, "unknown option %s", name);

@@ -107,6 +107,8 @@ struct sfr_matrix {
   int      nexc;
   double  *raw, *raw_ad, *val, *ad, *dinv, *dscale;
   float   *val32;
+  float   *val32r;       /* FP32 copy of val (unscaled), mixed BiCGSTAB     */
+  int      have32r;      /*   and whether it is current                    */
   const int32_t *h_ia, *h_ja;   /* host pattern identity (value-only uploads) */
   /* adjoint support, built on first use */
   struct sfr_matrix *T;  /* transposed operator (shares raw, ad, dinv)     */
@@ -157,8 +159,14 @@ struct sfr_context {
   double      *nb_val;          /* species batch: interleaved values       */
   double      *nb_raw;          /*   and one system's raw values           */
   int64_t      nb_val_n, nb_raw_n;
-  double       ru_delta;        /* MIXED: reliable-update residual drop    */
+  double       ru_delta;        /* MIXED PCG: reliable-update residual drop */
+  double       ru_delta_b;      /* MIXED BiCGSTAB: the same                */
   void        *scratch;         /* device address of the static scratch    */
+  int          precision;       /* host-pointer and batch entry points     */
+  float       *mb32;            /* mixed BiCGSTAB: 7 vectors, diagonal and */
+  int64_t      mb32_n;          /*   inverse, 9 x mb32_n floats            */
+  float       *nb_val32;        /* species batch: FP32 interleaved values  */
+  int64_t      nb_val32_n;
 };
 
 int sfr_set_error(sfr_context *ctx, int code, const char *fmt, ...);
@@ -219,6 +227,9 @@ int sfr_tune_pcg(sfr_context *ctx, sfr_matrix *A);
 /* GMRES(SFR_GM_K), FP64 (reference src/Iters.c:587-790) */
 int sfr_gm_workspace(sfr_context *ctx, int64_t n);
 int sfr_nb_buffers(sfr_context *ctx, int64_t nnz, int64_t nraw);
+/* mixed BiCGSTAB: FP32 values of A (unscaled); 9 FP32 vectors of n entries */
+int sfr_matrix_prepare_f32(sfr_context *ctx, sfr_matrix *A);
+int sfr_mb_buffers(sfr_context *ctx, int64_t n);
 int sfr_k_gm_setup(hipStream_t s, const sfr_mat_view *A, sfr_geom g,
                    const double *b, double *x, int newX, double tol,
                    unsigned maxit, sfr_scalars *sc, double *part);
@@ -303,6 +314,28 @@ int sfr_k_mx_inner_iter(hipStream_t s, const sfr_mat_view *A, sfr_geom g,
 int sfr_k_mx_a_only(hipStream_t s, const sfr_mat_view *A, sfr_geom g,
                     const float *r, const float *po, float *pn, float *q,
                     float *y, sfr_scalars *sc, double *part);
+
+/* mixed precision BiCGSTAB for nb (1..4) systems on one pattern, vectors
+ * interleaved per row (nb = 1: plain), FP32 iterations with FP64 reliable
+ * updates.  w: y, r, r0, p, z, v, t (nb*n floats each); ad32, dinv32: the
+ * FP32 diagonal and inverse (nb*n).  ad, b, x: FP64, interleaved.  The
+ * FP32 operator is val32 (the shared values) or valnb32 (per-system values
+ * interleaved per entry) when not NULL; the view's valnb likewise for the
+ * FP64 products of the folds. */
+int sfr_k_mb_start(hipStream_t s, const sfr_mat_view *A, sfr_geom g, int nb,
+                   const double *ad, const double *b, double *x, float *ad32,
+                   float *dinv32, int newX, double tol, unsigned maxit,
+                   double delta, sfr_scalars *sc, double *part);
+int sfr_k_mb_fold(hipStream_t s, const sfr_mat_view *A, sfr_geom g, int nb,
+                  const double *ad, const double *b, double *x,
+                  float *const w[7], const float *ad32, const float *dinv32,
+                  int init, int zero_x, sfr_scalars *sc, double *part);
+int sfr_k_mb_iter(hipStream_t s, const sfr_mat_view *A, sfr_geom g, int nb,
+                  const float *val32, const float *valnb32, float *const w[7],
+                  const float *ad32, const float *dinv32, sfr_scalars *sc,
+                  double *part);
+/* dst[k] = (float) src[k], k < n */
+int sfr_k_to_f32(hipStream_t s, int64_t n, const double *src, float *dst);
 
 /* device assembly: dst[k] = src[map[k] - 1] (map > 0), 0 (map == 0),
  * unchanged (map < 0) */

@@ -84,6 +84,29 @@ check(f"species rc {rc} it {[sts[c].iters for c in range(3)]} rel {rs:.1e}", rc 
 lam = np.zeros(s.neq); gal = np.zeros(len(z["mom_al"])); gad = np.zeros(s.neq); st = St()
 rc = lib.sfr_solve_adjoint(ctx, A, 2, 1e-8, maxit, x.ctypes.data, bb[0].ctypes.data, lam.ctypes.data, gal.ctypes.data, gad.ctypes.data, C.byref(st))
 check(f"adjoint bicg rc {rc} it {st.iters}", rc == 0 and st.iters > 0)
+# mixed-precision BiCGSTAB (FP32 inner, FP64 reliable updates): one system,
+# the u/v/w batch and the species batch, against the oracle's FP64 solves
+lib.sfr_set_option.argtypes = [vp, C.c_char_p, C.c_double]
+oracle_uvw = z["mom_xo_uvw"]
+x = np.zeros(s.neq); st = St()
+rc = lib.sfr_solve_device(ctx, A, 2, 1, 1e-8, maxit, bb[0].ctypes.data, x.ctypes.data, 1, C.byref(st))
+check(f"mixed bicg rc {rc} it {st.iters} outer {st.outer} rel {rel(x, xo):.1e}",
+      rc == 0 and rel(x, xo) < 1e-6 and st.relres < 1e-8)
+assert lib.sfr_set_option(ctx, b"precision", 1.0) == 0
+xs = np.zeros(3 * s.neq); sts = (St * 3)()
+rc = lib.sfr_solve_shared_device(ctx, A, 3, None, flat.ctypes.data, xs.ctypes.data, 2, 1e-8, maxit, 1, C.cast(sts, vp))
+rs = max(rel(xs[c * s.neq:(c + 1) * s.neq], oracle_uvw[c]) for c in range(3))
+check(f"mixed shared rc {rc} it {[sts[c].iters for c in range(3)]} rel {rs:.1e}",
+      rc == 0 and rs < 1e-6 and max(sts[c].relres for c in range(3)) < 1e-8)
+spx = [np.zeros(s.neq) for c in range(3)]
+sts = (St * 3)()
+rc = lib.sfr_solve_batched(ctx, C.byref(s), 3, C.cast(P3(*[a.ctypes.data for a in spal]), vp),
+                           C.cast(P3(*[a.ctypes.data for a in spad]), vp), None,
+                           C.cast(P3(*[a.ctypes.data for a in spb]), vp),
+                           C.cast(P3(*[a.ctypes.data for a in spx]), vp), 2, 1e-10, maxit, 1, C.cast(sts, vp))
+rs = max(rel(spx[c], z[f"sp{c}_xo"]) for c in range(3))
+check(f"mixed species rc {rc} it {[sts[c].iters for c in range(3)]} rel {rs:.1e}", rc == 0 and rs < 1e-7)
+assert lib.sfr_set_option(ctx, b"precision", 0.0) == 0
 lib.sfr_matrix_destroy(ctx, A)
 # far columns (escaped 16-bit offsets): 3 BiCGSTAB iterations vs the oracle's 3
 s, maxit = mk("far"); A = vp(); assert lib.sfr_matrix_create(ctx, C.byref(s), 0, C.byref(A)) == 0

@@ -43,6 +43,8 @@ def write_cases(path):
     put("pois", B.system, B.b, 1, 1000)
     M = systems.momentum3d(5)
     put("mom", M.system, M.extra["b_all"], 2, 1000)
+    d["mom_xo_uvw"] = np.stack([oracle.pbicgstab(M.system, M.extra["b_all"][c], 1e-8, 1000)[0]
+                                for c in range(3)])
     xb, sb = oracle.pbicgstabl2(M.system, M.extra["b_all"][1], 1e-10, 1000)
     d["mom_b2_xo"], d["mom_b2_it"] = xb, np.array([sb["iters"]])
     # species batch: three value sets on the momentum pattern
@@ -116,7 +118,7 @@ def test_kernels_under_cpu_emulation(emu_lib, tmp_path):
     print(res.stdout)
     assert res.returncode == 0, res.stdout + res.stderr[-3000:]
     assert "SMOKE OK" in res.stdout
-    assert res.stdout.count("PASS") == 17
+    assert res.stdout.count("PASS") == 20
 
 
 @pytest.fixture(scope="module")
