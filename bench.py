@@ -33,6 +33,11 @@ TOL = 1e-8
 K_MX_A_TRAFFIC_128 = 130.7e6
 
 
+def finite(v):
+    """v, or None when it is not finite (the JSON line stays valid JSON)."""
+    return v if np.isfinite(v) else None
+
+
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
@@ -55,7 +60,7 @@ def main():
                     help="with --profile-kernels: rows per thread of the "
                          "kernels (0: the library's default)")
     ap.add_argument("--only", choices=["solver_ids", "species", "uvw", "uneven",
-                                       "adjoint"],
+                                       "adjoint", "cells"],
                     help="run one of the other workloads alone and print its "
                          "JSON (for rocprofv3 runs of that workload)")
     ap.add_argument("--solvers", default="PBICGSTAB,PBICGSTABL2,GMRES,PCG,MINRES",
@@ -77,6 +82,7 @@ def main():
         leg = {"solver_ids": lambda: bench_solvers(ctx, S, systems, cpu=not args.no_cpu,
                                                    names=args.solvers.split(",")),
                "species": lambda: bench_species(ctx, S, systems),
+               "cells": lambda: bench_cells(ctx, S, systems),
                "uvw": lambda: bench_uvw(ctx, S, systems, lib),
                "uneven": lambda: bench_uneven(ctx, S, systems),
                "adjoint": lambda: bench_adjoint(ctx, S, systems)}[args.only]
@@ -183,6 +189,8 @@ def main():
                  "rows_per_thread_default": info["rows_per_thread"],
                  "speedup_mixed_vs_fp64_tuned": round(t64t / tmx, 3)}
 
+    bicg_long = None if args.no_extra else bench_bicg_long(ctx, S, A, db, dx, x64)
+
     copy_gbs = None if args.no_extra else hbm_copy_ceiling(ctx)
     assembly = None if args.no_extra else bench_assembly(ctx, S, A, B, db)
 
@@ -203,6 +211,7 @@ def main():
         extra["adjoint"] = bench_adjoint(ctx, S, systems)
         extra["solver_ids"] = bench_solvers(ctx, S, systems, cpu=not args.no_cpu)
         extra["species"] = bench_species(ctx, S, systems)
+        extra["cells_pres"] = bench_cells(ctx, S, systems)
 
     out = {
         "metric": "Krylov solves/s + HBM GB/s, 128³ Poisson, mixed vs FP64 @ relres 1e-8",
@@ -234,6 +243,7 @@ def main():
                  "ref_criterion": [st64["norm_r_m"], st64["conv"]],
                  "solve_GBs_algorithmic": round(solve_bytes64 / t64 / 1e9, 1)},
         "fp64_tuned": tuned,
+        "bicgstab_long": bicg_long,
         "speedup_mixed_vs_fp64": round(t64 / tmx, 3),
         "roofline": {"bound": "hbm",
                      "kernel": "k_mx_a (fused FP32 iteration kernel: ELL SpMV "
@@ -258,6 +268,33 @@ def main():
     if args.no_extra:
         A.destroy()
     ctx.close()
+
+
+def bench_bicg_long(ctx, S, A, db, dx, x64, reps=2):
+    """BiCGSTAB on the same 128^3 system, FP64 and mixed (FP32 iterations,
+    FP64 reliable updates): a long solve, where the FP32 iterations rather
+    than the folds set the time."""
+    from sfr3d_amd._lib import lib
+    out = {"workload": "the 128^3 Poisson system above, Jacobi-BiCGSTAB to "
+                       "tol 1e-8 (reference stop test ||r|| < tol ||b||)"}
+    for name, prec in (("fp64", S.PREC_FP64), ("mixed", S.PREC_MIXED)):
+        st = ctx.solve_device(A, S.PBICGSTAB, TOL, 100000, db, dx, precision=prec)
+        lib.sfr_synchronize(ctx.h)
+        t1 = time.perf_counter()
+        for _ in range(reps):
+            st = ctx.solve_device(A, S.PBICGSTAB, TOL, 100000, db, dx,
+                                  precision=prec)
+        lib.sfr_synchronize(ctx.h)
+        ms = (time.perf_counter() - t1) / reps * 1e3
+        x = dx.get()
+        err = float(np.linalg.norm(x - x64) / np.linalg.norm(x64))
+        out[name] = {"ms": round(ms, 2), "iters": st["iters"],
+                     "folds": st["outer"], "status": st["status"],
+                     "relres_true": finite(st["relres"]),
+                     "relerr_vs_fp64_pcg": finite(err)}
+        log(f"[bench] BiCGSTAB {name}: {ms:.1f} ms, {st['iters']} iterations")
+    out["speedup_mixed"] = round(out["fp64"]["ms"] / out["mixed"]["ms"], 3)
+    return out
 
 
 def profile_kernels(ctx, A, lib):
@@ -516,6 +553,51 @@ def bench_adjoint(ctx, S, systems, reps=5):
     A.destroy()
     for d in (db, dx, dg, dl, dgal, dgad):
         d.free()
+    return out
+
+
+def bench_cells(ctx, S, systems, n=128, reps=20):
+    """The SIMPLE pressure-correction cell loop on the device at n^3 hex
+    cells (warped, graded mesh; every pressure-side load type): one call
+    turns the fields into lA / lB / rCell.  Algorithmic bytes per cell, each
+    stream once: six face slots of 9 doubles + 2 ints (80 B each), the
+    cell's density, pressure, gradient, velocity and d field (88 B) and its
+    face count (1 B), lA (7 doubles), lB and rCell (72 B): 641 B."""
+    t0 = time.time()
+    m = systems.hex_mesh(n, n, n, ratio=(1.01, 0.99, 1.0),
+                         shear=(0.05, 0.02, -0.03), warp=0.02)
+    case = systems.simple_pres_case(m, seed=1, diffusion=4, boundary="mixed")
+    log(f"[bench] cells: {n}^3 mesh and fields built ({time.time() - t0:.1f}s)")
+    numel = m["numel"]
+    t1 = time.perf_counter()
+    cells = ctx.cells(m, case)
+    create_s = time.perf_counter() - t1
+    d = {k: ctx.array(case[k]) for k in ("pres", "gradPres", "vel", "dField")}
+    d_lA, d_lB, d_rc = ctx.empty(numel * 7), ctx.empty(numel), ctx.empty(numel)
+    from sfr3d_amd._lib import lib
+
+    def call():
+        cells.pres(4, d["pres"], d["gradPres"], d["vel"], d["dField"], d_lA,
+                   d_lB, d_rc)
+    call()
+    lib.sfr_synchronize(ctx.h)
+    t1 = time.perf_counter()
+    for _ in range(reps):
+        call()
+    lib.sfr_synchronize(ctx.h)
+    ms = (time.perf_counter() - t1) / reps * 1e3
+    nbytes = 641 * numel
+    out = {"workload": f"{n}^3 warped hex mesh, SIMPLE pressure-correction "
+                       "cell loop (systFormSimplePres / cellSimplePres3D), "
+                       "over-relaxed diffusion, mixed boundary loads",
+           "cells": numel, "ms_per_call": round(ms, 4),
+           "bytes_per_cell": 641,
+           "GBs_algorithmic": round(nbytes / (ms * 1e-3) / 1e9, 1),
+           "create_s": round(create_s, 3)}
+    log(f"[bench] cells: {ms:.3f} ms per call, {out['GBs_algorithmic']} GB/s")
+    for a in list(d.values()) + [d_lA, d_lB, d_rc]:
+        a.free()
+    cells.destroy()
     return out
 
 

@@ -125,8 +125,9 @@ int  sfr_matrix_set_values(sfr_context *ctx, sfr_matrix *A, const double *al,
 void sfr_matrix_destroy(sfr_context *ctx, sfr_matrix *A);
 /* launch shape of the solve kernels: rows per thread (1, 2, 4, 8) and, for
  * the CSR format, lanes per row (1, 2, 4, 8).  An explicit rpt applies to
- * every loop; without one, the mixed loop of a matrix of 2^21 rows or more
- * runs 2 rows per thread on at most 1,024 workgroups.  rpt = 0 measures:
+ * every loop; without one, a matrix of 2^21 rows or more runs its mixed
+ * loop at 2 rows per thread on at most 1,024 workgroups and its FP64 PCG
+ * loop at 2 rows per thread.  rpt = 0 measures:
  * short FP64 PCG runs at each rows-per-thread choice pick the FP64 PCG
  * loop's (the other loops keep theirs).  It runs solves on the context's
  * stream, allocates two temporary n-vectors and takes tens of
@@ -141,7 +142,8 @@ int  sfr_matrix_info(const sfr_matrix *A, int32_t *neq, int *format,
  * 32768, e.g. banded after RCM), else 4 */
 int  sfr_matrix_index_bytes(const sfr_matrix *A);
 /* rows per thread of the solve kernels and of the FP64 PCG loop (the
- * latter differs after sfr_matrix_tune with rpt = 0) */
+ * latter differs from 2^21 rows up and after sfr_matrix_tune with
+ * rpt = 0) */
 int  sfr_matrix_rows_per_thread(const sfr_matrix *A, int *rpt, int *rpt_pcg);
 
 /* The host-pointer entry points (sfr_solve, sfr_solverC, sfr_solve_shared,
@@ -322,6 +324,100 @@ int  sfr_assembly_create(sfr_context *ctx, sfr_matrix *A,
 int  sfr_assemble_device(sfr_context *ctx, sfr_assembly *plan,
                          const double *d_lA, const double *d_lB, double *d_b);
 void sfr_assembly_destroy(sfr_context *ctx, sfr_assembly *plan);
+
+/* ---- cell coefficients on the device ---------------------------------------
+ * The finite-volume cell loop of SFR3D's SIMPLE pressure correction
+ * (systFormSimplePres, src/CellLoopInc.c:471-679, called at
+ * src/Simple.c:3053-3080; cell kernel cellSimplePres3D,
+ * src/CellPres.c:903-1149) run on the GPU: the mesh's geometry is uploaded
+ * once, and each call turns the current fields into the per-cell
+ * coefficients lA / lB (layout of sfr_assemble: numel*(maxViz+1) doubles,
+ * the centre at index maxViz) and the mass residual rCell, for
+ * sfr_assemble_device.  Together with the solve, nothing crosses the host
+ * link between SIMPLE iterations but the fields the caller chooses to read.
+ *
+ * sfr_cell_mesh holds SFR3D's own mesh arrays (struct Mesh), in its layout
+ * and 1-based conventions: nelcon (elm.adj.nelcon, -1 on the boundary),
+ * nFace (elm.adj.nViz), cellFace (elm.cellFace), owner (face.owner, two
+ * per face, the owner first), the cell geometry elm.geom.volume / dcca /
+ * xmcc and the face geometry face.mksi / ksi / area / normal / vSkew, the
+ * material mat (elm.mat, 1-based) with the property table prop
+ * (elm.material.prop, MAXPROP = 10 per material, density at index 1), and
+ * the boundary markers faceRvel / faceRpres (elm.faceRvel / faceRpres,
+ * maxViz+1 per cell, 1-based load index or 0; a cell whose own entry
+ * faceRpres[c*(maxViz+1)+nFace] is set has no pressure equation and is
+ * skipped, its lA / lB left as they are).  ndm must be 3; nFace <= maxViz
+ * <= 6.
+ *
+ * sfr_load is a constant SFR3D load (struct Loads with nTypeVar constant):
+ * type (Define.h: DIRICHLETBC 1, INLET 6, OUTLET 7, MOVEWALL 8,
+ * INLETSTAICTPRES 9, OPEN 10, INLETTOTALPRES 13, STATICWALL 14,
+ * FLUXPRES 15), its np parameters par and density.  loadsVel and loadsPres
+ * are the tables faceRvel / faceRpres index (loadsVel and loadsPresC at
+ * Simple.c:3053).  A load type the pressure cell does not use adds
+ * nothing, as in the reference (src/Loads.c:265-727, 728-845); load
+ * functions of position or time are refused.
+ *
+ * sfr_cells_pres_device: diffusion = the scheme of difusionSchemeAnisotropic
+ * (src/CellLib.c:6054-6190; 1 ORTHOGONAL or 4 OVERRELAXED: the reference
+ * leaves MINIMAL and ORTHOGONALC undefined); device fields in the
+ * reference's layout: pres[numel], gradPres[numel*3], vel[numel*3],
+ * dField[numel*3] (the SIMPLE d field); outputs d_lA, d_lB, and d_rCell
+ * when not NULL.  The arithmetic keeps the reference's operation order
+ * without fused multiply-adds, so lA / lB are bitwise those of the
+ * reference's cell loop on the same inputs. */
+typedef struct {
+  int32_t numel;            /* cells (numelNov)                           */
+  int32_t nface;            /* faces: length of the face arrays          */
+  short maxViz;             /* face slots per cell                       */
+  short ndm;                /* 3                                          */
+  const short   *nFace;     /* [numel]                                   */
+  const int32_t *nelcon;    /* [numel*maxViz] 1-based, -1 boundary       */
+  const int32_t *cellFace;  /* [numel*maxViz] 1-based face index         */
+  const int32_t *owner;     /* [nface*2] 1-based owner cell first        */
+  const double  *volume;    /* [numel]                                   */
+  const double  *dcca;      /* [numel*maxViz]                            */
+  const double  *xmcc;      /* [numel*maxViz*3]                          */
+  const double  *mksi;      /* [nface]                                   */
+  const double  *ksi;       /* [nface*3]                                 */
+  const double  *area;      /* [nface]                                   */
+  const double  *normal;    /* [nface*3]                                 */
+  const double  *vSkew;     /* [nface*3]                                 */
+  const short   *mat;       /* [numel] 1-based material                  */
+  const short   *faceRvel;  /* [numel*(maxViz+1)]                        */
+  const short   *faceRpres; /* [numel*(maxViz+1)]                        */
+} sfr_cell_mesh;
+
+typedef struct {
+  short  type;              /* load type (Define.h)                      */
+  short  np;                /* parameters in par                         */
+  double par[8];
+  double density;           /* Loads.density                             */
+} sfr_load;
+
+typedef struct sfr_cells sfr_cells;
+int  sfr_cells_create(sfr_context *ctx, const sfr_cell_mesh *mesh,
+                      const double *prop, int nmat,
+                      const sfr_load *loadsVel, int nloadsVel,
+                      const sfr_load *loadsPres, int nloadsPres,
+                      sfr_cells **cells);
+int  sfr_cells_pres_device(sfr_context *ctx, sfr_cells *cells, int diffusion,
+                           const double *d_pres, const double *d_gradPres,
+                           const double *d_vel, const double *d_dField,
+                           double *d_lA, double *d_lB, double *d_rCell);
+void sfr_cells_destroy(sfr_context *ctx, sfr_cells *cells);
+/* The two field updates around SIMPLE's pressure-correction solve, on
+ * device arrays: sfr_cells_update_value is updateCellValueBlock
+ * (src/CellLoop.c:4087-4130, ndf 1, one process): u[c] = x[id[c]-1], or
+ * u[c] += x[id[c]-1] with add, for every cell with an equation (id > 0),
+ * others untouched.  sfr_simple_update is simpleUpdate
+ * (src/Simple.c:1615-1668, ndm 3): vel -= dField * gradPresC per component,
+ * pres += alphaPres * presC, in the reference's operation order. */
+int  sfr_cells_update_value(sfr_context *ctx, int numel, const int32_t *d_id,
+                            const double *d_x, double *d_u, int add);
+int  sfr_simple_update(sfr_context *ctx, int numel, double alphaPres,
+                       double *d_vel, double *d_pres, const double *d_presC,
+                       const double *d_gradPresC, const double *d_dField);
 
 /* Measurement hook: average device time of one launch of a loop kernel,
  * HIP events on the context stream.  which = 0: FP64 PCG SpMV + p.q;

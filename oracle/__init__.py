@@ -263,3 +263,108 @@ def pcg_omp(rows, ad, b, tol, maxit, nthreads=0):
     if rc:
         raise RuntimeError("orc_pcg_omp")
     return x, st.as_dict()
+
+
+# ---- the SIMPLE pressure-correction cell loop (cells.c) ---------------------
+
+def load_table(loads):
+    """Flat arrays (type, np, par[n*8], density) of a list of constant loads,
+    each a dict with "type", "par" (at most 8 values) and "density"."""
+    n = max(len(loads), 1)
+    t = np.zeros(n, np.int16)
+    npar = np.zeros(n, np.int16)
+    par = np.zeros(n * 8)
+    dens = np.zeros(n)
+    for i, ld in enumerate(loads):
+        t[i] = ld["type"]
+        p = list(ld.get("par", []))
+        npar[i] = len(p)
+        par[i * 8:i * 8 + len(p)] = p
+        dens[i] = ld.get("density", 0.0)
+    return t, npar, par, dens
+
+
+def _cells_lib():
+    lib = orc()
+    if not getattr(lib, "_cells_ready", False):
+        f = lib.orc_simple_pres3d
+        f.restype = C.c_int
+        f.argtypes = ([C.c_int, C.c_short, i16p, i32p, i32p, i32p, f64p, f64p,
+                       f64p, f64p, f64p, f64p, f64p, i16p, f64p, i16p, i16p]
+                      + [C.c_int, i16p, i16p, f64p, f64p] * 2
+                      + [C.c_short, f64p, f64p, f64p, f64p, f64p, f64p, f64p])
+        lib._cells_ready = True
+    return lib
+
+
+def simple_pres3d(mesh, case):
+    """lA (numel x (maxViz+1), the centre last), lB and rCell of the SIMPLE
+    pressure-correction cell loop (systFormSimplePres + cellSimplePres3D) for
+    a mesh from sfr3d_amd.systems.hex_mesh and a case (fields, materials,
+    boundary markers, loads, diffusion scheme).  Cells without an equation
+    keep lA = lB = 0."""
+    numel, mv = mesh["numel"], mesh["maxViz"]
+    lA = np.zeros(numel * (mv + 1))
+    lB = np.zeros(numel)
+    rc = np.zeros(numel)
+    lv = load_table(case["loadsVel"])
+    lp = load_table(case["loadsPres"])
+    r = _cells_lib().orc_simple_pres3d(
+        numel, mv, mesh["nFace"], mesh["nelcon"], mesh["cellFace"],
+        mesh["owner"], mesh["dcca"], mesh["xmcc"], mesh["mksi"], mesh["ksi"],
+        mesh["area"], mesh["normal"], mesh["vSkew"], case["mat"],
+        case["prop"], case["faceRvel"], case["faceRpres"],
+        len(case["loadsVel"]), *lv, len(case["loadsPres"]), *lp,
+        case["diffusion"], case["pres"], case["gradPres"], case["vel"],
+        case["dField"], lA, lB, rc)
+    if r:
+        raise ValueError("orc_simple_pres3d: diffusion scheme must be 1 or 4")
+    return lA.reshape(numel, mv + 1), lB, rc
+
+
+def ref_simple_pres(mesh, case, storage=2):
+    """The reference's own systFormSimplePres (src/CellLoopInc.c:471) on the
+    same mesh and case, assembled by its dataStruct / assbly in `storage`
+    (symmetric): (ia, ja, al, ad, b, rCell)."""
+    lib = ref()
+    if not getattr(lib, "_pres_ready", False):
+        f = lib.ref_simple_pres
+        f.restype = C.c_int
+        f.argtypes = ([C.c_int, C.c_short, i16p, i32p, i32p, i32p, f64p, f64p,
+                       f64p, f64p, f64p, f64p, f64p, f64p, f64p, f64p, f64p,
+                       i16p, f64p, C.c_int, i16p, i16p, i32p]
+                      + [C.c_int, i16p, i16p, f64p, f64p] * 2
+                      + [C.c_short, f64p, f64p, f64p, f64p, f64p, f64p, f64p,
+                         f64p, C.c_long])
+        lib._pres_ready = True
+    numel, mv = mesh["numel"], mesh["maxViz"]
+    id_ = np.ascontiguousarray(case["id"], np.int32)
+    num = np.ascontiguousarray(case["num"], np.int32)
+    neq = int(id_.max())
+    out = np.zeros(3, np.int64)
+    lib.ref_init(1 << 31)          # the arena size the other harness users set
+    lib.ref_datastruct(storage, 0, id_, num, mesh["nelcon"], mesh["nFace"],
+                       numel, mv, neq, out)
+    ia = np.zeros(out[0], np.int32)
+    ja = np.zeros(out[1], np.int32)
+    lib.ref_datastruct_copy(ia, ja, out[0], out[1])
+    al = np.zeros(out[2])
+    nrow = neq + 1 if storage == 4 else neq
+    ad = np.zeros(nrow)
+    b = np.zeros(nrow)
+    rc = np.zeros(numel)
+    lv = load_table(case["loadsVel"])
+    lp = load_table(case["loadsPres"])
+    nmat = len(case["prop"]) // 10
+    r = lib.ref_simple_pres(
+        numel, mv, mesh["nFace"], mesh["nelcon"], mesh["cellFace"],
+        mesh["owner"], mesh["volume"], mesh["dcca"], mesh["xmcc"],
+        mesh["mksi"], mesh["ksi"], mesh["eta"], mesh["area"], mesh["normal"],
+        mesh["xm"], mesh["mvSkew"], mesh["vSkew"], case["mat"], case["prop"],
+        nmat, case["faceRvel"], case["faceRpres"], id_,
+        len(case["loadsVel"]), *lv, len(case["loadsPres"]), *lp,
+        case["diffusion"], case["pres"], case["gradPres"], case["vel"],
+        case["dField"], al, ad, b, rc, int(out[2]))
+    if r:
+        raise RuntimeError("ref_simple_pres")
+    return ia, ja, al, ad, b, rc

@@ -10,6 +10,8 @@
  *   solverC        src/Solv.c:64 (PCG / PBICGSTAB, serial, log line captured;
  *                  with -fopenmp also its OpenMP solvers, ref_solve_omp)
  *   genrcm         src/Rcm.c:56            matvecs     src/HccaBlas.c
+ *   systFormSimplePres  src/CellLoopInc.c:471 (the SIMPLE pressure-correction
+ *                  cell loop, with cellSimplePres3D, src/CellPres.c:903)
  * Used only to pin the oracle and to generate golden vectors.
  */
 #include <stdio.h>
@@ -19,6 +21,7 @@
 #include <Reord.h>
 #include <Sisteq.h>
 #include <Assbly.h>
+#include <CellLoop.h>
 
 static Memoria g_mem;
 static int g_init;
@@ -263,5 +266,81 @@ int ref_matvec(short storage, int unsym, int neq, int *ia, int *ja,
   setMatVec(&mv, storage, unsym != 0, false);
   ompVar.fSolver = o;
   mv(neq, ia, ja, al, ad, x, y);
+  return 0;
+}
+
+/* Constant loads from flat arrays: type[i], np[i], par[i*8 + k], density[i]. */
+static void fill_loads(Loads *ld, int n, const short *type, const short *np,
+                       const double *par, const double *density)
+{
+  memset(ld, 0, sizeof(Loads) * MAXLOAD);
+  for (int i = 0; i < n && i < MAXLOAD; i++) {
+    ld[i].fUse = true;
+    ld[i].nTypeVar = LVARCONST;
+    ld[i].type = type[i];
+    ld[i].np = np[i];
+    for (int k = 0; k < 8; k++) ld[i].par[k] = par[i * 8 + k];
+    ld[i].density = density[i];
+  }
+}
+
+/* The reference's SIMPLE pressure-correction cell loop, systFormSimplePres
+ * (src/CellLoopInc.c:471-679), unchanged, assembling into the structure held
+ * by ref_datastruct (which must have been built for these cells).  Mesh and
+ * field arrays in the reference's layout (see Simple.c:3053-3080 for which
+ * Mesh member each one is); ndm = 3, calType 1 for every material.  Writes
+ * al (nal), ad and b (neq) and rCell (numel). */
+int ref_simple_pres(int numel, short maxViz, short *nFace, int *nelcon,
+                    int *cellFace, int *fOwner, double *volume, double *dcca,
+                    double *xmcc, double *mksi, double *ksi, double *eta,
+                    double *area, double *normal, double *xm, double *mvSkew,
+                    double *vSkew, short *mat, double *prop, int nmat,
+                    short *faceRvel, short *faceRpres, int *id,
+                    int nlv, short *lv_type, short *lv_np, double *lv_par,
+                    double *lv_density,
+                    int nlp, short *lp_type, short *lp_np, double *lp_par,
+                    double *lp_density,
+                    short diffusion, double *pres, double *gradPres,
+                    double *vel, double *dField, double *al, double *ad,
+                    double *b, double *rCell, long nal)
+{
+  if (!g_have_sys || nmat > MAXMAT) return -1;
+  /* cellSimplePres3D reads the velocity loads from the global loadsVel
+   * (src/CellPres.c:1119, `&loadsVel[nCarg]`), whatever table its caller
+   * passes; Simple.c:3053 passes that same global */
+  static Loads lpres[MAXLOAD];
+  fill_loads(loadsVel, nlv, lv_type, lv_np, lv_par, lv_density);
+  fill_loads(lpres, nlp, lp_type, lp_np, lp_par, lp_density);
+  Loads *lvel = loadsVel;
+  Diffusion dif = {diffusion};
+  Turbulence tm_;
+  memset(&tm_, 0, sizeof tm_);
+  Temporal ddt;
+  memset(&ddt, 0, sizeof ddt);
+  short *calType = calloc((size_t)nmat, sizeof(short));
+  short *geomType = malloc(sizeof(short) * (size_t)numel);
+  short *nen = malloc(sizeof(short) * (size_t)numel);
+  double *wallPar = calloc((size_t)numel * NWALLPAR, sizeof(double));
+  double *gCc = calloc((size_t)numel * 3, sizeof(double));
+  for (int m = 0; m < nmat; m++) calType[m] = 1;
+  for (int c = 0; c < numel; c++) { geomType[c] = HEXACELL; nen[c] = 8; }
+  const int neq = g_sys.neq;
+  zero(g_sys.al, nal, DOUBLEC);
+  zero(g_sys.ad, neq, DOUBLEC);
+  double *bb = calloc((size_t)neq + 1, sizeof(double));
+  ompVar.fCell = false;
+  ompVar.nThreadsCell = 1;
+  systFormSimplePres(lvel, lpres, &dif, &tm_, NULL, nelcon, nen, nFace,
+                     cellFace, fOwner, volume, dcca, xmcc, gCc, mksi, ksi,
+                     eta, area, normal, xm, mvSkew, vSkew, geomType, prop,
+                     calType, mat, g_sys.ia, g_sys.ja, g_sys.al, g_sys.ad,
+                     bb, id, faceRvel, faceRpres, pres, gradPres, vel,
+                     dField, wallPar, rCell, &ddt, neq, neq, g_sys.nad, 0,
+                     8, maxViz, 3, numel, 1, g_sys.storage, true, true, true,
+                     g_sys.unsym);
+  memcpy(al, g_sys.al, sizeof(double) * (size_t)nal);
+  memcpy(ad, g_sys.ad, sizeof(double) * (size_t)neq);
+  memcpy(b, bb, sizeof(double) * (size_t)neq);
+  free(bb); free(calType); free(geomType); free(nen); free(wallPar); free(gCc);
   return 0;
 }

@@ -74,6 +74,20 @@ class SfrMtx(C.Structure):
                 ("al", C.POINTER(C.c_double)), ("ad", C.POINTER(C.c_double))]
 
 
+class SfrCellMesh(C.Structure):
+    _fields_ = [("numel", C.c_int32), ("nface", C.c_int32),
+                ("maxViz", C.c_short), ("ndm", C.c_short)] + [
+        (k, C.c_void_p) for k in ("nFace", "nelcon", "cellFace", "owner",
+                                  "volume", "dcca", "xmcc", "mksi", "ksi",
+                                  "area", "normal", "vSkew", "mat",
+                                  "faceRvel", "faceRpres")]
+
+
+class SfrLoad(C.Structure):
+    _fields_ = [("type", C.c_short), ("np", C.c_short),
+                ("par", C.c_double * 8), ("density", C.c_double)]
+
+
 def _sig(name, res, *args):
     f = getattr(lib, name)
     f.restype = res
@@ -124,6 +138,14 @@ _sig("sfr_assembly_create", C.c_int, vp, vp, C.POINTER(SfrPattern), i32p,
      i32p, i16p, C.c_int, C.c_int, C.POINTER(vp))
 _sig("sfr_assemble_device", C.c_int, vp, vp, vp, vp, vp)
 _sig("sfr_assembly_destroy", None, vp, vp)
+# cell coefficients on the device
+_sig("sfr_cells_create", C.c_int, vp, C.POINTER(SfrCellMesh), f64p, C.c_int,
+     C.POINTER(SfrLoad), C.c_int, C.POINTER(SfrLoad), C.c_int, C.POINTER(vp))
+_sig("sfr_cells_pres_device", C.c_int, vp, vp, C.c_int, vp, vp, vp, vp, vp,
+     vp, vp)
+_sig("sfr_cells_destroy", None, vp, vp)
+_sig("sfr_cells_update_value", C.c_int, vp, C.c_int, vp, vp, vp, C.c_int)
+_sig("sfr_simple_update", C.c_int, vp, C.c_int, C.c_double, vp, vp, vp, vp, vp)
 # uneven batch
 _sig("sfr_batch_create", C.c_int, vp, C.c_int, C.POINTER(SfrSystem), C.c_int,
      C.POINTER(vp))

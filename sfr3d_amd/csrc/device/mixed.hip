@@ -371,12 +371,26 @@ extern "C" int sfr_k_mx_fold(hipStream_t s, const sfr_mat_view *A,
   return hipGetLastError();
 }
 
+// MB's launch shape.  MB streams two words a row and gathers nothing, so
+// what bounds it is the loads a thread has in flight: a grid-stride pass
+// issues its rows' loads only after the previous pass's arrive.  MB
+// therefore takes the fewest passes the workgroup cap allows (up to 8 rows
+// per thread) whatever rows per thread MA runs; its partial count is MA's
+// `gin`.
+static inline sfr_geom mb_geom(int64_t n, sfr_geom g0)
+{
+  const int64_t cap = (g0.cap > 0 && g0.cap < SFR_GRIDCAP) ? g0.cap : SFR_GRIDCAP;
+  int rpt = g0.rpt;
+  while (rpt < 8 && (int64_t)rpt * SFR_BLOCK * cap < n) rpt *= 2;
+  return {rpt, grid_for(n, rpt, 1, g0.cap)};
+}
+
 extern "C" int sfr_k_mx_inner_iter(hipStream_t s, const sfr_mat_view *A,
                                    sfr_geom g0, float *y, float *r,
                                    float *const pb[2], float *q, int par,
                                    sfr_scalars *sc, double *part)
 {
-  const sfr_geom gv = {g0.rpt, grid_for(A->n, g0.rpt, 1, g0.cap)};
+  const sfr_geom gb = mb_geom(A->n, g0);
   const sfr_geom gm = {g0.rpt, grid_for(A->n, g0.rpt, op_lanes(A), g0.cap)};
   double *ra = REGION(part, 0), *rb = REGION(part, 1);
   const float *po = pb[par];
@@ -384,9 +398,9 @@ extern "C" int sfr_k_mx_inner_iter(hipStream_t s, const sfr_mat_view *A,
   with_op32(A, [&](auto op) {
     using Op = decltype(op);
     SFR_LAUNCH_RPT(k_mx_a, Op, gm, s, op, (const float *)r, po, pn, q, y,
-                   (const double *)rb, gv.grid, ra, par, sc);
+                   (const double *)rb, gb.grid, ra, par, sc);
   });
-  SFR_LAUNCH_V(k_mx_b, gv, s, A->n, (const float *)q, r, (const double *)ra,
+  SFR_LAUNCH_V(k_mx_b, gb, s, A->n, (const float *)q, r, (const double *)ra,
                gm.grid, rb, par, sc);
   return hipGetLastError();
 }

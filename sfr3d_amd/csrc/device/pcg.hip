@@ -474,11 +474,18 @@ extern "C" int sfr_k_pcg_setup(hipStream_t s, const sfr_mat_view *A,
   return hipGetLastError();
 }
 
+// The two streaming kernels of an iteration take the fewest passes a grid
+// of at most 1,024 workgroups allows (up to 8 rows per thread), whatever
+// rows per thread the SpMV runs: a grid-stride pass issues its loads only
+// after the previous pass's arrive, so one pass keeps the most loads in
+// flight.  The r-update's partial count is the direction kernel's `gin`.
 extern "C" int sfr_k_pcg_iter(hipStream_t s, const sfr_mat_view *A,
                               sfr_geom g0, double *x, double *r, double *p,
                               double *q, sfr_scalars *sc, double *part)
 {
-  const sfr_geom gv = {g0.rpt, grid_for(A->n, g0.rpt, 1)};
+  int rv = g0.rpt;
+  while (rv < 8 && (int64_t)rv * SFR_BLOCK * 1024 < A->n) rv *= 2;
+  const sfr_geom gv = {rv, grid_for(A->n, rv, 1)};
   const sfr_geom gm = {g0.rpt, grid_for(A->n, g0.rpt, op_lanes(A))};
   double *ra = REGION(part, 0), *rb = REGION(part, 1);
   with_op64(A, [&](auto op) {

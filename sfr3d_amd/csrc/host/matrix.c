@@ -725,7 +725,7 @@ int sfr_matrix_rows_per_thread(const sfr_matrix *A, int *rpt, int *rpt_pcg)
 {
   if (!A) return SFR_BAD_ARG;
   if (rpt) *rpt = A->rpt;
-  if (rpt_pcg) *rpt_pcg = A->rpt_pcg ? A->rpt_pcg : A->rpt;
+  if (rpt_pcg) *rpt_pcg = sfr_pcg_rpt(A);
   return SFR_OK;
 }
 

@@ -366,7 +366,7 @@ static int solve_pcg64(sfr_context *ctx, sfr_matrix *A, double tol,
 {
   CHECK(sfr_workspace(ctx, A->n));
   sfr_mat_view v = sfr_view(A);
-  sfr_geom g = {A->rpt_pcg ? A->rpt_pcg : A->rpt, 0, 0};
+  sfr_geom g = {sfr_pcg_rpt(A), 0, 0};
   chunks ch;
   CHECK(get_chunks(ctx, G_PCG, A, d_x, NULL, &v, g, &ch));
   TRY(hipEventRecord(ctx->ev_start, ctx->stream));
@@ -997,7 +997,7 @@ int sfr_solve_device_noalloc(sfr_context *ctx, sfr_matrix *A, int solver,
   memset(&d, 0, sizeof d);
   d.solver = solver;
   d.v = sfr_view(A);
-  d.g = (sfr_geom){(solver == SFR_PCG && A->rpt_pcg) ? A->rpt_pcg : A->rpt, 0, 0};
+  d.g = (sfr_geom){solver == SFR_PCG ? sfr_pcg_rpt(A) : A->rpt, 0, 0};
   d.x = d_x;
   d.sc = (sfr_scalars *)base;
   d.part = (double *)(base + SFR_SCRATCH_PART);
@@ -1028,7 +1028,7 @@ int sfr_kernel_time(sfr_context *ctx, sfr_matrix *A, int which, int reps,
   CHECK(sfr_workspace(ctx, A->n));
   if (which == 1) CHECK(sfr_matrix_prepare_mixed(ctx, A));
   sfr_mat_view v = sfr_view(A);
-  sfr_geom g = {(which == 0 && A->rpt_pcg) ? A->rpt_pcg : A->rpt, 0, 0};
+  sfr_geom g = {which == 0 ? sfr_pcg_rpt(A) : A->rpt, 0, 0};
   if (which == 1) g = mixed_geom(A);
   hipStream_t s = ctx->stream;
   float *const *w32 = ctx->w32;

@@ -221,8 +221,36 @@ int sfr_k_final(hipStream_t s, const sfr_mat_view *A, sfr_geom g,
                 const double *b, const double *x, sfr_scalars *sc,
                 double *part);
 
+/* SIMPLE pressure-correction cell loop, one work-item per cell (the
+ * per-slot planes of host/cells.c) */
+int sfr_k_cell_pres(hipStream_t s, int numel, int maxViz, int diffusion,
+                    const signed char *nface, const int32_t *nb,
+                    const int32_t *bc, const double *g, const double *rho,
+                    const sfr_load *lv, const sfr_load *lp, const double *pres,
+                    const double *gp, const double *vel, const double *df,
+                    double *lA, double *lB, double *rc);
+
+/* u[c] (+)= x[id[c]-1] for the cells with an equation */
+int sfr_k_cell_value(hipStream_t s, int64_t n, const int32_t *id,
+                     const double *x, double *u, int add);
+/* w -= d grad(p'), p += alpha p' (ndm 3) */
+int sfr_k_simple_update(hipStream_t s, int64_t n, double alpha, double *w,
+                        double *p, const double *pc, const double *gpc,
+                        const double *d);
+
 /* measured rows per thread of the FP64 PCG loop -> A->rpt_pcg */
 int sfr_tune_pcg(sfr_context *ctx, sfr_matrix *A);
+/* Rows per thread of the FP64 PCG loop: the measured choice when there is
+ * one; else 2 from 2^21 rows up unless the caller fixed the shape (the
+ * 128^3 and 160^3 Poisson loops, whose FP64 matrices sit at the edge of the
+ * 256 MB last-level cache, run fastest there, and it is the tuner's pick at
+ * both); else the matrix's default. */
+static inline int sfr_pcg_rpt(const sfr_matrix *A)
+{
+  if (A->rpt_pcg) return A->rpt_pcg;
+  if (!A->rpt_set && A->n >= ((int64_t)1 << 21)) return 2;
+  return A->rpt;
+}
 
 /* GMRES(SFR_GM_K), FP64 (reference src/Iters.c:587-790) */
 int sfr_gm_workspace(sfr_context *ctx, int64_t n);

@@ -132,6 +132,62 @@ class Assembly:
             self.h = None
 
 
+def load_array(loads):
+    """ctypes array of SfrLoad from a list of dicts with "type", "par" (at
+    most 8 values) and "density" (SFR3D's constant Loads)."""
+    arr = (L.SfrLoad * max(len(loads), 1))()
+    for i, ld in enumerate(loads):
+        par = list(ld.get("par", []))
+        arr[i].type = ld["type"]
+        arr[i].np = len(par)
+        for k, v in enumerate(par):
+            arr[i].par[k] = v
+        arr[i].density = ld.get("density", 0.0)
+    return arr
+
+
+class Cells:
+    """Cell coefficients on the device (sfr_cells_*): the mesh geometry of
+    SFR3D's Mesh (a dict as systems.hex_mesh makes), material properties
+    and load tables uploaded once; pres() runs the SIMPLE pressure-correction
+    cell loop on device fields into lA / lB / rCell."""
+
+    _MESH = ("nFace", "nelcon", "cellFace", "owner", "volume", "dcca", "xmcc",
+             "mksi", "ksi", "area", "normal", "vSkew")
+
+    def __init__(self, ctx, mesh, prop, mat, faceRvel, faceRpres, loadsVel,
+                 loadsPres):
+        self.ctx = ctx
+        keep = {k: np.ascontiguousarray(mesh[k]) for k in self._MESH}
+        keep["mat"] = np.ascontiguousarray(mat, np.int16)
+        keep["faceRvel"] = np.ascontiguousarray(faceRvel, np.int16)
+        keep["faceRpres"] = np.ascontiguousarray(faceRpres, np.int16)
+        m = L.SfrCellMesh(numel=mesh["numel"], nface=mesh["nface"],
+                          maxViz=mesh["maxViz"], ndm=3)
+        for k, v in keep.items():
+            setattr(m, k, v.ctypes.data)
+        prop = np.ascontiguousarray(prop, np.float64)
+        lv, lp = load_array(loadsVel), load_array(loadsPres)
+        h = C.c_void_p()
+        ctx._check(lib.sfr_cells_create(ctx.h, C.byref(m), prop,
+                                        len(prop) // 10, lv, len(loadsVel),
+                                        lp, len(loadsPres), C.byref(h)))
+        self.h = h.value
+        self.numel, self.maxViz = mesh["numel"], mesh["maxViz"]
+
+    def pres(self, diffusion, d_pres, d_gradPres, d_vel, d_dField, d_lA, d_lB,
+             d_rCell=None):
+        self.ctx._check(lib.sfr_cells_pres_device(
+            self.ctx.h, self.h, diffusion, d_pres.ptr, d_gradPres.ptr,
+            d_vel.ptr, d_dField.ptr, d_lA.ptr, d_lB.ptr,
+            d_rCell.ptr if d_rCell else None))
+
+    def destroy(self):
+        if self.h:
+            lib.sfr_cells_destroy(self.ctx.h, self.h)
+            self.h = None
+
+
 class Batch:
     """Uneven batch of independent systems (sfr_batch_*): mixed-precision
     PCG over all of them in one stream of launches."""
@@ -250,6 +306,23 @@ class Context:
 
     def batch(self, systems, tune=True):
         return Batch(self, systems, tune)
+
+    def cells_update_value(self, d_id, d_x, d_u, add=False):
+        """u[c] = x[id[c]-1] (or +=) on the device, cells with an equation."""
+        self._check(lib.sfr_cells_update_value(self.h, d_u.nbytes // 8, d_id.ptr,
+                                               d_x.ptr, d_u.ptr, 1 if add else 0))
+
+    def simple_update(self, alpha, d_vel, d_pres, d_presC, d_gradPresC, d_dField):
+        """vel -= dField * gradPresC, pres += alpha * presC on the device."""
+        self._check(lib.sfr_simple_update(self.h, d_pres.nbytes // 8, alpha,
+                                          d_vel.ptr, d_pres.ptr, d_presC.ptr,
+                                          d_gradPresC.ptr, d_dField.ptr))
+
+    def cells(self, mesh, case):
+        """Cells for a mesh from systems.hex_mesh and a case from
+        systems.simple_pres_case (materials, markers and loads)."""
+        return Cells(self, mesh, case["prop"], case["mat"], case["faceRvel"],
+                     case["faceRpres"], case["loadsVel"], case["loadsPres"])
 
     def solve(self, system, solver, tol, maxit, b, x=None, newX=True):
         """Host-array solve (sfr_solve).  Returns (x, stats dict)."""

@@ -481,3 +481,196 @@ def _pressure_csr(sysm):
     rows = np.repeat(np.arange(n), np.diff(sysm.ia))
     L = sp.csr_matrix((sysm.al, (rows, sysm.ja)), shape=(n, n))
     return L + L.T + sp.diags(sysm.ad[:n])
+
+
+# ---- hex meshes with SFR3D's geometry arrays --------------------------------
+
+HEX_FACE_NODES = np.array([[0, 3, 2, 1], [4, 5, 6, 7], [0, 1, 5, 4],
+                           [1, 2, 6, 5], [2, 3, 7, 6], [3, 0, 4, 7]])
+HEX_OPPOSITE = np.array([1, 0, 4, 5, 2, 3])      # z-/z+, y-/y+, x+/x-
+
+
+def _graded(n, ratio):
+    """n+1 node coordinates on [0, 1] with a geometric cell-size ratio."""
+    if ratio == 1.0:
+        return np.linspace(0.0, 1.0, n + 1)
+    w = ratio ** np.arange(n)
+    return np.concatenate([[0.0], np.cumsum(w) / w.sum()])
+
+
+def hex_mesh(nx, ny, nz, ratio=(1.0, 1.0, 1.0), shear=(0.0, 0.0, 0.0),
+             warp=0.0):
+    """An nx*ny*nz hex mesh of the unit box, graded by `ratio` per axis,
+    sheared by x += s0*y + s1*z, y += s2*z (faces stay planar) and warped by
+    x += w sin(pi y) sin(pi z), y += w sin(pi x) sin(pi z) (interior faces
+    bend, and the line between two centroids misses the face's midpoint),
+    with the
+    arrays SFR3D's Mesh holds for the finite-volume cell loops, in its layout
+    and 1-based conventions:
+
+      nelcon [numel*6]  neighbour across each face (faces z-, z+, y-, x+, y+,
+                        x-; -1 on the boundary), nFace [numel] = 6
+      cellFace [numel*6], owner [nface*2]  face of each cell slot; owner cell
+                        first, the other cell second (-1 on the boundary)
+      volume [numel], dcca [numel*6], xmcc [numel*6*3], cc [numel*3]
+      mksi, area [nface], ksi, normal, xm, vSkew, eta [nface*3], mvSkew
+                        face quantities as the owner sees them
+
+    The definitions follow cellGeom3D (reference src/CellLib.c:1901-2290):
+    the centroid is the mean of the nodes; a face's normal is the normalised
+    mean of its four corner cross products and its area the length of that
+    mean; xm is the mean of its nodes, xmcc = xm - cc, dcca = xmcc . normal;
+    ksi is the unit vector to the neighbour's centroid (to xm on the
+    boundary) and mksi that distance; vSkew = xi - xm, xi the point where the
+    line along ksi meets the face (vectorKm3d, src/CellLib.c:4247-4310); the
+    volume is the Green-Gauss sum over faces.  eta holds the face's first
+    edge (the reference's per-face eta is not used by the cell loops here)."""
+    gx, gy, gz = (_graded(n, r) for n, r in zip((nx, ny, nz), ratio))
+    Z, Y, X = np.meshgrid(gz, gy, gx, indexing="ij")
+    px = X + shear[0] * Y + shear[1] * Z + warp * np.sin(np.pi * Y) * np.sin(np.pi * Z)
+    py = Y + shear[2] * Z + warp * np.sin(np.pi * X) * np.sin(np.pi * Z)
+    P = np.stack([px, py, Z], axis=-1)              # (nz+1, ny+1, nx+1, 3)
+    numel = nx * ny * nz
+    # node o of every cell: offsets (di, dj, dk) of the reference's hex order
+    off = [(0, 0, 0), (1, 0, 0), (1, 1, 0), (0, 1, 0),
+           (0, 0, 1), (1, 0, 1), (1, 1, 1), (0, 1, 1)]
+    cn = np.stack([P[dk:dk + nz, dj:dj + ny, di:di + nx].reshape(numel, 3)
+                   for di, dj, dk in off], axis=1)  # (numel, 8, 3)
+    cc = cn.sum(axis=1) * (1.0 / 8.0)
+    nelcon, nviz = S.hex_adjacency(nx, ny, nz)
+    nelcon = np.ascontiguousarray(nelcon, np.int32)
+    nb = nelcon - 1                                    # -2 on the boundary
+    normal = np.empty((numel, 6, 3))
+    area = np.empty((numel, 6))
+    xm = np.empty((numel, 6, 3))
+    eta = np.empty((numel, 6, 3))
+    for f in range(6):
+        x1, x2, x3, x4 = (cn[:, HEX_FACE_NODES[f, q]] for q in range(4))
+        v1, v2, v3, v4 = x2 - x1, x3 - x2, x4 - x3, x1 - x4
+        nv = 0.25 * (np.cross(v4, v1) + np.cross(v1, v2) + np.cross(v2, v3)
+                     + np.cross(v3, v4))
+        a = np.sqrt((nv * nv).sum(axis=1))
+        normal[:, f] = nv / a[:, None]
+        area[:, f] = a
+        xm[:, f] = 0.25 * (x1 + x2 + x3 + x4)
+        eta[:, f] = v1
+    xmcc = xm - cc[:, None, :]
+    dcca = (xmcc * normal).sum(axis=2)
+    bnd = nb < 0
+    cnb = np.where(bnd[..., None], xm, cc[np.where(bnd, 0, nb)])
+    kv = np.where(bnd[..., None], xmcc, cnb - cc[:, None, :])
+    mksi = np.sqrt((kv * kv).sum(axis=2))
+    ksi = kv / mksi[..., None]
+    t = (normal * xmcc).sum(axis=2) / (normal * ksi).sum(axis=2)
+    xi = cc[:, None, :] + t[..., None] * ksi
+    vskew = xi - xm
+    mvskew = np.sqrt((vskew * vskew).sum(axis=2))
+    volume = ((xm * normal).sum(axis=2) * area).sum(axis=1) / 3.0
+    # faces in cell order: a slot owns its face on the boundary or when the
+    # neighbour comes later
+    cells = np.arange(numel)[:, None]
+    own = bnd | (nb > cells)
+    fid = np.full((numel, 6), -1, np.int64)
+    fid[own] = np.arange(int(own.sum()))
+    oc, of = np.nonzero(~own)
+    fid[oc, of] = fid[nb[oc, of], HEX_OPPOSITE[of]]
+    nface = int(own.sum())
+    ocell, oslot = np.nonzero(own)                    # in face order
+    owner = np.empty((nface, 2), np.int32)
+    owner[:, 0] = ocell + 1
+    owner[:, 1] = np.where(bnd[ocell, oslot], -1, nb[ocell, oslot] + 1)
+
+    def fsel(a):
+        return np.ascontiguousarray(a[ocell, oslot])
+
+    return {
+        "nx": nx, "ny": ny, "nz": nz, "numel": numel, "nface": nface,
+        "maxViz": 6, "nelcon": nelcon.reshape(-1),
+        "nFace": np.ascontiguousarray(nviz, np.int16),
+        "cellFace": np.ascontiguousarray((fid + 1).reshape(-1), np.int32),
+        "owner": owner.reshape(-1), "volume": volume, "cc": cc.reshape(-1),
+        "dcca": np.ascontiguousarray(dcca.reshape(-1)),
+        "xmcc": np.ascontiguousarray(xmcc.reshape(-1)),
+        "mksi": fsel(mksi), "ksi": fsel(ksi).reshape(-1),
+        "area": fsel(area), "normal": fsel(normal).reshape(-1),
+        "xm": fsel(xm).reshape(-1), "vSkew": fsel(vskew).reshape(-1),
+        "mvSkew": fsel(mvskew), "eta": fsel(eta).reshape(-1),
+    }
+
+
+# SFR3D load types (reference include/Define.h:317-331)
+DIRICHLETBC, INLET, OUTLET, MOVEWALL = 1, 6, 7, 8
+INLETSTAICTPRES, OPEN, INLETTOTALPRES, STATICWALL, FLUXPRES = 9, 10, 13, 14, 15
+
+
+def simple_pres_case(mesh, seed=1, diffusion=1, boundary="mixed",
+                     skip_cell=None):
+    """Fields, materials, boundary markers and loads for the SIMPLE
+    pressure-correction cell loop on a hex_mesh, as SFR3D holds them:
+    mat (1-based), prop (10 per material, density at index 1), faceRvel /
+    faceRpres (maxViz+1 per cell, 1-based load index or 0), the load tables,
+    pres, gradPres, vel and the SIMPLE d field (positive), equation numbers
+    id (all cells, natural order) and num.
+
+    boundary "cavity": a moving lid (z+) and static walls, no pressure
+    loads.  "mixed": every load type the pressure cell reads: z+ moving wall;
+    z- static-pressure inlet and total-pressure inlet in alternate cells;
+    x- velocity inlet; x+ outlet with prescribed pressure; y- static wall
+    with a pressure-flux load; y+ open boundary (flow both ways).
+    skip_cell: a cell marked as having no pressure equation."""
+    nx, ny, nz = mesh["nx"], mesh["ny"], mesh["nz"]
+    numel, mv = mesh["numel"], mesh["maxViz"]
+    c = np.arange(numel)
+    i, j, k = c % nx, (c // nx) % ny, c // (nx * ny)
+    mat = np.where(k < max(nz // 2, 1), 1, 2).astype(np.int16)
+    prop = np.zeros(20)
+    prop[1], prop[11] = 1.0, 1.25
+    prop[0], prop[10] = 1e-3, 2e-3
+    u = lambda s, n: u01(seed, s, np.arange(n))
+    pres = 2.0 * u(1, numel) - 1.0
+    gradPres = 2.0 * u(2, 3 * numel) - 1.0
+    vel = 2.0 * u(3, 3 * numel) - 1.0
+    dField = 1e-3 * (0.5 + u(4, 3 * numel))
+    fv = np.zeros((numel, mv + 1), np.int16)
+    fp = np.zeros((numel, mv + 1), np.int16)
+    # faces z-, z+, y-, x+, y+, x- (boundary where the cell is at the side)
+    side = [k == 0, k == nz - 1, j == 0, i == nx - 1, j == ny - 1, i == 0]
+    if boundary == "cavity":
+        loadsVel = [{"type": MOVEWALL, "par": [1.0, 0.0, 0.0, 0.0]},
+                    {"type": STATICWALL}]
+        loadsPres = []
+        for f in range(6):
+            fv[side[f], f] = 1 if f == 1 else 2
+    else:
+        loadsVel = [{"type": MOVEWALL, "par": [1.0, 0.0, 0.0, 0.0]},
+                    {"type": STATICWALL},
+                    {"type": INLETSTAICTPRES},
+                    {"type": INLETTOTALPRES, "par": [1.1, 2.0, 0.0, 0.0, 1.0]},
+                    {"type": INLET, "par": [0.8, 0.1, -0.05, 1.05]},
+                    {"type": OUTLET},
+                    {"type": OPEN, "par": [0.3], "density": 1.15}]
+        loadsPres = [{"type": DIRICHLETBC, "par": [0.5]},
+                     {"type": INLETTOTALPRES, "par": [1.1, 2.0, 0.0, 0.0, 1.0]},
+                     {"type": FLUXPRES},
+                     {"type": OPEN, "par": [0.3]}]
+        fv[side[1], 1] = 1
+        zm = side[0]
+        fv[zm & (i % 2 == 0), 0] = 3
+        fv[zm & (i % 2 == 1), 0] = 4
+        fp[zm & (i % 2 == 1), 0] = 2
+        fv[side[5], 5] = 5
+        fv[side[3], 3] = 6
+        fp[side[3], 3] = 1
+        fv[side[2], 2] = 2
+        fp[side[2], 2] = 3
+        fv[side[4], 4] = 7
+        fp[side[4], 4] = 4
+    if skip_cell is not None:
+        fp[skip_cell, mesh["nFace"][skip_cell]] = 1
+    return {"mat": mat, "prop": prop, "pres": pres, "gradPres": gradPres,
+            "vel": vel, "dField": dField,
+            "faceRvel": np.ascontiguousarray(fv.reshape(-1)),
+            "faceRpres": np.ascontiguousarray(fp.reshape(-1)),
+            "loadsVel": loadsVel, "loadsPres": loadsPres,
+            "diffusion": diffusion,
+            "id": (c + 1).astype(np.int32), "num": (c + 1).astype(np.int32)}

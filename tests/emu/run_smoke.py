@@ -132,7 +132,7 @@ s, maxit = mk("pois"); A = vp(); assert lib.sfr_matrix_create(ctx, C.byref(s), 0
 lib.sfr_matrix_tune.argtypes = [vp, vp, C.c_int, C.c_int]
 b = np.ascontiguousarray(z["pois_b"]); xo = z["pois_xo"]
 ok, its = True, []
-for rpt in (8, 1, 8):
+for rpt in (8, 2, 1, 8):
     ok &= lib.sfr_matrix_tune(ctx, A, rpt, 4) == 0
     for solver, prec in ((1, 0), (1, 1)):
         x = np.zeros(s.neq); st = St()
@@ -142,6 +142,31 @@ for rpt in (8, 1, 8):
         its.append(st.iters)
 check(f"retune it {its}", ok)
 lib.sfr_matrix_destroy(ctx, A)
+# the benchmark's solver-id sequence in one context: BiCGSTAB, BiCGSTAB(2),
+# GMRES (its workspace grows and the cached loop graphs are dropped) on the
+# momentum matrix, then PCG and MINRES on the Poisson matrix
+ok, seq = True, []
+sm, mm = mk("mom"); Am = vp(); assert lib.sfr_matrix_create(ctx, C.byref(sm), 0, C.byref(Am)) == 0
+bm = np.ascontiguousarray(z["mom_b"])
+for solver, tol, xo_ in ((2, 1e-8, z["mom_xo"]), (3, 1e-10, z["mom_b2_xo"]),
+                         (4, 1e-10, z["mom_gm_xo"])):
+    x = np.zeros(sm.neq); st = St()
+    rhs = bm[1] if solver == 3 else bm[0]
+    rc = lib.sfr_solve_device(ctx, Am, solver, 0, tol, mm, rhs.ctypes.data, x.ctypes.data,
+                              1, C.byref(st))
+    ok &= rc == 0 and rel(x, xo_) < 1e-6
+    seq.append(int(st.iters))
+sp_, mp_ = mk("pois"); Ap = vp(); assert lib.sfr_matrix_create(ctx, C.byref(sp_), 0, C.byref(Ap)) == 0
+bp = np.ascontiguousarray(z["pois_b"])
+for solver, xo_ in ((1, z["pois_xo"]), (5, z["pois_mr_xo"])):
+    x = np.zeros(sp_.neq); st = St()
+    rc = lib.sfr_solve_device(ctx, Ap, solver, 0, 1e-8, mp_, bp.ctypes.data, x.ctypes.data,
+                              1, C.byref(st))
+    ok &= rc == 0 and rel(x, xo_) < 1e-6
+    seq.append(int(st.iters))
+check(f"solver-id sequence it {seq}", ok)
+lib.sfr_matrix_destroy(ctx, Am)
+lib.sfr_matrix_destroy(ctx, Ap)
 # device assembly vs host assembly + set_values: bitwise the same solve
 class Pat(C.Structure):
     _fields_ = [("neq", C.c_int32), ("nad", C.c_int32), ("nia", C.c_int32),

@@ -118,7 +118,7 @@ def test_kernels_under_cpu_emulation(emu_lib, tmp_path):
     print(res.stdout)
     assert res.returncode == 0, res.stdout + res.stderr[-3000:]
     assert "SMOKE OK" in res.stdout
-    assert res.stdout.count("PASS") == 20
+    assert res.stdout.count("PASS") == 21
 
 
 @pytest.fixture(scope="module")
@@ -168,3 +168,46 @@ def test_dropin_exit_paths_emulated(emu_lib, dropin_cases, mode, code, out, err)
     assert "NOT REACHED" not in res.stdout
     assert res.stdout == out
     assert err in res.stderr
+
+
+@pytest.mark.timeout(300)
+def test_cells_emulated(emu_lib, tmp_path):
+    """The device cell loop of the SIMPLE pressure correction (device/cells.hip
+    with host/cells.c) on the CPU: bitwise the oracle's coefficients (the
+    oracle is bit-exact with the reference's systFormSimplePres), on a warped
+    graded mesh with every load type and a cell without an equation, and on
+    the lid-driven cavity's walls, both diffusion schemes."""
+    import oracle
+    from sfr3d_amd import systems as Y
+    keys = ("nFace", "nelcon", "cellFace", "owner", "volume", "dcca", "xmcc",
+            "mksi", "ksi", "area", "normal", "vSkew")
+    m = Y.hex_mesh(8, 7, 6, ratio=(1.1, 0.9, 1.05), shear=(0.1, 0.05, -0.1),
+                   warp=0.04)
+    d = {}
+    cases = [("mixed", 1, 33), ("mixed", 4, 100), ("cavity", 4, None)]
+    for i, (bc, diff, skip) in enumerate(cases):
+        case = Y.simple_pres_case(m, seed=21 + i, diffusion=diff, boundary=bc,
+                                  skip_cell=skip)
+        lA, lB, rc = oracle.simple_pres3d(m, case)
+        p = f"c{i}_"
+        for k in keys:
+            d[p + k] = m[k]
+        for k in ("mat", "faceRvel", "faceRpres", "prop", "pres", "gradPres",
+                  "vel", "dField"):
+            d[p + k] = case[k]
+        for name, loads in (("lv", case["loadsVel"]), ("lp", case["loadsPres"])):
+            t, npar, par, dens = oracle.load_table(loads)
+            d[p + name + "_type"], d[p + name + "_np"] = t, npar
+            d[p + name + "_par"], d[p + name + "_density"] = par, dens
+            d[p + name + "_n"] = np.array([len(loads)])
+        d[p + "dims"] = np.array([m["numel"], m["nface"], m["maxViz"], diff])
+        d[p + "lA"], d[p + "lB"], d[p + "rCell"] = lA, lB, rc
+    d["ncases"] = np.array([len(cases)])
+    path = str(tmp_path / "cells.npz")
+    np.savez(path, **d)
+    res = subprocess.run([sys.executable, os.path.join(HERE, "emu", "run_cells.py"),
+                          emu_lib, path], capture_output=True, text=True,
+                         env=_env(), timeout=280)
+    print(res.stdout)
+    assert res.returncode == 0, res.stdout + res.stderr[-3000:]
+    assert res.stdout.count("PASS") == 3 * len(cases) and "FAIL" not in res.stdout
