@@ -598,6 +598,53 @@ def bench_cells(ctx, S, systems, n=128, reps=20):
     for a in list(d.values()) + [d_lA, d_lB, d_rc]:
         a.free()
     cells.destroy()
+    out["momentum"] = bench_cells_vel(ctx, S, systems, m, reps)
+    return out
+
+
+def bench_cells_vel(ctx, S, systems, m, reps):
+    """The SIMPLE momentum cell loop (systFormSimpleVel / cellSimpleVel3D) on
+    the same mesh: SoUp advection, over-relaxed diffusion, Rhie-Chow,
+    second-order face pressure, explicit viscous terms, eddy viscosity,
+    BACKWARD time term, lid and walls.  Algorithmic bytes per cell, each
+    stream once: six face slots of 15 doubles + 2 ints (768 B); the cell's
+    density, viscosity, volume, face count, pressure, pressure gradient,
+    velocity, velocity gradient, d field, eddy viscosity and equation
+    number (189 B); lA (7 doubles), the three diagonals and right-hand
+    sides, rCell and the new d (152 B): 1,109 B."""
+    from sfr3d_amd._lib import lib
+    numel = m["numel"]
+    case = systems.simple_vel_case(m, seed=1)
+    t1 = time.perf_counter()
+    cells = ctx.cells(m, dict(case, loadsPres=[]))
+    create_s = time.perf_counter() - t1
+    neq = int(case["id"].max())
+    d = {k: ctx.array(case[k]) for k in ("pres", "gradPres", "vel", "gradVel",
+                                         "dField", "eddyVisc")}
+    d_id = ctx.array(case["id"])
+    d_out, d_lA = ctx.empty(3 * numel), ctx.empty(numel * 7)
+    d_ad, d_b, d_rc = ctx.empty(3 * neq), ctx.empty(3 * neq), ctx.empty(3 * numel)
+
+    def call():
+        cells.vel(case["opts"], d_id, neq, d["pres"], d["gradPres"], d["vel"],
+                  d["gradVel"], d["eddyVisc"], None, d["dField"], d_out, d_lA,
+                  d_ad, d_b, d_rc)
+    call()
+    lib.sfr_synchronize(ctx.h)
+    t1 = time.perf_counter()
+    for _ in range(reps):
+        call()
+    lib.sfr_synchronize(ctx.h)
+    ms = (time.perf_counter() - t1) / reps * 1e3
+    out = {"workload": "the same mesh, SIMPLE momentum cell loop (u, v, w), "
+                       "SoUp, over-relaxed diffusion, Rhie-Chow, BACKWARD",
+           "ms_per_call": round(ms, 4), "bytes_per_cell": 1109,
+           "GBs_algorithmic": round(1109 * numel / (ms * 1e-3) / 1e9, 1),
+           "create_s": round(create_s, 3)}
+    log(f"[bench] momentum cells: {ms:.3f} ms per call, {out['GBs_algorithmic']} GB/s")
+    for a in list(d.values()) + [d_id, d_out, d_lA, d_ad, d_b, d_rc]:
+        a.free()
+    cells.destroy()
     return out
 
 

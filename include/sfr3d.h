@@ -386,7 +386,22 @@ typedef struct {
   const short   *mat;       /* [numel] 1-based material                  */
   const short   *faceRvel;  /* [numel*(maxViz+1)]                        */
   const short   *faceRpres; /* [numel*(maxViz+1)]                        */
+  const double  *cc;        /* [numel*3] centroids (momentum only)       */
+  const double  *xm;        /* [nface*3] face midpoints (momentum only)  */
 } sfr_cell_mesh;
+
+/* Options of the momentum cell loop (cellSimpleVel3D): advection 1 FoUp,
+ * 2 Cd, 3 SoUp (advVel->iCod1); diffusion 1-4 (diffVel->iCod); turb /
+ * wall_model (tModel->fTurb / fWall; tModel->fTurbStruct is not supported);
+ * rhie, so_pressure, viscosity, residual (momentumModel->fRhieChowInt,
+ * fSoPressure, fViscosity, fRes); simplec (typeSimple SIMPLEC, else
+ * SIMPLE); time / backward (ddt->flag, ddt->type BACKWARD, else EULER);
+ * dt, dt0 (ddt->dt[0], dt[1]); underU (the velocity under-relaxation). */
+typedef struct {
+  int    advection, diffusion, turb, wall_model, rhie, so_pressure,
+         viscosity, residual, simplec, time, backward;
+  double dt, dt0, underU;
+} sfr_vel_opts;
 
 typedef struct {
   short  type;              /* load type (Define.h)                      */
@@ -405,6 +420,32 @@ int  sfr_cells_pres_device(sfr_context *ctx, sfr_cells *cells, int diffusion,
                            const double *d_pres, const double *d_gradPres,
                            const double *d_vel, const double *d_dField,
                            double *d_lA, double *d_lB, double *d_rCell);
+/* The momentum cell loop of SIMPLE (systFormSimpleVel,
+ * src/CellLoopInc.c:98-385; cellSimpleVel3D, src/CellSimple.c:106-638),
+ * u, v and w together, for a mesh created with cc and xm.  Inputs in the
+ * reference's layout: d_id (the momentum equations' numbers, -1 for a cell
+ * of prescribed velocity), pres, gradPres (3/cell), vel (3), gradVel (9,
+ * d u_i / d x_j at i*3+j), eddyVisc (1; NULL unless turb), wallPar (4; NULL
+ * unless wall_model), dField (3).  Outputs: d_lA in sfr_assemble's layout
+ * (the shared off-diagonal coefficients, the u diagonal at maxViz) for
+ * sfr_assemble_device; d_ad and d_b (3*neq, component c at c*neq, the
+ * layout csrSimple writes and sfr_solve_shared_device reads); d_rCell
+ * (3*numel, component-major); d_dOut (3/cell) the new d field, which must
+ * not alias d_dField.  Velocity loads: MOVEWALL, STATICWALL, INLET,
+ * OUTLET (INLETSTAICTPRES, OPEN and INLETTOTALPRES are refused).
+ * Coefficients are bitwise the reference's loop, except that every cell
+ * reads its neighbours' input d: the reference's serial loop writes d in
+ * place, so with Rhie-Chow interpolation a cell there sees the new d of
+ * lower-numbered neighbours (an order its OpenMP loop does not keep
+ * either). */
+int  sfr_cells_vel_device(sfr_context *ctx, sfr_cells *cells,
+                          const sfr_vel_opts *opts, const int32_t *d_id,
+                          int neq, const double *d_pres,
+                          const double *d_gradPres, const double *d_vel,
+                          const double *d_gradVel, const double *d_eddyVisc,
+                          const double *d_wallPar, const double *d_dField,
+                          double *d_dOut, double *d_lA, double *d_ad,
+                          double *d_b, double *d_rCell);
 void sfr_cells_destroy(sfr_context *ctx, sfr_cells *cells);
 /* The two field updates around SIMPLE's pressure-correction solve, on
  * device arrays: sfr_cells_update_value is updateCellValueBlock
@@ -432,7 +473,9 @@ int sfr_kernel_time(sfr_context *ctx, sfr_matrix *A, int which, int reps,
  * PBICGSTAB; the other solver ids stay FP64); "ru_delta" / "ru_delta_bicg"
  * = MIXED PCG / BiCGSTAB: drop of the FP32 residual norm (relative to its
  * largest value since the last fold) that triggers an FP64 reliable update
- * (defaults 0.1 and 1e-3). */
+ * (both default to 0.1).  A mixed BiCGSTAB solve whose FP64 residual stops
+ * improving (or breaks down) is finished in FP64 from its last x, within
+ * the same call; stats.iters then counts both parts. */
 int sfr_set_option(sfr_context *ctx, const char *name, double value);
 
 /* Drop-in twin of solverC (reference include/Solv.h:59-70), same prototype

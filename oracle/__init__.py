@@ -368,3 +368,103 @@ def ref_simple_pres(mesh, case, storage=2):
     if r:
         raise RuntimeError("ref_simple_pres")
     return ia, ja, al, ad, b, rc
+
+
+# ---- the SIMPLE momentum cell loop (cells.c) --------------------------------
+
+VEL_OPTS = ("advection", "diffusion", "turb", "wall_model", "rhie",
+            "so_pressure", "viscosity", "residual", "simplec", "time",
+            "backward")
+
+
+def vel_options(case):
+    """(opt int32[11], optd f64[3]) of a momentum case."""
+    o = case["opts"]
+    opt = np.array([int(o[k]) for k in VEL_OPTS], np.int32)
+    optd = np.array([o["dt"], o["dt0"], o["underU"]], np.float64)
+    return opt, optd
+
+
+def simple_vel3d(mesh, case, in_place=False):
+    """lA (numel x (maxViz+3): faces, then the u, v, w diagonals), lB
+    (numel x 3), rCell (3 x numel) and the new d field (numel x 3) of the
+    SIMPLE momentum cell loop (systFormSimpleVel + cellSimpleVel3D).
+    in_place: cells read the updated d of lower-numbered neighbours, as the
+    reference's serial loop does; otherwise every neighbour's input d."""
+    lib = orc()
+    if not getattr(lib, "_vel_ready", False):
+        f = lib.orc_simple_vel3d
+        f.restype = C.c_int
+        f.argtypes = ([C.c_int, C.c_short, i16p, i32p, i32p, i32p] + [f64p] * 10
+                      + [i16p, f64p, i16p, i32p, C.c_int, i16p, i16p, f64p, f64p,
+                         i32p, f64p] + [f64p] * 8 + [C.c_int, f64p, f64p, f64p])
+        lib._vel_ready = True
+    numel, mv = mesh["numel"], mesh["maxViz"]
+    lA = np.zeros(numel * (mv + 3))
+    lB = np.zeros(numel * 3)
+    rc = np.zeros(3 * numel)
+    d_in = np.ascontiguousarray(case["dField"], np.float64)
+    d_out = d_in.copy() if in_place else np.zeros(3 * numel)
+    opt, optd = vel_options(case)
+    lv = load_table(case["loadsVel"])
+    eddy = np.ascontiguousarray(case.get("eddyVisc", np.zeros(numel)), np.float64)
+    wall = np.ascontiguousarray(case.get("wallPar", np.zeros(4 * numel)), np.float64)
+    r = lib.orc_simple_vel3d(
+        numel, mv, mesh["nFace"], mesh["nelcon"], mesh["cellFace"], mesh["owner"],
+        mesh["volume"], mesh["dcca"], mesh["xmcc"], mesh["cc"], mesh["mksi"],
+        mesh["ksi"], mesh["area"], mesh["normal"], mesh["xm"], mesh["vSkew"],
+        case["mat"], case["prop"], case["faceRvel"],
+        np.ascontiguousarray(case["id"], np.int32), len(case["loadsVel"]), *lv,
+        opt, optd, case["pres"], case["gradPres"], case["vel"], case["gradVel"],
+        eddy, wall, d_in, d_out, 1 if in_place else 0, lA, lB, rc)
+    if r:
+        raise ValueError(f"orc_simple_vel3d: {r} (advection, diffusion or load type)")
+    return lA.reshape(numel, mv + 3), lB.reshape(numel, 3), rc.reshape(3, numel), \
+        d_out.reshape(numel, 3)
+
+
+def ref_simple_vel(mesh, case):
+    """The reference's own systFormSimpleVel (src/CellLoopInc.c:98) on the
+    same mesh and case, in CSRD unsym: (ia, ja, al, ad[3*neq], b[3*neq],
+    rCell[3*numel], dField updated in place)."""
+    lib = ref()
+    if not getattr(lib, "_vel_ready", False):
+        f = lib.ref_simple_vel
+        f.restype = C.c_int
+        f.argtypes = ([C.c_int, C.c_short, i16p, i32p, i32p, i32p] + [f64p] * 12
+                      + [i16p, f64p, C.c_int, i16p, i16p, i32p, C.c_int, i16p,
+                         i16p, f64p, f64p, i32p, f64p] + [f64p] * 7
+                      + [f64p, f64p, f64p, f64p, C.c_long])
+        lib._vel_ready = True
+    numel, mv = mesh["numel"], mesh["maxViz"]
+    id_ = np.ascontiguousarray(case["id"], np.int32)
+    num = np.ascontiguousarray(case["num"], np.int32)
+    neq = int(id_.max())
+    out = np.zeros(3, np.int64)
+    lib.ref_init(1 << 31)
+    lib.ref_datastruct(2, 1, id_, num, mesh["nelcon"], mesh["nFace"], numel, mv,
+                       neq, out)
+    ia = np.zeros(out[0], np.int32)
+    ja = np.zeros(out[1], np.int32)
+    lib.ref_datastruct_copy(ia, ja, out[0], out[1])
+    al = np.zeros(out[2])
+    ad = np.zeros(3 * neq)
+    b = np.zeros(3 * neq)
+    rc = np.zeros(3 * numel)
+    d = np.ascontiguousarray(case["dField"], np.float64).copy()
+    opt, optd = vel_options(case)
+    lv = load_table(case["loadsVel"])
+    eddy = np.ascontiguousarray(case.get("eddyVisc", np.zeros(numel)), np.float64)
+    wall = np.ascontiguousarray(case.get("wallPar", np.zeros(4 * numel)), np.float64)
+    nmat = len(case["prop"]) // 10
+    r = lib.ref_simple_vel(
+        numel, mv, mesh["nFace"], mesh["nelcon"], mesh["cellFace"], mesh["owner"],
+        mesh["volume"], mesh["dcca"], mesh["xmcc"], mesh["cc"], mesh["mksi"],
+        mesh["ksi"], mesh["eta"], mesh["area"], mesh["normal"], mesh["xm"],
+        mesh["mvSkew"], mesh["vSkew"], case["mat"], case["prop"], nmat,
+        case["faceRvel"], case["faceRpres"], id_, len(case["loadsVel"]), *lv,
+        opt, optd, case["pres"], case["gradPres"], case["vel"], case["gradVel"],
+        eddy, wall, d, al, ad, b, rc, int(out[2]))
+    if r:
+        raise RuntimeError("ref_simple_vel")
+    return ia, ja, al, ad, b, rc, d

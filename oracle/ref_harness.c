@@ -344,3 +344,75 @@ int ref_simple_pres(int numel, short maxViz, short *nFace, int *nelcon,
   free(bb); free(calType); free(geomType); free(nen); free(wallPar); free(gCc);
   return 0;
 }
+
+/* The reference's SIMPLE momentum cell loop, systFormSimpleVel
+ * (src/CellLoopInc.c:98-385, with cellSimpleVel3D,
+ * src/CellSimple.c:106-638), unchanged, assembling u/v/w (ndf 3) into the
+ * CSRD unsym structure held by ref_datastruct: al (nal, shared), ad and b
+ * (3*neq, component-major), rCell (3*numel), and dField updated in place as
+ * the loop does.  opt = {advection, diffusion, fTurb, fWall, fRhieChowInt,
+ * fSoPressure, fViscosity, fRes, SIMPLEC, fTime, BACKWARD}; optd = {dt, dt0,
+ * underU}. */
+int ref_simple_vel(int numel, short maxViz, short *nFace, int *nelcon,
+                   int *cellFace, int *fOwner, double *volume, double *dcca,
+                   double *xmcc, double *cc, double *mksi, double *ksi,
+                   double *eta, double *area, double *normal, double *xm,
+                   double *mvSkew, double *vSkew, short *mat, double *prop,
+                   int nmat, short *faceRvel, short *faceRpres, int *id,
+                   int nlv, short *lv_type, short *lv_np, double *lv_par,
+                   double *lv_density, int *opt, double *optd, double *pres,
+                   double *gradPres, double *vel, double *gradVel,
+                   double *eddyVisc, double *wallPar, double *dField,
+                   double *al, double *ad, double *b, double *rCell, long nal)
+{
+  if (!g_have_sys || nmat > MAXMAT) return -1;
+  static Loads lpres[MAXLOAD];
+  fill_loads(loadsVel, nlv, lv_type, lv_np, lv_par, lv_density);
+  memset(lpres, 0, sizeof lpres);
+  Advection adv;
+  memset(&adv, 0, sizeof adv);
+  adv.iCod1 = (short)opt[0];
+  Diffusion dif = {(short)opt[1]};
+  Turbulence tm_;
+  memset(&tm_, 0, sizeof tm_);
+  tm_.fTurb = opt[2] != 0;
+  tm_.fWall = opt[3] != 0;
+  MomentumModel mm;
+  memset(&mm, 0, sizeof mm);
+  mm.fRhieChowInt = opt[4] != 0;
+  mm.fSoPressure = opt[5] != 0;
+  mm.fViscosity = opt[6] != 0;
+  mm.fRes = opt[7] != 0;
+  Temporal ddt;
+  memset(&ddt, 0, sizeof ddt);
+  ddt.flag = opt[9] != 0;
+  ddt.type = opt[10] ? BACKWARD : EULER;
+  ddt.dt[0] = optd[0];
+  ddt.dt[1] = optd[1];
+  short *calType = calloc((size_t)nmat, sizeof(short));
+  short *geomType = malloc(sizeof(short) * (size_t)numel);
+  short *nen = malloc(sizeof(short) * (size_t)numel);
+  double *stressR = calloc((size_t)numel * 6, sizeof(double));
+  for (int m = 0; m < nmat; m++) calType[m] = 1;
+  for (int c = 0; c < numel; c++) { geomType[c] = HEXACELL; nen[c] = 8; }
+  const int neq = g_sys.neq;
+  zero(g_sys.al, nal, DOUBLEC);
+  double *ad3 = calloc((size_t)3 * neq, sizeof(double));
+  double *b3 = calloc((size_t)3 * neq, sizeof(double));
+  ompVar.fCell = false;
+  ompVar.nThreadsCell = 1;
+  systFormSimpleVel(loadsVel, lpres, &adv, &dif, &tm_, &mm, NULL,
+                    opt[8] ? SIMPLEC : SIMPLE, NULL, nelcon, nen, nFace,
+                    cellFace, fOwner, volume, dcca, xmcc, cc, mksi, ksi, eta,
+                    area, normal, xm, mvSkew, vSkew, geomType, prop, calType,
+                    mat, g_sys.ia, g_sys.ja, g_sys.al, ad3, b3, id, faceRvel,
+                    faceRpres, pres, gradPres, vel, gradVel, dField, optd[2],
+                    rCell, stressR, eddyVisc, wallPar, &ddt, neq, neq,
+                    g_sys.nad, 0, 8, maxViz, 3, numel, 3, 6, g_sys.storage,
+                    true, true, true, g_sys.unsym);
+  memcpy(al, g_sys.al, sizeof(double) * (size_t)nal);
+  memcpy(ad, ad3, sizeof(double) * (size_t)3 * neq);
+  memcpy(b, b3, sizeof(double) * (size_t)3 * neq);
+  free(ad3); free(b3); free(calType); free(geomType); free(nen); free(stressR);
+  return 0;
+}

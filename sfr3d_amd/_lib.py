@@ -80,7 +80,15 @@ class SfrCellMesh(C.Structure):
         (k, C.c_void_p) for k in ("nFace", "nelcon", "cellFace", "owner",
                                   "volume", "dcca", "xmcc", "mksi", "ksi",
                                   "area", "normal", "vSkew", "mat",
-                                  "faceRvel", "faceRpres")]
+                                  "faceRvel", "faceRpres", "cc", "xm")]
+
+
+class SfrVelOpts(C.Structure):
+    _fields_ = [(k, C.c_int) for k in ("advection", "diffusion", "turb",
+                                       "wall_model", "rhie", "so_pressure",
+                                       "viscosity", "residual", "simplec",
+                                       "time", "backward")] + [
+        ("dt", C.c_double), ("dt0", C.c_double), ("underU", C.c_double)]
 
 
 class SfrLoad(C.Structure):
@@ -144,6 +152,8 @@ _sig("sfr_cells_create", C.c_int, vp, C.POINTER(SfrCellMesh), f64p, C.c_int,
 _sig("sfr_cells_pres_device", C.c_int, vp, vp, C.c_int, vp, vp, vp, vp, vp,
      vp, vp)
 _sig("sfr_cells_destroy", None, vp, vp)
+_sig("sfr_cells_vel_device", C.c_int, vp, vp, C.POINTER(SfrVelOpts), vp,
+     C.c_int, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp)
 _sig("sfr_cells_update_value", C.c_int, vp, C.c_int, vp, vp, vp, C.c_int)
 _sig("sfr_simple_update", C.c_int, vp, C.c_int, C.c_double, vp, vp, vp, vp, vp)
 # uneven batch

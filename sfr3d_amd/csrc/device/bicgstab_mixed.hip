@@ -19,10 +19,22 @@
 // true one (residual replacement: H. van der Vorst, Q. Ye, "Residual
 // replacement strategies for Krylov subspace iterative methods for the
 // convergence of true residuals", SIAM J. Sci. Comput. 22 (2000)) and the
-// recurrence goes on with the same shadow r0 and direction:
-//   rho = r.r0, beta = (rho / rho_prev) (alpha / omega), p = r + beta (p - omega v).
-// A fold requested at the half step (omega unknown) restarts the direction,
-// p = r; a breakdown restarts it with a new shadow r0 = r.
+// recurrence restarts from it with a new shadow, r0 = r, p = r.  Going on
+// with the old shadow and direction (beta from the replaced residual) was
+// measured on the CPU and dropped: in FP32 the replaced residual and the
+// old Krylov quantities disagree enough that BiCGSTAB's irregular
+// convergence turns into stagnation (80^3 Poisson, delta 1e-3: 1,498
+// iterations against 645 with restarts and 554 in FP64; on the 128^3
+// system the continued recurrence ended in a breakdown).  A fold requested
+// at the half step (omega unknown, the residual met the test) keeps the
+// shadow with p = r.  alpha, omega and beta must be finite in FP32 as well:
+// a value beyond FLT_MAX is a breakdown, not an update.  The fold threshold
+// defaults to delta = 0.1 (option ru_delta_bicg): each restarted FP32 cycle
+// must reduce its residual by delta, and on harder systems the attainable
+// FP32 reduction (about kappa * eps32) is short of 1e-3 (128^3 Poisson:
+// 1,397 FP32 iterations with delta 0.1, stagnation with 1e-3).  When the
+// FP64 true residual stops improving (SFR_MB_STALL folds without halving)
+// the loop ends as a breakdown and the host finishes in FP64 from x.
 //
 // Vectors are interleaved per row, x[i*NB + c].  Each system keeps its own
 // scalars (sc[c]) and its own fold requests: a system waiting for its fold
@@ -42,12 +54,21 @@ static_assert(4 * 4 * SFR_GRIDCAP <= 4 * SFR_MAXG, "partials region too small");
 
 #define OWNER(c) (blockIdx.x == 0 && threadIdx.x == (unsigned)(c))
 
+// folds in a row without progress that end the FP32 loop (see F2)
+#define SFR_MB_STALL 8.0
+
+// a finite double that stays finite in FP32
+__device__ __forceinline__ bool fits32(double v)
+{
+  return fabs(v) <= 3.4028234663852886e38;
+}
+
 // Scalars of system c (no kernel reads a value its own first workgroup
 // writes): run2..run5 hand the run flag M1->M2->M3->M4->M5; ru = fold
 // requested (1 at the end of an iteration, 2 at the half step, 3 after a
 // breakdown), read by the fold kernels, cleared by F2; pbuf = system is
 // iterating (set by F2, cleared with a request); d = rho (r.r0), copied to
-// d_prev by M4 for M5 and F2; the largest ||r||^2 since the last fold is staged in
+// d_prev by M4 for M5; the largest ||r||^2 since the last fold is staged in
 // res[5] by M5 / F2 and copied to rho_max by M1; fresh = the fold mode,
 // staged by F1 for F2.
 
@@ -117,7 +138,7 @@ __global__ __launch_bounds__(SFR_BLOCK) void k_mb_s(int64_t n, MbState st,
 #pragma unroll
   for (int c = 0; c < NB; c++) {
     alpha[c] = sc[c].d / vr0[c];
-    const bool brk = act[c] && (!(vr0[c] != 0.0) || !isfinite(alpha[c]));
+    const bool brk = act[c] && (!(vr0[c] != 0.0) || !fits32(alpha[c]));
     if (OWNER(c)) {
       if (brk) { sc[c].ru = 3; sc[c].pbuf = 0; }
       sc[c].alpha = alpha[c];
@@ -228,7 +249,7 @@ __global__ __launch_bounds__(SFR_BLOCK) void k_mb_r(int64_t n, MbState st,
 #pragma unroll
   for (int c = 0; c < NB; c++) {
     w[c] = tt[c] / tt[NB + c];
-    const bool brk = act[c] && (!(tt[NB + c] != 0.0) || !(w[c] != 0.0) || !isfinite(w[c]));
+    const bool brk = act[c] && (!(tt[NB + c] != 0.0) || !(w[c] != 0.0) || !fits32(w[c]));
     if (OWNER(c)) {
       if (brk) { sc[c].ru = 3; sc[c].pbuf = 0; }
       sc[c].omega = w[c];
@@ -285,7 +306,7 @@ __global__ __launch_bounds__(SFR_BLOCK) void k_mb_p(int64_t n, MbState st,
     beta[c] = (rr[NB + c] / rho) * (sc[c].alpha / w[c]);
     const bool fold = act[c] && (sqrt(rr[c]) < sc[c].conv || rr[c] == 0.0 ||
                                  rr[c] < sc[c].delta2 * sc[c].rho_max || !isfinite(rr[c]));
-    const bool brk = act[c] && !fold && (!(rho != 0.0) || !isfinite(beta[c]));
+    const bool brk = act[c] && !fold && (!(rho != 0.0) || !fits32(beta[c]));
     if (OWNER(c) && act[c]) {
       const unsigned it = sc[c].iter + 1;
       sc[c].iter = it;
@@ -414,8 +435,8 @@ __global__ __launch_bounds__(SFR_BLOCK) void k_mb_addy(int64_t n,
   }
 }
 
-// F2: stop test on the true residual; the new direction (continued after an
-// end-of-iteration fold, p = r otherwise; a new shadow after a breakdown)
+// F2: stop test on the true residual; the new direction p = r (with a new
+// shadow r0 = r after an end-of-iteration fold or a breakdown)
 template <int RPT, int NB>
 __global__ __launch_bounds__(SFR_BLOCK) void k_mb_f2(int64_t n, MbState st,
     int init, const double *pin, int gin, sfr_scalars *sc)
@@ -429,42 +450,46 @@ __global__ __launch_bounds__(SFR_BLOCK) void k_mb_f2(int64_t n, MbState st,
     any |= act[c];
   }
   if (!any) return;
-  double t[2 * NB], beta[NB], w[NB];
+  double t[2 * NB];
   reduce_partials<2 * NB>(pin, gin, t);
   any = false;
 #pragma unroll
   for (int c = 0; c < NB; c++) {
     const double rr = t[c];
     double rho = t[NB + c];
-    w[c] = sc[c].omega;
     const bool conv = act[c] && (sqrt(rr) < sc[c].conv || rr == 0.0);
     const bool bad = act[c] && !conv && !isfinite(rr);
     const bool maxed = act[c] && !conv && !bad && sc[c].iter >= sc[c].maxit;
-    unsigned md = mode[c];
-    double bt = 0.0;
-    if (md == 1) {
-      // rho of the iteration that requested the fold (M4 copied it)
-      bt = (rho / sc[c].d_prev) * (sc[c].alpha / w[c]);
-      if (!(rho != 0.0) || !isfinite(bt)) md = 3;
-    }
-    if (md == 3) {                         // a new shadow: r itself
-      rho = rr;
-      bt = 0.0;
-    }
+    // stagnation: SFR_MB_STALL folds in a row without the true residual
+    // norm halving below the best one seen (beta holds that best ||r||^2,
+    // res[4] the count) ends the FP32 loop as a breakdown; the host then
+    // finishes the solve in FP64 from this x
+    bool stall = false;
+    if (act[c] && !conv && !bad && !maxed && !init)
+      stall = !(rr < 0.25 * sc[c].beta) && sc[c].res[4] + 1.0 >= SFR_MB_STALL;
+    // an end-of-iteration fold (1) or a breakdown (3): a new shadow, r
+    // itself; a half-step fold (2): the same shadow
+    const unsigned md = mode[c] == 2 ? 2u : 3u;
+    if (md == 3) rho = rr;
     mode[c] = md;
-    beta[c] = bt;
     if (OWNER(c) && act[c]) {
       sc[c].res[3] = rr;
       if (!init) sc[c].outer = sc[c].outer + 1;
+      if (init || rr < 0.25 * sc[c].beta) {
+        sc[c].beta = rr;
+        sc[c].res[4] = 0.0;
+      } else {
+        sc[c].res[4] = sc[c].res[4] + 1.0;
+      }
       if (conv) sc[c].done = SFR_DONE_CONV;
-      else if (bad) sc[c].done = SFR_DONE_BREAK;
+      else if (bad || stall) sc[c].done = SFR_DONE_BREAK;
       else if (maxed) sc[c].done = SFR_DONE_MAXIT;
       sc[c].d = rho;
       sc[c].res[5] = rr;                   // largest ||r||^2 since this fold
       sc[c].ru = 0;
       sc[c].pbuf = 1;
     }
-    act[c] = act[c] && !conv && !bad && !maxed;
+    act[c] = act[c] && !conv && !bad && !maxed && !stall;
     any |= act[c];
   }
   if (!any) return;
@@ -481,10 +506,8 @@ __global__ __launch_bounds__(SFR_BLOCK) void k_mb_f2(int64_t n, MbState st,
           const int64_t k = i * NB + c;
           const float ri = st.r[k];
           if (mode[c] == 3) st.r0[k] = ri;
-          const float pi = mode[c] == 1
-              ? ri + (float)beta[c] * (st.p[k] - (float)w[c] * st.v[k]) : ri;
-          st.p[k] = pi;
-          st.z[k] = pi * st.dinv[k];
+          st.p[k] = ri;
+          st.z[k] = ri * st.dinv[k];
         }
       }
     }

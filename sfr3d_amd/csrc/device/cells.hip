@@ -197,6 +197,324 @@ __global__ __launch_bounds__(SFR_BLOCK) void k_cell_pres(CellPres a)
   if (a.rc) a.rc[c] = p;
 }
 
+// ---- the momentum cell loop ------------------------------------------------
+// systFormSimpleVel (src/CellLoopInc.c:98-385) with cellSimpleVel3D
+// (src/CellSimple.c:106-638) for u, v, w together, one work-item per cell.
+// Every neighbour's d is the input one (the reference's serial loop writes
+// d in place; see sfr3d.h).  Planes 9-11 hold xmcc, 12-14 the vector from
+// the neighbour's centroid to the face midpoint.
+
+constexpr int L_MOVEWALL = 8, L_STATICWALL = 14;
+
+// difusionSchemeNew (src/CellLib.c:5824-5990), ndm 3: e and t
+__device__ __forceinline__ void diffusion_et(int code, const double *s,
+                                             const double *k, double *e,
+                                             double *t)
+{
+#pragma clang fp contract(off)
+  if (code == 2) {
+    const double nk = s[0] * k[0] + s[1] * k[1] + s[2] * k[2];
+    e[0] = nk * k[0];
+    e[1] = nk * k[1];
+    e[2] = nk * k[2];
+  } else if (code == 4) {
+    const double ss = s[0] * s[0] + s[1] * s[1] + s[2] * s[2];
+    const double nk = s[0] * k[0] + s[1] * k[1] + s[2] * k[2];
+    const double tmp = ss / nk;
+    e[0] = tmp * k[0];
+    e[1] = tmp * k[1];
+    e[2] = tmp * k[2];
+  } else {                                   // 1 ORTHOGONAL, 3 ORTHOGONALC
+    const double ss = sqrt(s[0] * s[0] + s[1] * s[1] + s[2] * s[2]);
+    e[0] = ss * k[0];
+    e[1] = ss * k[1];
+    e[2] = ss * k[2];
+  }
+#pragma unroll
+  for (int j = 0; j < 3; j++) t[j] = code == 1 ? 0.0 : s[j] - e[j];
+}
+
+// pLoadSimple, momentum part (src/Loads.c:265-520): MOVEWALL / STATICWALL
+// through moveWall (src/CellLib.c:7715-7760), INLET and OUTLET with stress
+// (src/CellLib.c:7013-7040) and gradFaceNull (src/CellLib.c:5477-5600)
+__device__ __forceinline__ void vel_load_mom(const sfr_load &ld, double *sPc,
+    double *p, const double *vc, const double *gv, double viscC,
+    double effC, const double *s, const double *n, const double *xmcc,
+    double rhoC, const double *wall, int wallModel, double A, double dcca)
+{
+#pragma clang fp contract(off)
+  if (ld.type == L_MOVEWALL || ld.type == L_STATICWALL) {
+    double tA[3];
+#pragma unroll
+    for (int j = 0; j < 3; j++) tA[j] = ld.type == L_MOVEWALL ? ld.par[j] : 0.0;
+    double vw;
+    if (wallModel) {
+      const double yPlus = wall[0], uPlus = wall[1];
+      vw = yPlus > 0.01 ? viscC * yPlus / uPlus : viscC;
+    } else {
+      vw = effC;
+    }
+    const double aP = vw * A / dcca;
+    sPc[0] += aP * (1.0 - n[0] * n[0]);
+    sPc[1] += aP * (1.0 - n[1] * n[1]);
+    sPc[2] += aP * (1.0 - n[2] * n[2]);
+    p[0] += aP * (tA[0] * (1.0 - n[0] * n[0]) + (vc[1] - tA[1]) * n[1] * n[0]
+                  + (vc[2] - tA[2]) * n[2] * n[0]);
+    p[1] += aP * (tA[1] * (1.0 - n[1] * n[1]) + (vc[0] - tA[0]) * n[0] * n[1]
+                  + (vc[2] - tA[2]) * n[2] * n[1]);
+    p[2] += aP * (tA[2] * (1.0 - n[2] * n[2]) + (vc[0] - tA[0]) * n[0] * n[2]
+                  + (vc[1] - tA[1]) * n[1] * n[2]);
+    return;
+  }
+  if (ld.type != L_INLET && ld.type != L_OUTLET) return;
+  const double tr = gv[0] + gv[4] + gv[8];
+  double ss[6];
+  ss[0] = 2.0 * viscC * gv[0] + 0.0 * tr;
+  ss[1] = 2.0 * viscC * gv[4] + 0.0 * tr;
+  ss[2] = 2.0 * viscC * gv[8] + 0.0 * tr;
+  ss[3] = viscC * (gv[3] + gv[1]);
+  ss[4] = viscC * (gv[7] + gv[5]);
+  ss[5] = viscC * (gv[6] + gv[2]);
+  if (ld.type == L_INLET) {
+    double wfn = ld.par[0] * n[0] + ld.par[1] * n[1];
+    wfn += ld.par[2] * n[2];
+    const double m = ld.par[ld.np - 1] * wfn * A;
+    p[0] += ss[0] * s[0] + ss[3] * s[1] + ss[5] * s[2];
+    p[1] += ss[3] * s[0] + ss[1] * s[1] + ss[4] * s[2];
+    p[2] += ss[5] * s[0] + ss[4] * s[1] + ss[2] * s[2];
+    p[0] -= m * ld.par[0];
+    p[1] -= m * ld.par[1];
+    p[2] -= m * ld.par[2];
+    return;
+  }
+  double wfn = vc[0] * n[0] + vc[1] * n[1];
+  wfn += vc[2] * n[2];
+  const double m = rhoC * wfn * A;
+  p[0] += ss[0] * s[0] + ss[3] * s[1] + ss[5] * s[2];
+  p[1] += ss[3] * s[0] + ss[1] * s[1] + ss[4] * s[2];
+  p[2] += ss[5] * s[0] + ss[4] * s[1] + ss[2] * s[2];
+  const double mod = sqrt(xmcc[0] * xmcc[0] + xmcc[1] * xmcc[1] + xmcc[2] * xmcc[2]);
+  const double ef[3] = {xmcc[0] / mod, xmcc[1] / mod, xmcc[2] / mod};
+#pragma unroll
+  for (int i = 0; i < 3; i++) {
+    const double pr = gv[i * 3] * ef[0] + gv[i * 3 + 1] * ef[1] + gv[i * 3 + 2] * ef[2];
+    const double g0 = gv[i * 3] - pr * ef[0], g1 = gv[i * 3 + 1] - pr * ef[1],
+                 g2 = gv[i * 3 + 2] - pr * ef[2];
+    const double tmp = g0 * xmcc[0] + g1 * xmcc[1] + g2 * xmcc[2];
+    p[i] -= m * tmp;
+  }
+  sPc[0] += m;
+  sPc[1] += m;
+  sPc[2] += m;
+}
+
+struct CellVel {
+  sfr_vel_opts o;
+  int numel, maxViz, neq;
+  const signed char *nface;
+  const int32_t *nb, *bc;
+  const double *g, *rho, *visc, *vol;
+  const sfr_load *lv;
+  const int32_t *id;
+  const double *pres, *gp, *vel, *gv, *eddy, *wall, *din;
+  double *dout, *lA, *ad, *b, *rc;
+};
+
+__global__ __launch_bounds__(SFR_BLOCK) void k_cell_vel(CellVel a)
+{
+#pragma clang fp contract(off)
+  const int64_t c = (int64_t)xcd_chunk(blockIdx.x, gridDim.x) * SFR_BLOCK + threadIdx.x;
+  if (c >= a.numel) return;
+  const sfr_vel_opts &o = a.o;
+  const int nf = a.nface[c];
+#pragma unroll
+  for (int k = 0; k < 3; k++) a.rc[k * a.numel + c] = 0.0;
+  if (nf < 0) {                              // prescribed velocity: d kept
+#pragma unroll
+    for (int j = 0; j < 3; j++) a.dout[c * 3 + j] = a.din[c * 3 + j];
+    return;
+  }
+  const int64_t S = (int64_t)a.maxViz * a.numel;
+  const double rhoC = a.rho[c], viscC = a.visc[c];
+  const double effC = viscC + (o.turb ? a.eddy[c] : 0.0);
+  const double presC = a.pres[c], V = a.vol[c];
+  double vc[3], gpC[3], dC[3], gvC[9];
+#pragma unroll
+  for (int j = 0; j < 3; j++) {
+    vc[j] = a.vel[c * 3 + j];
+    gpC[j] = a.gp[c * 3 + j];
+    dC[j] = a.din[c * 3 + j];
+  }
+#pragma unroll
+  for (int j = 0; j < 9; j++) gvC[j] = a.gv[c * 9 + j];
+  double sP = 0.0, p[3] = {0.0, 0.0, 0.0}, pf[3] = {0.0, 0.0, 0.0},
+         sPc[3] = {0.0, 0.0, 0.0}, la[6];
+  for (int f = 0; f < 6; f++) {
+    if (f >= nf) break;
+    const int64_t q = (int64_t)f * a.numel + c;
+    const double n[3] = {a.g[q], a.g[S + q], a.g[2 * S + q]};
+    const double k[3] = {a.g[3 * S + q], a.g[4 * S + q], a.g[5 * S + q]};
+    const double A = a.g[6 * S + q], mk = a.g[7 * S + q], w = a.g[8 * S + q];
+    const double xc[3] = {a.g[9 * S + q], a.g[10 * S + q], a.g[11 * S + q]};
+    const int nb = a.nb[q];
+    double s[3], e[3], t[3];
+    if (nb > -1) {
+      const double rv[3] = {a.g[12 * S + q], a.g[13 * S + q], a.g[14 * S + q]};
+      const double rhoV = a.rho[nb], viscV = a.visc[nb];
+      const double effV = viscV + (o.turb ? a.eddy[nb] : 0.0);
+      const double presV = a.pres[nb];
+      const double *gvV = a.gv + (int64_t)nb * 9;
+      double vV[3], gpV[3], dV[3], duDksi[3];
+#pragma unroll
+      for (int j = 0; j < 3; j++) {
+        vV[j] = a.vel[nb * 3 + j];
+        gpV[j] = a.gp[nb * 3 + j];
+        dV[j] = a.din[nb * 3 + j];
+        duDksi[j] = (vV[j] - vc[j]) / mk;
+      }
+#pragma unroll
+      for (int j = 0; j < 3; j++) s[j] = A * n[j];
+      diffusion_et(o.diffusion, s, k, e, t);
+      const double alpha = w, am1 = 1.0 - alpha;
+      // mediaHarmonica (src/CellLib.c:7879-7884)
+      const double coef = (effC == 0.0 || effV == 0.0) ? 0.0
+                          : 1.0 / (alpha / effC + am1 / effV);
+      const double modE = sqrt(e[0] * e[0] + e[1] * e[1] + e[2] * e[2]);
+      const double dfd = coef * modE / mk;
+      double gf[9], wf[3];
+#pragma unroll
+      for (int i = 0; i < 3; i++) {
+#pragma unroll
+        for (int j = 0; j < 3; j++)
+          gf[i * 3 + j] = am1 * gvC[i * 3 + j] + alpha * gvV[i * 3 + j];
+        wf[i] = am1 * vc[i] + alpha * vV[i];
+      }
+      const double density = am1 * rhoC + alpha * rhoV;
+      double wfn = wf[0] * n[0] + wf[1] * n[1] + wf[2] * n[2];
+      if (o.rhie) {                          // interpolFaceVel
+        double dF[3], gfp[3];
+#pragma unroll
+        for (int j = 0; j < 3; j++) {
+          dF[j] = am1 * dC[j] + alpha * dV[j];
+          gfp[j] = am1 * gpC[j] + alpha * gpV[j];
+        }
+        const double g0 = (presV - presC) / mk;
+        const double nk = dF[0] * k[0] * n[0] + dF[1] * k[1] * n[1]
+                        + dF[2] * k[2] * n[2];
+        double g1 = gfp[0] * k[0] + gfp[1] * k[1] + gfp[2] * k[2];
+        g1 = (g1 - g0) * nk;
+        wfn += g1;
+      }
+      double gc[9], dfdc[3];
+#pragma unroll
+      for (int i = 0; i < 3; i++) {
+        double gk = gf[i * 3] * k[0] + gf[i * 3 + 1] * k[1] + gf[i * 3 + 2] * k[2];
+        const double du = duDksi[i] - gk;
+#pragma unroll
+        for (int j = 0; j < 3; j++) gc[i * 3 + j] = gf[i * 3 + j] + du * k[j];
+        gk = gc[i * 3] * t[0] + gc[i * 3 + 1] * t[1] + gc[i * 3 + 2] * t[2];
+        dfdc[i] = coef * gk;
+      }
+      const double cv = density * wfn * A;
+      double cvc[3];
+#pragma unroll
+      for (int i = 0; i < 3; i++) {
+        if (o.advection == 1) {
+          cvc[i] = 0.0;
+        } else if (o.advection == 2) {       // deferredCd
+          const double cd = am1 * vc[i] + alpha * vV[i];
+          cvc[i] = cd - (cv < 0.0 ? vV[i] : vc[i]);
+        } else {                             // upwindLinearV1
+          const double *gg = cv < 0.0 ? gvV + i * 3 : gvC + i * 3;
+          const double *r = cv < 0.0 ? rv : xc;
+          double u = gg[0] * r[0] + gg[1] * r[1];
+          u += gg[2] * r[2];
+          cvc[i] = u;
+        }
+      }
+      la[f] = dfd - (cv < 0.0 ? cv : 0.0);
+      sP += cv;
+#pragma unroll
+      for (int i = 0; i < 3; i++) p[i] += dfdc[i] - cv * cvc[i];
+      double pFace;                          // facePressure
+      if (o.so_pressure) {
+        const double p1 = presC + gpC[0] * xc[0] + gpC[1] * xc[1] + gpC[2] * xc[2];
+        const double p2 = presV + gpV[0] * rv[0] + gpV[1] * rv[1] + gpV[2] * rv[2];
+        pFace = 0.5 * (p1 + p2);
+      } else {
+        pFace = am1 * presC + alpha * presV;
+      }
+      const double tp = pFace * A;
+#pragma unroll
+      for (int j = 0; j < 3; j++) pf[j] += tp * n[j];
+      if (o.viscosity) {                     // viscosityPartExp, no divergence
+        const double aP = viscC * A;
+#pragma unroll
+        for (int j = 0; j < 3; j++)
+          p[j] += aP * (gc[j] * n[0] + gc[3 + j] * n[1] + gc[6 + j] * n[2]);
+      }
+    } else {
+      la[f] = 0.0;
+      const double tp = presC * A;
+#pragma unroll
+      for (int j = 0; j < 3; j++) pf[j] += tp * n[j];
+#pragma unroll
+      for (int j = 0; j < 3; j++) s[j] = A * n[j];
+      diffusion_et(o.diffusion, s, k, e, t);
+      const int lvi = (a.bc[q] & 0xffff) - 1;
+      if (lvi >= 0)
+        vel_load_mom(a.lv[lvi], sPc, p, vc, gvC, viscC, effC, s, n, xc, rhoC,
+                     o.wall_model ? a.wall + c * 4 : nullptr, o.wall_model, A, w);
+    }
+  }
+  if (o.time) {
+    if (!o.backward) {
+      sP += rhoC * V / o.dt;
+    } else {
+      const double tmp = 1.0 / o.dt + 1.0 / (o.dt + o.dt0);
+      sP += tmp * rhoC * V;
+    }
+  }
+  double lAn = 0.0;
+  for (int f = 0; f < 6; f++)
+    if (f < nf) lAn += la[f];
+  double ap[3], rcl[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    ap[k] = sP + sPc[k] + lAn;
+    ap[k] = ap[k] / o.underU;
+    p[k] += (1 - o.underU) * ap[k] * vc[k];
+    p[k] -= pf[k];
+  }
+  for (int f = 0; f < 6; f++) {
+    if (f >= nf) break;
+    const int nb = a.nb[(int64_t)f * a.numel + c];
+    if (nb > -1) {
+      const bool fixed = a.id[nb] - 1 == -2;
+#pragma unroll
+      for (int k = 0; k < 3; k++) {
+        if (fixed) p[k] += la[f] * a.vel[nb * 3 + k];
+        else rcl[k] += la[f] * a.vel[nb * 3 + k];
+      }
+    }
+  }
+  double *out = a.lA + c * (a.maxViz + 1);
+  for (int f = 0; f < 6; f++)
+    if (f < nf) out[f] = -la[f];
+  out[a.maxViz] = ap[0];
+  const int32_t row = a.id[c] - 1;
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    rcl[k] += p[k] - ap[k] * vc[k];
+    a.dout[c * 3 + k] = o.simplec ? V / (ap[k] - lAn) : V / ap[k];
+    a.rc[k * a.numel + c] = rcl[k];
+    if (row > -1 && row < a.neq) {
+      a.ad[(int64_t)k * a.neq + row] = ap[k];
+      a.b[(int64_t)k * a.neq + row] = o.residual ? rcl[k] : p[k];
+    }
+  }
+}
+
 // updateCellValueBlock (src/CellLoop.c:4087-4130), ndf 1: cell values from
 // a solution vector through the equation numbers
 __global__ __launch_bounds__(SFR_BLOCK) void k_cell_value(int64_t n,
@@ -247,6 +565,26 @@ extern "C" int sfr_k_simple_update(hipStream_t s, int64_t n, double alpha,
   if (n <= 0) return hipSuccess;
   hipLaunchKernelGGL(k_simple_update, dim3(cell_grid(n)), dim3(SFR_BLOCK), 0, s,
                      n, alpha, w, p, pc, gpc, d);
+  return hipGetLastError();
+}
+
+extern "C" int sfr_k_cell_vel(hipStream_t s, int numel, int maxViz,
+                              const sfr_vel_opts *o, const signed char *nfacev,
+                              const int32_t *nb, const int32_t *bc,
+                              const double *g, const double *rho,
+                              const double *visc, const double *vol,
+                              const sfr_load *lv, const int32_t *id, int neq,
+                              const double *pres, const double *gp,
+                              const double *vel, const double *gv,
+                              const double *eddy, const double *wall,
+                              const double *din, double *dout, double *lA,
+                              double *ad, double *b, double *rc)
+{
+  if (numel <= 0) return hipSuccess;
+  const CellVel a = {*o, numel, maxViz, neq, nfacev, nb, bc, g, rho, visc, vol,
+                     lv, id, pres, gp, vel, gv, eddy, wall, din, dout, lA, ad,
+                     b, rc};
+  hipLaunchKernelGGL(k_cell_vel, dim3(cell_grid(numel)), dim3(SFR_BLOCK), 0, s, a);
   return hipGetLastError();
 }
 

@@ -75,7 +75,7 @@ int sfr_init(sfr_context **out, int device)
   ctx->device = device;
   ctx->chunk = 16;
   ctx->ru_delta = 0.1;
-  ctx->ru_delta_b = 1e-3;
+  ctx->ru_delta_b = 0.1;
   if (hipSetDevice(device) != hipSuccess) { free(ctx); return SFR_NO_DEVICE; }
   int rc = SFR_OK;
 #define INIT_TRY(call) do { if ((call) != hipSuccess) { rc = SFR_HIP_ERROR; goto fail; } } while (0)

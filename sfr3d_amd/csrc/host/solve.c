@@ -593,7 +593,22 @@ static int solve_bicg_mixed(sfr_context *ctx, sfr_matrix *A, double tol,
   CHECK(mb_plan_make(ctx, A, 1, NULL, &pl));
   TRY(hipEventRecord(ctx->ev_start, ctx->stream));
   CHECK(mb_loop(ctx, A, &pl, 1, A->ad, d_b, d_x, tol, maxit, newX));
-  return finish(ctx, &pl.v, pl.g, d_b, d_x, st);
+  sfr_stats s32;
+  const int rc = finish(ctx, &pl.v, pl.g, d_b, d_x, &s32);
+  if (rc != SFR_BREAKDOWN) {
+    if (st) *st = s32;
+    return rc;
+  }
+  /* the FP32 loop stagnated or broke down: FP64 from its last x (from zero
+   * when that x is not finite) */
+  sfr_stats s64;
+  const int rc64 = solve_bicg64(ctx, A, tol, maxit, d_b, d_x, isfinite(s32.norm_x) ? 0 : 1,
+                                &s64);
+  s64.iters += s32.iters;
+  s64.outer = s32.outer;
+  s64.time_s += s32.time_s;
+  if (st) *st = s64;
+  return rc64;
 }
 
 /* Launch geometry of the mixed loop.  From 2^21 rows up it runs 2 rows per
@@ -801,13 +816,32 @@ static int shared_run(sfr_context *ctx, sfr_matrix *A, int nsys,
   TRY(hipStreamSynchronize(s));
   float ms = 0.0f;
   TRY(hipEventElapsedTime(&ms, ctx->ev_start, ctx->ev_stop));
-  int worst = SFR_OK;
+  int worst = SFR_OK, broke = 0;
+  sfr_stats all[SFR_BATCH_MAX];
   for (int c = 0; c < nsys; c++) {
-    sfr_stats st;
-    sfr_fill_stats(&ctx->h_sc[c], &st, ms);
-    if (stats) stats[c] = st;
-    if (st.status != SFR_OK && worst == SFR_OK) worst = st.status;
+    sfr_fill_stats(&ctx->h_sc[c], &all[c], ms);
+    if (all[c].status != SFR_OK && worst == SFR_OK) worst = all[c].status;
+    broke |= all[c].status == SFR_BREAKDOWN;
   }
+  if (mixed && broke) {
+    /* a system's FP32 loop stagnated or broke down: the batch again in
+     * FP64 from its solutions (a converged system stops at once; a
+     * non-finite x starts from zero) */
+    for (int c = 0; c < nsys; c++)
+      if (!isfinite(all[c].norm_x))
+        TRY(sfr_k_fill(s, n, 0.0, d_x + (size_t)c * (size_t)n));
+    sfr_stats s64[SFR_BATCH_MAX];
+    worst = shared_run(ctx, A, nsys, d_ad, d_b, d_x, valnb, tol, maxit, 0,
+                       SFR_PREC_FP64, s64);
+    for (int c = 0; c < nsys; c++) {
+      s64[c].iters += all[c].iters;
+      s64[c].outer = all[c].outer;
+      s64[c].time_s += all[c].time_s;
+      all[c] = s64[c];
+    }
+  }
+  if (stats)
+    for (int c = 0; c < nsys; c++) stats[c] = all[c];
   return worst;
 }
 

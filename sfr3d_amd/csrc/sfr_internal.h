@@ -230,6 +230,15 @@ int sfr_k_cell_pres(hipStream_t s, int numel, int maxViz, int diffusion,
                     const double *gp, const double *vel, const double *df,
                     double *lA, double *lB, double *rc);
 
+/* SIMPLE momentum cell loop, one work-item per cell (planes 0-14) */
+int sfr_k_cell_vel(hipStream_t s, int numel, int maxViz, const sfr_vel_opts *o,
+                   const signed char *nfacev, const int32_t *nb,
+                   const int32_t *bc, const double *g, const double *rho,
+                   const double *visc, const double *vol, const sfr_load *lv,
+                   const int32_t *id, int neq, const double *pres,
+                   const double *gp, const double *vel, const double *gv,
+                   const double *eddy, const double *wall, const double *din,
+                   double *dout, double *lA, double *ad, double *b, double *rc);
 /* u[c] (+)= x[id[c]-1] for the cells with an equation */
 int sfr_k_cell_value(hipStream_t s, int64_t n, const int32_t *id,
                      const double *x, double *u, int add);

@@ -153,12 +153,12 @@ class Cells:
     cell loop on device fields into lA / lB / rCell."""
 
     _MESH = ("nFace", "nelcon", "cellFace", "owner", "volume", "dcca", "xmcc",
-             "mksi", "ksi", "area", "normal", "vSkew")
+             "mksi", "ksi", "area", "normal", "vSkew", "cc", "xm")
 
     def __init__(self, ctx, mesh, prop, mat, faceRvel, faceRpres, loadsVel,
                  loadsPres):
         self.ctx = ctx
-        keep = {k: np.ascontiguousarray(mesh[k]) for k in self._MESH}
+        keep = {k: np.ascontiguousarray(mesh[k]) for k in self._MESH if k in mesh}
         keep["mat"] = np.ascontiguousarray(mat, np.int16)
         keep["faceRvel"] = np.ascontiguousarray(faceRvel, np.int16)
         keep["faceRpres"] = np.ascontiguousarray(faceRpres, np.int16)
@@ -181,6 +181,17 @@ class Cells:
             self.ctx.h, self.h, diffusion, d_pres.ptr, d_gradPres.ptr,
             d_vel.ptr, d_dField.ptr, d_lA.ptr, d_lB.ptr,
             d_rCell.ptr if d_rCell else None))
+
+    def vel(self, opts, d_id, neq, d_pres, d_gradPres, d_vel, d_gradVel,
+            d_eddyVisc, d_wallPar, d_dField, d_dOut, d_lA, d_ad, d_b, d_rCell):
+        """The momentum cell loop: opts a dict with the sfr_vel_opts fields."""
+        o = L.SfrVelOpts(**{k: opts[k] for k, _ in L.SfrVelOpts._fields_})
+        self.ctx._check(lib.sfr_cells_vel_device(
+            self.ctx.h, self.h, C.byref(o), d_id.ptr, neq, d_pres.ptr,
+            d_gradPres.ptr, d_vel.ptr, d_gradVel.ptr,
+            d_eddyVisc.ptr if d_eddyVisc else None,
+            d_wallPar.ptr if d_wallPar else None, d_dField.ptr, d_dOut.ptr,
+            d_lA.ptr, d_ad.ptr, d_b.ptr, d_rCell.ptr))
 
     def destroy(self):
         if self.h:

@@ -674,3 +674,71 @@ def simple_pres_case(mesh, seed=1, diffusion=1, boundary="mixed",
             "loadsVel": loadsVel, "loadsPres": loadsPres,
             "diffusion": diffusion,
             "id": (c + 1).astype(np.int32), "num": (c + 1).astype(np.int32)}
+
+
+def simple_vel_case(mesh, seed=1, advection=3, diffusion=4, boundary="cavity",
+                    rhie=True, so_pressure=True, viscosity=True, residual=True,
+                    simplec=False, time="backward", turb=True, wall_model=False,
+                    fixed_cell=None):
+    """Fields, materials, markers, loads and model options for the SIMPLE
+    momentum cell loop on a hex_mesh, as SFR3D holds them: vel, gradVel
+    (9 per cell, d u_i / d x_j at i*3 + j), pres, gradPres, the d field,
+    eddy viscosity and wall parameters (4 per cell), mat / prop (viscosity
+    at index 0, density at 1), faceRvel / faceRpres, the velocity load
+    table, id / num and the options of cellSimpleVel3D (advection 1 FoUp,
+    2 Cd, 3 SoUp; diffusion 1-4; time "steady", "euler" or "backward").
+
+    boundary "cavity": moving lid (z+) and static walls.  "channel": inlet
+    at x-, outlet at x+, moving lid, static walls elsewhere.  fixed_cell: a
+    cell with prescribed velocity (no equation, id -1): its neighbours move
+    its coefficient to the right-hand side."""
+    nx, ny, nz = mesh["nx"], mesh["ny"], mesh["nz"]
+    numel, mv = mesh["numel"], mesh["maxViz"]
+    c = np.arange(numel)
+    i, j, k = c % nx, (c // nx) % ny, c // (nx * ny)
+    mat = np.where(k < max(nz // 2, 1), 1, 2).astype(np.int16)
+    prop = np.zeros(20)
+    prop[0], prop[1] = 1e-2, 1.0
+    prop[10], prop[11] = 1.5e-2, 1.2
+    u = lambda s, n: u01(seed, s, np.arange(n))
+    vel = 2.0 * u(11, 3 * numel) - 1.0
+    gradVel = 2.0 * u(12, 9 * numel) - 1.0
+    pres = 2.0 * u(13, numel) - 1.0
+    gradPres = 2.0 * u(14, 3 * numel) - 1.0
+    dField = 1e-3 * (0.5 + u(15, 3 * numel))
+    eddy = 1e-3 * u(16, numel)
+    wall = np.zeros(4 * numel)
+    wall[0::4] = 0.02 * u(17, numel)          # y+ either side of 0.01
+    wall[1::4] = 1.0 + u(18, numel)           # u+
+    fv = np.zeros((numel, mv + 1), np.int16)
+    fp = np.zeros((numel, mv + 1), np.int16)
+    side = [k == 0, k == nz - 1, j == 0, i == nx - 1, j == ny - 1, i == 0]
+    loads = [{"type": MOVEWALL, "par": [1.0, 0.2, 0.0, 0.0]},
+             {"type": STATICWALL}]
+    for f in range(6):
+        fv[side[f], f] = 2
+    fv[side[1], 1] = 1
+    if boundary == "channel":
+        loads += [{"type": INLET, "par": [0.8, 0.05, -0.02, 1.1]},
+                  {"type": OUTLET}]
+        fv[side[5], 5] = 3
+        fv[side[3], 3] = 4
+    id_ = (c + 1).astype(np.int32)
+    if fixed_cell is not None:
+        fv[fixed_cell, mesh["nFace"][fixed_cell]] = 1
+        id_[fixed_cell] = -1
+        id_[fixed_cell + 1:] -= 1
+    t = {"steady": (0, 0), "euler": (1, 0), "backward": (1, 1)}[time]
+    opts = {"advection": advection, "diffusion": diffusion, "turb": int(turb),
+            "wall_model": int(wall_model), "rhie": int(rhie),
+            "so_pressure": int(so_pressure), "viscosity": int(viscosity),
+            "residual": int(residual), "simplec": int(simplec),
+            "time": t[0], "backward": t[1], "dt": 0.05, "dt0": 0.04,
+            "underU": 0.7}
+    return {"mat": mat, "prop": prop, "vel": vel, "gradVel": gradVel,
+            "pres": pres, "gradPres": gradPres, "dField": dField,
+            "eddyVisc": eddy, "wallPar": wall,
+            "faceRvel": np.ascontiguousarray(fv.reshape(-1)),
+            "faceRpres": np.ascontiguousarray(fp.reshape(-1)),
+            "loadsVel": loads, "id": id_, "num": (c + 1).astype(np.int32),
+            "opts": opts}

@@ -1,4 +1,7 @@
-"""Writes tests/golden/ref_simple_pres.npz: outputs of the reference's own
+"""Writes tests/golden/ref_simple_pres.npz and ref_simple_vel.npz: outputs of
+the reference's own compiled SIMPLE cell loops.
+
+ref_simple_pres.npz: outputs of the reference's own
 compiled SIMPLE pressure-correction cell loop (systFormSimplePres,
 src/CellLoopInc.c:471-679, with cellSimplePres3D, src/CellPres.c:903-1149,
 through oracle/_ref) on a small warped hex mesh, assembled by its own
@@ -26,6 +29,47 @@ MESH_KEYS = ("nFace", "nelcon", "cellFace", "owner", "volume", "dcca", "xmcc",
              "mksi", "ksi", "eta", "area", "normal", "xm", "mvSkew", "vSkew")
 CASE_KEYS = ("mat", "prop", "pres", "gradPres", "vel", "dField", "faceRvel",
              "faceRpres", "id", "num")
+
+
+VEL_CASES = [dict(rhie=False),
+             dict(rhie=False, advection=1, diffusion=1, time="euler", simplec=True,
+                  residual=False, so_pressure=False, viscosity=False, turb=False),
+             dict(rhie=False, advection=2, diffusion=2, boundary="channel",
+                  wall_model=True, fixed_cell=7, time="steady"),
+             dict(rhie=False, diffusion=3, boundary="channel"),
+             dict(rhie=True),
+             dict(rhie=True, boundary="channel", wall_model=True, fixed_cell=11)]
+VEL_KEYS = ("mat", "prop", "vel", "gradVel", "pres", "gradPres", "dField",
+            "eddyVisc", "wallPar", "faceRvel", "faceRpres", "id", "num")
+
+
+def vel_main(m):
+    """ref_simple_vel.npz: systFormSimpleVel (src/CellLoopInc.c:98-385,
+    cellSimpleVel3D, src/CellSimple.c:106-638) on the same mesh, CSRD unsym,
+    over the option sets of VEL_CASES: the reference's al, ad, b (3*neq),
+    rCell (3*numel) and its d field after the loop (updated in place)."""
+    import oracle
+    from sfr3d_amd import systems as Y
+    out = {f"mesh_{k}": m[k] for k in MESH_KEYS + ("cc",)}
+    out["mesh_dims"] = np.array([m["nx"], m["ny"], m["nz"], m["numel"],
+                                 m["nface"], m["maxViz"]], np.int64)
+    for i, kw in enumerate(VEL_CASES):
+        case = Y.simple_vel_case(m, seed=3 + i, **kw)
+        ia, ja, al, ad, b, rc, d = oracle.ref_simple_vel(m, case)
+        for k in VEL_KEYS:
+            out[f"v{i}_{k}"] = case[k]
+        t, npar, par, dens = oracle.load_table(case["loadsVel"])
+        out[f"v{i}_lv_type"], out[f"v{i}_lv_np"] = t, npar
+        out[f"v{i}_lv_par"], out[f"v{i}_lv_density"] = par, dens
+        out[f"v{i}_lv_n"] = np.array([len(case["loadsVel"])])
+        out[f"v{i}_opt"], out[f"v{i}_optd"] = oracle.vel_options(case)
+        out[f"v{i}_ia"], out[f"v{i}_ja"] = ia, ja
+        out[f"v{i}_al"], out[f"v{i}_ad"], out[f"v{i}_b"] = al, ad, b
+        out[f"v{i}_rCell"], out[f"v{i}_dOut"] = rc, d
+    out["ncases"] = np.array([len(VEL_CASES)])
+    path = os.path.join(HERE, "ref_simple_vel.npz")
+    np.savez_compressed(path, **out)
+    print("wrote", path, os.path.getsize(path), "bytes")
 
 
 def main():
@@ -56,6 +100,7 @@ def main():
     path = os.path.join(HERE, "ref_simple_pres.npz")
     np.savez_compressed(path, **out)
     print("wrote", path, os.path.getsize(path), "bytes")
+    vel_main(m)
 
 
 if __name__ == "__main__":

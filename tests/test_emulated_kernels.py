@@ -203,6 +203,43 @@ def test_cells_emulated(emu_lib, tmp_path):
         d[p + "dims"] = np.array([m["numel"], m["nface"], m["maxViz"], diff])
         d[p + "lA"], d[p + "lB"], d[p + "rCell"] = lA, lB, rc
     d["ncases"] = np.array([len(cases)])
+    # the momentum loop: oracle in its order-free form (every neighbour's
+    # input d), as a parallel loop computes it
+    vcases = [dict(), dict(advection=1, diffusion=1, time="euler", simplec=True,
+                           residual=False, so_pressure=False, viscosity=False),
+              dict(advection=2, diffusion=2, boundary="channel", wall_model=True,
+                   fixed_cell=40, time="steady")]
+    for i, kw in enumerate(vcases):
+        case = Y.simple_vel_case(m, seed=31 + i, **kw)
+        lA, lB, rc, dn = oracle.simple_vel3d(m, case, in_place=False)
+        p = f"v{i}_"
+        for k in keys + ("cc", "xm"):
+            d[p + k] = m[k]
+        for k in ("mat", "faceRvel", "faceRpres", "prop", "pres", "gradPres",
+                  "vel", "gradVel", "eddyVisc", "wallPar", "dField", "id"):
+            d[p + k] = case[k]
+        for name, loads in (("lv", case["loadsVel"]), ("lp", [])):
+            t, npar, par, dens = oracle.load_table(loads)
+            d[p + name + "_type"], d[p + name + "_np"] = t, npar
+            d[p + name + "_par"], d[p + name + "_density"] = par, dens
+            d[p + name + "_n"] = np.array([len(loads)])
+        d[p + "opt"], d[p + "optd"] = oracle.vel_options(case)
+        d[p + "dims"] = np.array([m["numel"], m["nface"], m["maxViz"]])
+        numel, neq = m["numel"], int(case["id"].max())
+        l1 = np.zeros((numel, 7))
+        skip = case["faceRvel"].reshape(-1, 7)[np.arange(numel), m["nFace"]] > 0
+        l1[:, :6] = lA[:, :6]
+        l1[:, 6] = lA[:, 6]
+        l1[skip] = 0.0
+        ad = np.zeros(3 * neq); b = np.zeros(3 * neq)
+        rows = case["id"] - 1
+        ok = (rows >= 0) & ~skip
+        for k in range(3):
+            ad[k * neq + rows[ok]] = lA[ok, 6 + k]
+            b[k * neq + rows[ok]] = lB[ok, k]
+        d[p + "lA1"], d[p + "ad"], d[p + "b"] = l1.ravel(), ad, b
+        d[p + "rCell"], d[p + "dOut"] = rc.ravel(), dn.ravel()
+    d["nvel"] = np.array([len(vcases)])
     path = str(tmp_path / "cells.npz")
     np.savez(path, **d)
     res = subprocess.run([sys.executable, os.path.join(HERE, "emu", "run_cells.py"),
@@ -210,4 +247,5 @@ def test_cells_emulated(emu_lib, tmp_path):
                          env=_env(), timeout=280)
     print(res.stdout)
     assert res.returncode == 0, res.stdout + res.stderr[-3000:]
-    assert res.stdout.count("PASS") == 3 * len(cases) and "FAIL" not in res.stdout
+    assert res.stdout.count("PASS") == 3 * len(cases) + 2 * len(vcases)
+    assert "FAIL" not in res.stdout

@@ -18,6 +18,7 @@ import pytest
 import oracle
 import sfr3d_amd as S
 from sfr3d_amd import structure as ST, systems as Y
+from sfr3d_amd._lib import lib
 
 from tests.test_cells import golden_cases
 
@@ -179,3 +180,129 @@ def test_outer_loop_stays_on_device(ctx):
     for a in list(d.values()) + [d_id, d_gpc, d_lA, d_lB, d_b, d_x, d_pc, hb, hx]:
         a.free()
     C.destroy(); plan.destroy(); A_dev.destroy(); A_host.destroy()
+
+
+# ---- the momentum cell loop ---------------------------------------------------
+
+from tests.test_cells import golden_vel_cases, vel_assemble  # noqa: E402
+import ctypes as C  # noqa: E402
+
+VEL_FIELDS = ("pres", "gradPres", "vel", "gradVel", "dField")
+
+
+def device_vel(ctx, mesh, case, cells=None):
+    numel, mv = mesh["numel"], mesh["maxViz"]
+    C_ = cells or ctx.cells(mesh, dict(case, loadsPres=[]))
+    neq = int(case["id"].max())
+    d = {k: ctx.array(np.ascontiguousarray(case[k], np.float64)) for k in VEL_FIELDS}
+    o = case["opts"]
+    d_eddy = ctx.array(case["eddyVisc"]) if o["turb"] else None
+    d_wall = ctx.array(case["wallPar"]) if o["wall_model"] else None
+    d_id = ctx.array(np.ascontiguousarray(case["id"], np.int32))
+    d_out = ctx.array(np.zeros(3 * numel))
+    d_lA = ctx.array(np.zeros(numel * (mv + 1)))
+    d_ad, d_b = ctx.array(np.zeros(3 * neq)), ctx.array(np.zeros(3 * neq))
+    d_rc = ctx.array(np.full(3 * numel, 7.0))
+    C_.vel(o, d_id, neq, d["pres"], d["gradPres"], d["vel"], d["gradVel"], d_eddy,
+           d_wall, d["dField"], d_out, d_lA, d_ad, d_b, d_rc)
+    out = (d_lA.get().reshape(numel, mv + 1), d_ad.get(), d_b.get(),
+           d_rc.get(), d_out.get())
+    for a in list(d.values()) + [d_id, d_out, d_lA, d_ad, d_b, d_rc, d_eddy, d_wall]:
+        if a is not None:
+            a.free()
+    if cells is None:
+        C_.destroy()
+    return out
+
+
+def oracle_vel_expect(mesh, case):
+    """The oracle's order-free momentum loop in the device's output layout."""
+    lA, lB, rc, dn = oracle.simple_vel3d(mesh, case, in_place=False)
+    pat, al, ad, b = vel_assemble(mesh, case, lA, lB)
+    numel, mv = mesh["numel"], mesh["maxViz"]
+    skip = case["faceRvel"].reshape(numel, -1)[np.arange(numel), mesh["nFace"]] > 0
+    l1 = np.zeros((numel, mv + 1))
+    l1[~skip] = lA[~skip, :mv + 1]
+    return l1, ad, b, rc.ravel(), dn.ravel()
+
+
+@pytest.mark.parametrize("i", range(6))
+def test_vel_device_matches_reference_golden(ctx, i):
+    """Without Rhie-Chow interpolation the device's coefficients are the
+    reference's own (assembled by the bit-exact host assembler); with it, the
+    device takes every neighbour's input d and is compared with the oracle's
+    order-free form (itself pinned against the reference where the two
+    orders agree)."""
+    mesh, case, exp = golden_vel_cases()[i]
+    print(f"momentum golden case {i} {case['opts']}", flush=True)
+    lA, ad, b, rc, dn = device_vel(ctx, mesh, case)
+    if not case["opts"]["rhie"]:
+        numel = mesh["numel"]
+        nel = mesh["nelcon"].reshape(-1, 6)
+        neq = int(case["id"].max())
+        pat = ST.datastruct(2, 1, case["id"], case["num"], nel, mesh["nFace"], neq)
+        al, _, _ = ST.assemble(pat, case["id"], nel, mesh["nFace"], lA, np.zeros(numel))
+        assert np.array_equal(al[:len(exp["al"])], exp["al"])
+        assert np.array_equal(ad, exp["ad"]) and np.array_equal(b, exp["b"])
+        assert np.array_equal(rc, exp["rCell"]) and np.array_equal(dn, exp["dOut"])
+    l1, oad, ob, orc, odn = oracle_vel_expect(mesh, case)
+    assert np.array_equal(lA, l1)
+    assert np.array_equal(ad, oad) and np.array_equal(b, ob)
+    assert np.array_equal(rc, orc) and np.array_equal(dn, odn)
+
+
+@pytest.mark.parametrize("kw", [dict(), dict(advection=1, diffusion=1, time="euler",
+                                             simplec=True, residual=False),
+                                dict(advection=2, diffusion=2, boundary="channel",
+                                     wall_model=True, fixed_cell=777, time="steady"),
+                                dict(diffusion=3, so_pressure=False, viscosity=False)])
+def test_vel_device_matches_oracle(ctx, kw):
+    m = Y.hex_mesh(24, 20, 18, ratio=(1.02, 0.98, 1.01), shear=(0.1, 0.05, -0.1),
+                   warp=0.03)
+    case = Y.simple_vel_case(m, seed=9, **kw)
+    print(f"momentum 24x20x18 {kw}", flush=True)
+    got = device_vel(ctx, m, case)
+    exp = oracle_vel_expect(m, case)
+    for g, e in zip(got, exp):
+        assert np.array_equal(g, e)
+
+
+def test_vel_chain_shared_solve(ctx):
+    """u, v, w coefficients made on the device, the shared off-diagonals
+    scattered by the device assembly, the three diagonals and right-hand
+    sides written in csrSimple's layout: the u/v/w BiCGSTAB batch solves
+    bitwise like the host chain (oracle coefficients, host assembly)."""
+    m = Y.hex_mesh(32, 30, 28, ratio=(1.01, 1.0, 0.99), warp=0.02)
+    case = Y.simple_vel_case(m, seed=21, boundary="channel", rhie=True)
+    numel, neq = m["numel"], int(case["id"].max())
+    nel = m["nelcon"].reshape(-1, 6)
+    print(f"momentum chain {numel} cells", flush=True)
+    l1, ad, b, _, _ = oracle_vel_expect(m, case)
+    pat = ST.datastruct(S.CSRD, 1, case["id"], case["num"], nel, m["nFace"], neq)
+    al, _, _ = ST.assemble(pat, case["id"], nel, m["nFace"], l1, np.zeros(numel))
+    sysm = S.System(neq, pat.nad, pat.ia, pat.ja, al, ad[:neq].copy(), S.CSRD, True)
+    A_host, A_dev = ctx.matrix(sysm), ctx.matrix(sysm)
+    plan = ctx.assembly(A_dev, pat, case["id"], nel, m["nFace"])
+    cells = ctx.cells(m, dict(case, loadsPres=[]))
+    d = {k: ctx.array(np.ascontiguousarray(case[k], np.float64)) for k in VEL_FIELDS}
+    d_eddy, d_id = ctx.array(case["eddyVisc"]), ctx.array(case["id"])
+    d_out, d_lA = ctx.array(np.zeros(3 * numel)), ctx.array(np.zeros(numel * 7))
+    d_ad, d_b, d_rc = ctx.empty(3 * neq), ctx.empty(3 * neq), ctx.empty(3 * numel)
+    cells.vel(case["opts"], d_id, neq, d["pres"], d["gradPres"], d["vel"], d["gradVel"],
+              d_eddy, None, d["dField"], d_out, d_lA, d_ad, d_b, d_rc)
+    plan.assemble(d_lA)
+    h_ad, h_b = ctx.array(ad), ctx.array(b)
+    x1, x2 = ctx.array(np.zeros(3 * neq)), ctx.array(np.zeros(3 * neq))
+    st1, st2 = (S._lib.SfrStats * 3)(), (S._lib.SfrStats * 3)()
+    rc1 = lib.sfr_solve_shared_device(ctx.h, A_host.h, 3, h_ad.ptr, h_b.ptr, x1.ptr,
+                                      S.PBICGSTAB, 1e-8, 2000, 1, st1)
+    rc2 = lib.sfr_solve_shared_device(ctx.h, A_dev.h, 3, d_ad.ptr, d_b.ptr, x2.ptr,
+                                      S.PBICGSTAB, 1e-8, 2000, 1, st2)
+    assert rc1 == rc2 == 0
+    assert [st1[c].iters for c in range(3)] == [st2[c].iters for c in range(3)]
+    assert np.array_equal(d_ad.get(), ad) and np.array_equal(d_b.get(), b)
+    assert np.array_equal(x1.get(), x2.get())
+    for a in list(d.values()) + [d_eddy, d_id, d_out, d_lA, d_ad, d_b, d_rc, h_ad,
+                                 h_b, x1, x2]:
+        a.free()
+    cells.destroy(); plan.destroy(); A_host.destroy(); A_dev.destroy()

@@ -128,3 +128,28 @@ def test_mixed_option_reaches_the_dropin_path(ctx):
         x, st = ctx.solve(M.system, S.GMRES, TOL, 5000, M.b)
         xo, so = oracle.gmres(M.system, M.b, TOL, 5000)
         assert st["outer"] == so["cycles"] and relerr(x, xo) < 1e-6
+
+
+@pytest.mark.parametrize("n,kmin", [(48, 1e-2), (64, 1e-3)])
+def test_mixed_bicgstab_long_solve(ctx, n, kmin):
+    """A long solve (a variable-coefficient Poisson system, hundreds of
+    iterations): the FP32 loop restarts at each fold and, should its FP64
+    residual stop improving, the solve is finished in FP64 within the same
+    call -- either way the result meets the reference's stop test and agrees
+    with the FP64 BiCGSTAB solution."""
+    B = systems.poisson3d(n, seed=2, kmin=kmin, reorder=True, storage=S.ELLPACK)
+    print(f"mixed bicgstab long {n}^3 kmin {kmin}", flush=True)
+    A = ctx.matrix(B.system)
+    db, dx = ctx.array(B.b), ctx.empty(B.system.neq)
+    st64 = ctx.solve_device(A, S.PBICGSTAB, TOL, 20000, db, dx)
+    x64 = dx.get()
+    st = ctx.solve_device(A, S.PBICGSTAB, TOL, 20000, db, dx, precision=S.PREC_MIXED)
+    x = dx.get()
+    print(f"  fp64 {st64['iters']} iterations; mixed {st['iters']} iterations, "
+          f"{st['outer']} folds, status {st['status']}", flush=True)
+    assert st64["status"] == st["status"] == S._lib.OK
+    assert st["relres"] < TOL
+    # both meet ||b - Ax|| < tol ||b||; their distance is bounded by the
+    # condition number times tol, far below this on these systems
+    assert relerr(x, x64) < 1e-4
+    db.free(); dx.free(); A.destroy()
