@@ -338,19 +338,21 @@ __global__ __launch_bounds__(SFR_BLOCK) void k_cell_vel(CellVel a)
   const double rhoC = a.rho[c], viscC = a.visc[c];
   const double effC = viscC + (o.turb ? a.eddy[c] : 0.0);
   const double presC = a.pres[c], V = a.vol[c];
-  double vc[3], gpC[3], dC[3], gvC[9];
+  double vc[3], gpC[3], dC[3];
 #pragma unroll
   for (int j = 0; j < 3; j++) {
     vc[j] = a.vel[c * 3 + j];
     gpC[j] = a.gp[c * 3 + j];
     dC[j] = a.din[c * 3 + j];
   }
-#pragma unroll
-  for (int j = 0; j < 9; j++) gvC[j] = a.gv[c * 9 + j];
+  // the velocity gradients are read a row at a time where they are used
+  // (from cache), which keeps the live set small enough not to spill
+  const double *gvC = a.gv + c * 9;
   double sP = 0.0, p[3] = {0.0, 0.0, 0.0}, pf[3] = {0.0, 0.0, 0.0},
          sPc[3] = {0.0, 0.0, 0.0}, la[6];
+#pragma unroll
   for (int f = 0; f < 6; f++) {
-    if (f >= nf) break;
+    if (f < nf) {
     const int64_t q = (int64_t)f * a.numel + c;
     const double n[3] = {a.g[q], a.g[S + q], a.g[2 * S + q]};
     const double k[3] = {a.g[3 * S + q], a.g[4 * S + q], a.g[5 * S + q]};
@@ -364,14 +366,6 @@ __global__ __launch_bounds__(SFR_BLOCK) void k_cell_vel(CellVel a)
       const double effV = viscV + (o.turb ? a.eddy[nb] : 0.0);
       const double presV = a.pres[nb];
       const double *gvV = a.gv + (int64_t)nb * 9;
-      double vV[3], gpV[3], dV[3], duDksi[3];
-#pragma unroll
-      for (int j = 0; j < 3; j++) {
-        vV[j] = a.vel[nb * 3 + j];
-        gpV[j] = a.gp[nb * 3 + j];
-        dV[j] = a.din[nb * 3 + j];
-        duDksi[j] = (vV[j] - vc[j]) / mk;
-      }
 #pragma unroll
       for (int j = 0; j < 3; j++) s[j] = A * n[j];
       diffusion_et(o.diffusion, s, k, e, t);
@@ -381,22 +375,17 @@ __global__ __launch_bounds__(SFR_BLOCK) void k_cell_vel(CellVel a)
                           : 1.0 / (alpha / effC + am1 / effV);
       const double modE = sqrt(e[0] * e[0] + e[1] * e[1] + e[2] * e[2]);
       const double dfd = coef * modE / mk;
-      double gf[9], wf[3];
+      double wf[3];
 #pragma unroll
-      for (int i = 0; i < 3; i++) {
-#pragma unroll
-        for (int j = 0; j < 3; j++)
-          gf[i * 3 + j] = am1 * gvC[i * 3 + j] + alpha * gvV[i * 3 + j];
-        wf[i] = am1 * vc[i] + alpha * vV[i];
-      }
+      for (int i = 0; i < 3; i++) wf[i] = am1 * vc[i] + alpha * a.vel[nb * 3 + i];
       const double density = am1 * rhoC + alpha * rhoV;
       double wfn = wf[0] * n[0] + wf[1] * n[1] + wf[2] * n[2];
       if (o.rhie) {                          // interpolFaceVel
         double dF[3], gfp[3];
 #pragma unroll
         for (int j = 0; j < 3; j++) {
-          dF[j] = am1 * dC[j] + alpha * dV[j];
-          gfp[j] = am1 * gpC[j] + alpha * gpV[j];
+          dF[j] = am1 * dC[j] + alpha * a.din[nb * 3 + j];
+          gfp[j] = am1 * gpC[j] + alpha * a.gp[nb * 3 + j];
         }
         const double g0 = (presV - presC) / mk;
         const double nk = dF[0] * k[0] * n[0] + dF[1] * k[1] * n[1]
@@ -405,41 +394,50 @@ __global__ __launch_bounds__(SFR_BLOCK) void k_cell_vel(CellVel a)
         g1 = (g1 - g0) * nk;
         wfn += g1;
       }
-      double gc[9], dfdc[3];
-#pragma unroll
-      for (int i = 0; i < 3; i++) {
-        double gk = gf[i * 3] * k[0] + gf[i * 3 + 1] * k[1] + gf[i * 3 + 2] * k[2];
-        const double du = duDksi[i] - gk;
-#pragma unroll
-        for (int j = 0; j < 3; j++) gc[i * 3 + j] = gf[i * 3 + j] + du * k[j];
-        gk = gc[i * 3] * t[0] + gc[i * 3 + 1] * t[1] + gc[i * 3 + 2] * t[2];
-        dfdc[i] = coef * gk;
-      }
       const double cv = density * wfn * A;
-      double cvc[3];
+      la[f] = dfd - (cv < 0.0 ? cv : 0.0);
+      sP += cv;
+      // per velocity component i: the interpolated gradient row, its
+      // compact (Darwish-Moukalled) correction, the non-orthogonal flux and
+      // the deferred advection term; the explicit viscous flux accumulates
+      // the rows in the reference's order (row 0, 1, 2)
+      double vis[3];
 #pragma unroll
       for (int i = 0; i < 3; i++) {
+        const double vVi = a.vel[nb * 3 + i];
+        const double duDksi = (vVi - vc[i]) / mk;
+        double gf[3], gc[3];
+#pragma unroll
+        for (int j = 0; j < 3; j++)
+          gf[j] = am1 * gvC[i * 3 + j] + alpha * gvV[i * 3 + j];
+        double gk = gf[0] * k[0] + gf[1] * k[1] + gf[2] * k[2];
+        const double du = duDksi - gk;
+#pragma unroll
+        for (int j = 0; j < 3; j++) gc[j] = gf[j] + du * k[j];
+        gk = gc[0] * t[0] + gc[1] * t[1] + gc[2] * t[2];
+        const double dfdc = coef * gk;
+        double cvc;
         if (o.advection == 1) {
-          cvc[i] = 0.0;
+          cvc = 0.0;
         } else if (o.advection == 2) {       // deferredCd
-          const double cd = am1 * vc[i] + alpha * vV[i];
-          cvc[i] = cd - (cv < 0.0 ? vV[i] : vc[i]);
+          const double cd = am1 * vc[i] + alpha * vVi;
+          cvc = cd - (cv < 0.0 ? vVi : vc[i]);
         } else {                             // upwindLinearV1
           const double *gg = cv < 0.0 ? gvV + i * 3 : gvC + i * 3;
           const double *r = cv < 0.0 ? rv : xc;
           double u = gg[0] * r[0] + gg[1] * r[1];
           u += gg[2] * r[2];
-          cvc[i] = u;
+          cvc = u;
         }
-      }
-      la[f] = dfd - (cv < 0.0 ? cv : 0.0);
-      sP += cv;
+        p[i] += dfdc - cv * cvc;
 #pragma unroll
-      for (int i = 0; i < 3; i++) p[i] += dfdc[i] - cv * cvc[i];
+        for (int j = 0; j < 3; j++) vis[j] = i == 0 ? gc[j] * n[0] : vis[j] + gc[j] * n[i];
+      }
       double pFace;                          // facePressure
       if (o.so_pressure) {
         const double p1 = presC + gpC[0] * xc[0] + gpC[1] * xc[1] + gpC[2] * xc[2];
-        const double p2 = presV + gpV[0] * rv[0] + gpV[1] * rv[1] + gpV[2] * rv[2];
+        const double p2 = presV + a.gp[nb * 3] * rv[0] + a.gp[nb * 3 + 1] * rv[1]
+                          + a.gp[nb * 3 + 2] * rv[2];
         pFace = 0.5 * (p1 + p2);
       } else {
         pFace = am1 * presC + alpha * presV;
@@ -450,8 +448,7 @@ __global__ __launch_bounds__(SFR_BLOCK) void k_cell_vel(CellVel a)
       if (o.viscosity) {                     // viscosityPartExp, no divergence
         const double aP = viscC * A;
 #pragma unroll
-        for (int j = 0; j < 3; j++)
-          p[j] += aP * (gc[j] * n[0] + gc[3 + j] * n[1] + gc[6 + j] * n[2]);
+        for (int j = 0; j < 3; j++) p[j] += aP * vis[j];
       }
     } else {
       la[f] = 0.0;
@@ -466,6 +463,7 @@ __global__ __launch_bounds__(SFR_BLOCK) void k_cell_vel(CellVel a)
         vel_load_mom(a.lv[lvi], sPc, p, vc, gvC, viscC, effC, s, n, xc, rhoC,
                      o.wall_model ? a.wall + c * 4 : nullptr, o.wall_model, A, w);
     }
+    }
   }
   if (o.time) {
     if (!o.backward) {
@@ -476,6 +474,7 @@ __global__ __launch_bounds__(SFR_BLOCK) void k_cell_vel(CellVel a)
     }
   }
   double lAn = 0.0;
+#pragma unroll
   for (int f = 0; f < 6; f++)
     if (f < nf) lAn += la[f];
   double ap[3], rcl[3] = {0.0, 0.0, 0.0};
@@ -486,9 +485,9 @@ __global__ __launch_bounds__(SFR_BLOCK) void k_cell_vel(CellVel a)
     p[k] += (1 - o.underU) * ap[k] * vc[k];
     p[k] -= pf[k];
   }
+#pragma unroll
   for (int f = 0; f < 6; f++) {
-    if (f >= nf) break;
-    const int nb = a.nb[(int64_t)f * a.numel + c];
+    const int nb = f < nf ? a.nb[(int64_t)f * a.numel + c] : -1;
     if (nb > -1) {
       const bool fixed = a.id[nb] - 1 == -2;
 #pragma unroll
@@ -499,6 +498,7 @@ __global__ __launch_bounds__(SFR_BLOCK) void k_cell_vel(CellVel a)
     }
   }
   double *out = a.lA + c * (a.maxViz + 1);
+#pragma unroll
   for (int f = 0; f < 6; f++)
     if (f < nf) out[f] = -la[f];
   out[a.maxViz] = ap[0];
