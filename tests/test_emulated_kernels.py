@@ -88,6 +88,8 @@ def write_cases(path):
 
 
 ASAN = os.environ.get("SFR_EMU_ASAN") == "1"
+# time limits are for the plain build; AddressSanitizer runs about 3x slower
+SLOW = 3 if ASAN else 1
 
 
 @pytest.fixture(scope="module")
@@ -108,13 +110,13 @@ def _env():
     return env
 
 
-@pytest.mark.timeout(900)
+@pytest.mark.timeout(900 * SLOW)
 def test_kernels_under_cpu_emulation(emu_lib, tmp_path):
     cases = str(tmp_path / "cases.npz")
     write_cases(cases)
     res = subprocess.run([sys.executable, os.path.join(HERE, "emu", "run_smoke.py"),
                           emu_lib, cases], capture_output=True, text=True, env=_env(),
-                         timeout=850)
+                         timeout=850 * SLOW)
     print(res.stdout)
     assert res.returncode == 0, res.stdout + res.stderr[-3000:]
     assert "SMOKE OK" in res.stdout
@@ -132,10 +134,10 @@ def dropin_cases(tmp_path_factory):
 def _dropin(lib, cases, mode):
     return subprocess.run([sys.executable, os.path.join(HERE, "emu", "run_dropin.py"),
                            lib, cases, mode], capture_output=True, text=True,
-                          env=_env(), timeout=600)
+                          env=_env(), timeout=600 * SLOW)
 
 
-@pytest.mark.timeout(700)
+@pytest.mark.timeout(700 * SLOW)
 def test_dropin_alternating_patterns_emulated(emu_lib, dropin_cases):
     """sfr_solverC as SFR3D's SIMPLE step calls it: velocity, pressure and a
     third pattern in turn with new values each round, then two more
@@ -149,7 +151,7 @@ def test_dropin_alternating_patterns_emulated(emu_lib, dropin_cases):
     assert res.stdout.count("PASS") == 26
 
 
-@pytest.mark.timeout(300)
+@pytest.mark.timeout(300 * SLOW)
 @pytest.mark.parametrize("mode,code,out,err", [
     ("maxit_pcg", 1, "name : sistEqTest\nPCG: *** WARNING: no convergende reached !\n"
                      "MAXIT = 3 \n", ""),
@@ -170,7 +172,7 @@ def test_dropin_exit_paths_emulated(emu_lib, dropin_cases, mode, code, out, err)
     assert err in res.stderr
 
 
-@pytest.mark.timeout(300)
+@pytest.mark.timeout(300 * SLOW)
 def test_cells_emulated(emu_lib, tmp_path):
     """The device cell loop of the SIMPLE pressure correction (device/cells.hip
     with host/cells.c) on the CPU: bitwise the oracle's coefficients (the
@@ -244,7 +246,7 @@ def test_cells_emulated(emu_lib, tmp_path):
     np.savez(path, **d)
     res = subprocess.run([sys.executable, os.path.join(HERE, "emu", "run_cells.py"),
                           emu_lib, path], capture_output=True, text=True,
-                         env=_env(), timeout=280)
+                         env=_env(), timeout=280 * SLOW)
     print(res.stdout)
     assert res.returncode == 0, res.stdout + res.stderr[-3000:]
     assert res.stdout.count("PASS") == 3 * len(cases) + 2 * len(vcases)
