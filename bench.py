@@ -147,13 +147,18 @@ def main():
     t_k64 = ms.value * 1e-3
     s = info["slots"]
     ib = info["index_bytes"]
+    # value planes the FP64 SpMV streams: half of them when it reads a
+    # symmetric ELL through its face pairs (the other half from cache); the
+    # FP32 kernel reads every slot
+    pairs = A.info()["pair_planes"]
+    sv = s // 2 if pairs else s
     # algorithmic bytes per row of one launch (each stream counted once; ib =
     # bytes per stored column index, 2 for 16-bit row offsets):
-    #   fused FP32 kernel: s*(4 val + ib) + r_i, p_old_i (4+4) read,
+    #   fused FP32 kernel: s*4 val + s*ib + r_i, p_old_i (4+4) read,
     #                      p_new_i, q_i (4+4) written, y_i (4+4) read+written
-    #   FP64 SpMV        : s*(8 val + ib) + 8 (ad_i) + 8 (p_i) + 8 (q_i)
-    b32 = neq * (s * (4 + ib) + 24)
-    b64 = neq * (s * (8 + ib) + 24)
+    #   FP64 SpMV        : sv*8 val + s*ib + 8 (ad_i) + 8 (p_i) + 8 (q_i)
+    b32 = neq * (s * 4 + s * ib + 24)
+    b64 = neq * (sv * 8 + s * ib + 24)
     gbs32 = b32 / t_k32 / 1e9
     gbs64 = b64 / t_k64 / 1e9
     log(f"[bench] fused FP32 kernel {t_k32 * 1e6:.1f} us -> {gbs32:.0f} GB/s; "
@@ -161,7 +166,8 @@ def main():
 
     it64 = st64["iters"]
     itmx, outer = stmx["iters"], stmx["outer"]
-    solve_bytes64, solve_bytesmx = solve_bytes(neq, s, ib, st64, stmx)
+    solve_bytes64, solve_bytesmx = solve_bytes(neq, s, ib, st64, stmx, sv)
+    canon64, canonmx = canonical_bytes(neq, st64, stmx)
 
     relerr = float(np.linalg.norm(xmx - x64) / np.linalg.norm(x64))
     # error of both against an FP64 solve to tol 1e-13 (untimed): the stop
@@ -196,6 +202,24 @@ def main():
                  "rows_per_thread": A.info()["rows_per_thread_pcg"],
                  "rows_per_thread_default": info["rows_per_thread"],
                  "speedup_mixed_vs_fp64_tuned": round(t64t / tmx, 3)}
+
+    # the FP64 solve with every slot's value read (face pairs off): the same
+    # bits, the time of the full read
+    # (FP64 PCG, the loop that reads them; at the launch shape in force)
+    pairs_off = None
+    if not args.no_extra and pairs:
+        st1, t1_64, _ = run(S.PREC_FP64, 3, 1)
+        x_on = dx.get()
+        ctx.set_option("ell_pairs", 0)
+        st0, t0_64, _ = run(S.PREC_FP64, 3, 1)
+        same64 = bool(np.array_equal(dx.get(), x_on) and st0["iters"] == st1["iters"])
+        ctx.set_option("ell_pairs", 1)
+        pairs_off = {"fp64_solves_per_s_pairs": round(1.0 / t1_64, 4),
+                     "fp64_solves_per_s_full_read": round(1.0 / t0_64, 4),
+                     "same_bits": same64,
+                     "speedup_pairs": round(t0_64 / t1_64, 3)}
+        log(f"[bench] FP64 PCG with face pairs {1.0 / t1_64:.3f}, full read "
+            f"{1.0 / t0_64:.3f} solves/s (same bits: {same64})")
 
     bicg_long = None if args.no_extra else bench_bicg_long(ctx, S, A, db, dx, x64)
 
@@ -249,15 +273,19 @@ def main():
                   "relerr_vs_fp64_gpu": relerr,
                   "relerr_vs_tol1e-13": finite(errmx),
                   "solve_GBs_algorithmic": round(solve_bytesmx / tmx / 1e9, 1),
-                  "solve_frac": round(solve_bytesmx / tmx / 1e9 / HBM_PEAK_GBS, 4)},
+                  "solve_frac": round(solve_bytesmx / tmx / 1e9 / HBM_PEAK_GBS, 4),
+                  "solve_GBs_canonical": round(canonmx / tmx / 1e9, 1)},
         "fp64": {"solves_per_s": round(1.0 / t64, 4), "iters": it64,
                  "relres_true": st64["relres"],
                  "ref_criterion": [st64["norm_r_m"], st64["conv"]],
                  "relerr_vs_tol1e-13": finite(err64),
                  "solve_GBs_algorithmic": round(solve_bytes64 / t64 / 1e9, 1),
-                 "solve_frac": round(solve_bytes64 / t64 / 1e9 / HBM_PEAK_GBS, 4)},
+                 "solve_frac": round(solve_bytes64 / t64 / 1e9 / HBM_PEAK_GBS, 4),
+                 "solve_GBs_canonical": round(canon64 / t64 / 1e9, 1)},
         "fp64_tuned": tuned,
         "bicgstab_long": bicg_long,
+        "ell_pair_planes": pairs,
+        "face_pairs_off": pairs_off,
         "speedup_mixed_vs_fp64": round(t64 / tmx, 3),
         "roofline": {"bound": "hbm",
                      "kernel": "k_mx_a (fused FP32 iteration kernel: ELL SpMV "
@@ -284,26 +312,43 @@ def main():
     ctx.close()
 
 
-def solve_bytes(neq, s, ib, st64, stmx):
+def solve_bytes(neq, s, ib, st64, stmx, sv=None):
     """Algorithmic HBM bytes of one whole PCG solve, FP64 and mixed, on an
-    ELL matrix of s slots with ib-byte column offsets.  Per row: FP64
-    iteration = SpMV (s (8 + ib) + 24) + residual update 32 + direction 48;
-    mixed iteration = fused FP32 kernel (s (4 + ib) + 24) + 12 (alpha,
-    r -= alpha q, r.r); a fold = 8 + 12 (update) + FP64 residual + 16."""
-    m64 = s * (8 + ib) + 24
-    m32 = s * (4 + ib) + 24
-    b64 = neq * (st64["iters"] * (m64 + 80.0) + 16 + 2 * m64 + 32)
+    ELL matrix of s slots with ib-byte column offsets, of which the FP64
+    SpMV reads sv value planes (s/2 with face-pair reads of a symmetric
+    matrix).  Per row: FP64 iteration = SpMV (8 sv + s ib + 24) + residual
+    update 32 + direction 48; mixed iteration = fused FP32 kernel
+    (4 s + s ib + 24) + 12 (alpha, r -= alpha q, r.r); a fold = 8 + 12
+    (update) + FP64 residual (full rows) + 16."""
+    sv = s if sv is None else sv
+    m64 = 8 * sv + s * ib + 24
+    m32 = 4 * s + s * ib + 24
+    mfold = s * (8 + ib) + 24
+    b64 = neq * (st64["iters"] * (m64 + 80.0) + 16 + 2 * mfold + 32)
     bmx = neq * (stmx["iters"] * (m32 + 12.0)
-                 + (stmx["outer"] + 1) * (20.0 + m64 + 16) + 16 + m64)
+                 + (stmx["outer"] + 1) * (20.0 + mfold + 16) + 16 + mfold)
+    return b64, bmx
+
+
+def canonical_bytes(neq, st64, stmx):
+    """Bytes of one whole PCG solve priced by the reference's own dataflow
+    (src/Cg.c:103-152 as three phases, int32 ELL indices, 13 vector words a
+    row): 184 B/row per FP64 iteration, 104 per FP32 iteration,
+    124 per FP64 refinement step, set-up and final pass one SpMV (80) and
+    8 + 6 vector words each.  The kernels here move fewer bytes than that
+    (fused phases, 16-bit offsets), so the rate in this accounting is an
+    equivalent rate, above the bytes actually streamed."""
+    ends = 2 * 80 + 14 * 8
+    b64 = neq * (st64["iters"] * 184.0 + ends)
+    bmx = neq * (stmx["iters"] * 104.0 + (stmx["outer"] + 1) * 124.0 + ends)
     return b64, bmx
 
 
 def bench_out_of_mall(ctx, S, systems, lib, n=160, reps=2):
     """The same Poisson workload at n^3 rows beyond the 256 MB last-level
-    cache (MALL): at 160^3 (4,096,000 rows) the FP64 loop's working set is
-    344 MB and the FP32 loop's 246 MB, at the cache's edge; at 200^3
-    (8,000,000 rows) they are 672 and 480 MB, so every iteration streams
-    from HBM.  Whole solves FP64 and mixed, solve-level algorithmic GB/s and
+    cache (MALL): at 160^3 (4,096,000 rows) an SpMV launch streams 246 MB
+    (FP32, and FP64 with face pairs), at the cache's edge; at 200^3
+    (8,000,000 rows) 480 MB, so every iteration streams from HBM.  Whole solves FP64 and mixed, solve-level algorithmic GB/s and
     their fraction of the 8 TB/s peak, and the two SpMV kernels timed alone."""
     t0 = time.time()
     B = systems.poisson3d(n, seed=1, kmin=1e-2, reorder=True, storage=S.ELLPACK)
@@ -315,9 +360,7 @@ def bench_out_of_mall(ctx, S, systems, lib, n=160, reps=2):
     db, dx = ctx.array(B.b), ctx.empty(neq)
     out = {"workload": f"{n}^3 variable-coefficient Poisson (the 128^3 "
                        "workload's construction), RCM, ELL, Jacobi-PCG to tol 1e-8",
-           "neq": neq,
-           "working_set_MB": {"fp64": round(neq * (s * (8 + ib) + 24) / 1e6),
-                              "fp32": round(neq * (s * (4 + ib) + 24) / 1e6)}}
+           "neq": neq}
     sts, ts, xs = {}, {}, {}
     for name, prec in (("fp64", S.PREC_FP64), ("mixed", S.PREC_MIXED)):
         ctx.solve_device(A, S.PCG, TOL, 100000, db, dx, precision=prec)
@@ -329,14 +372,19 @@ def bench_out_of_mall(ctx, S, systems, lib, n=160, reps=2):
         lib.sfr_synchronize(ctx.h)
         ts[name] = (time.perf_counter() - t1) / reps
         xs[name] = dx.get()
-    b64, bmx = solve_bytes(neq, s, ib, sts["fp64"], sts["mixed"])
+    sv = s // 2 if A.info()["pair_planes"] else s
+    b64, bmx = solve_bytes(neq, s, ib, sts["fp64"], sts["mixed"], sv)
+    # bytes one SpMV launch streams (FP64 with face pairs when it read them)
+    out["spmv_stream_MB"] = {"fp64": round(neq * (8 * sv + s * ib + 24) / 1e6),
+                             "fp32": round(neq * (4 * s + s * ib + 24) / 1e6)}
+    out["ell_pair_planes"] = A.info()["pair_planes"]
     ms = C.c_double()
     lib.sfr_kernel_time(ctx.h, A.h, 1, 20, C.byref(ms))
     t32 = ms.value * 1e-3
     lib.sfr_kernel_time(ctx.h, A.h, 0, 20, C.byref(ms))
     t64k = ms.value * 1e-3
-    k32 = neq * (s * (4 + ib) + 24) / t32 / 1e9
-    k64 = neq * (s * (8 + ib) + 24) / t64k / 1e9
+    k32 = neq * (4 * s + s * ib + 24) / t32 / 1e9
+    k64 = neq * (8 * sv + s * ib + 24) / t64k / 1e9
     for name, nb in (("fp64", b64), ("mixed", bmx)):
         st = sts[name]
         gbs = nb / ts[name] / 1e9

@@ -141,6 +141,17 @@ int  sfr_matrix_info(const sfr_matrix *A, int32_t *neq, int *format,
  * ELL's columns are kept as 16-bit offsets from the row (every |j - i| <
  * 32768, e.g. banded after RCM), else 4 */
 int  sfr_matrix_index_bytes(const sfr_matrix *A);
+/* Face pairs the FP64 PCG SpMV reads, as of the last FP64 PCG solve on A:
+ * 1 (hex rows, slots (0,1) (2,4) (3,5)), 2 (quad rows, (0,2) (1,3)) or 0.
+ * A symmetric ELLPACK system in SFR3D's face order (src/EllPack.c:26-79
+ * with the face tables of src/Adjcency.c) stores each off-diagonal twice,
+ * in the two rows it couples; when the values mirror bit for bit (checked
+ * on the device once after they change), the FP64 PCG SpMV reads the first
+ * slot of each pair from the row and the second from the neighbour, half
+ * the value stream, with the same products and sums as the full read.  No
+ * counterpart in the reference (a performance option; context option
+ * "ell_pairs" = 0 turns it off). */
+int  sfr_matrix_pair_planes(const sfr_matrix *A);
 /* rows per thread of the solve kernels and of the FP64 PCG loop (the
  * latter differs from 2^21 rows up and after sfr_matrix_tune with
  * rpt = 0) */
@@ -479,7 +490,10 @@ int sfr_kernel_time(sfr_context *ctx, sfr_matrix *A, int which, int reps,
  * PBICGSTAB; the other solver ids stay FP64); "ru_delta" / "ru_delta_bicg"
  * = MIXED PCG / BiCGSTAB: drop of the FP32 residual norm (relative to its
  * largest value since the last fold) that triggers an FP64 reliable update
- * (both default to 0.1).  A mixed BiCGSTAB solve whose FP64 residual stops
+ * (both default to 0.1); "ell_pairs" = 1 (default) lets the FP64 PCG SpMV
+ * read mirrored ELL values through face pairs (sfr_matrix_pair_planes), 0 reads
+ * every slot (the results are the same bits either way).  A mixed BiCGSTAB
+ * solve whose FP64 residual stops
  * improving (or breaks down) is finished in FP64 from its last x, within
  * the same call; stats.iters then counts both parts. */
 int sfr_set_option(sfr_context *ctx, const char *name, double value);

@@ -123,6 +123,7 @@ _sig("sfr_matrix_tune", C.c_int, vp, vp, C.c_int, C.c_int)
 _sig("sfr_matrix_info", C.c_int, vp, C.POINTER(C.c_int32),
      C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int64))
 _sig("sfr_matrix_index_bytes", C.c_int, vp)
+_sig("sfr_matrix_pair_planes", C.c_int, vp)
 _sig("sfr_matrix_rows_per_thread", C.c_int, vp, C.POINTER(C.c_int), C.POINTER(C.c_int))
 _sig("sfr_solve", C.c_int, vp, C.POINTER(SfrSystem), C.c_int, C.c_double,
      C.c_uint, vp, vp, C.c_int, C.POINTER(SfrStats))

@@ -105,6 +105,20 @@ __device__ __forceinline__ void reduce_partials(const double *__restrict__ part,
 // SFR_D16_ESC: the gather loop reads the row's own x there (a select, no
 // branch), and a row that has one adds a_k (x[col] - x[i]) afterwards, with
 // the column from a short exception list (k = f*n + i, col).
+// a*b + c rounded once, written out rather than left to contraction: where
+// the compiler fuses a product into an add depends on the basic blocks the
+// two land in, so two kernels with the same source expression could round
+// differently; the row products and the direction updates are fused here
+// by the source, the same in every kernel that forms them
+__device__ __forceinline__ float fmad(float a, float b, float c)
+{
+  return __builtin_fmaf(a, b, c);
+}
+__device__ __forceinline__ double fmad(double a, double b, double c)
+{
+  return __builtin_fma(a, b, c);
+}
+
 __device__ __forceinline__ int64_t d16_col(const int16_t *__restrict__ dcol,
     const int64_t *xk, const int32_t *xc, int64_t nx, int64_t k, int64_t i)
 {
@@ -164,9 +178,9 @@ struct EllOp {
     bool esc = false;
     if constexpr (S > 0) {
 #pragma unroll
-      for (int f = 0; f < S; f++) acc += val[f * n + i] * x[cg(f, i, esc)];
+      for (int f = 0; f < S; f++) acc = fmad(val[f * n + i], x[cg(f, i, esc)], acc);
     } else {
-      for (int f = 0; f < s; f++) acc += val[f * n + i] * x[cg(f, i, esc)];
+      for (int f = 0; f < s; f++) acc = fmad(val[f * n + i], x[cg(f, i, esc)], acc);
     }
     if constexpr (D16)
       if (esc) acc += esc_fix(i, [&](int64_t j) { return x[j]; });
@@ -180,22 +194,79 @@ struct EllOp {
       const T *__restrict__ r, const T *__restrict__ p, T beta) const
   {
     if (!valid) return T(0);
-    T acc = r[i] + beta * p[i];
+    T acc = fmad(beta, p[i], r[i]);
     bool esc = false;
     if constexpr (S > 0) {
 #pragma unroll
       for (int f = 0; f < S; f++) {
         const int64_t j = cg(f, i, esc);
-        acc += val[f * n + i] * (r[j] + beta * p[j]);
+        acc = fmad(val[f * n + i], fmad(beta, p[j], r[j]), acc);
       }
     } else {
       for (int f = 0; f < s; f++) {
         const int64_t j = cg(f, i, esc);
-        acc += val[f * n + i] * (r[j] + beta * p[j]);
+        acc = fmad(val[f * n + i], fmad(beta, p[j], r[j]), acc);
       }
     }
     if constexpr (D16)
-      if (esc) acc += esc_fix(i, [&](int64_t j) { return r[j] + beta * p[j]; });
+      if (esc) acc += esc_fix(i, [&](int64_t j) { return fmad(beta, p[j], r[j]); });
+    return acc;
+  }
+};
+
+// Symmetric slot-major ELL read through face pairs.  SFR3D's face order
+// pairs the slots of a hex row as (0,1) (2,4) (3,5) and of a quad row as
+// (0,2) (1,3): the neighbour j across slot f of row i faces back to i
+// across the other slot of the pair.  When the values mirror too (a_ij in
+// row i's slot f equals, bit for bit, a_ji in j's paired slot; a slot
+// facing no neighbour, offset 0, holds +0 in the second slot of a pair),
+// a row reads the first slot of each pair from its own plane and the
+// second from the neighbour's first-slot plane, at the offset it reads for
+// the gather anyway: half the value stream, the other half from cache.
+// The values and their order are EllOp's, so the products are bit for bit
+// EllOp's (16-bit offsets without escapes only).
+template <int S>
+__device__ __forceinline__ constexpr int pair_src(int f)
+{
+  if constexpr (S == 6) return f == 1 ? 0 : (f == 4 ? 2 : (f == 5 ? 3 : f));
+  else return f >= 2 ? f - 2 : f;
+}
+
+template <typename T, int S, bool UNIT>
+struct PairOp {
+  static_assert(S == 6 || S == 4, "face pairs of hex or quad rows");
+  static constexpr int L = 1;
+  const T *val;
+  const T *ad;
+  int64_t n;
+  const int16_t *dcol;
+  __device__ __forceinline__ T value(int f, int64_t i, int d) const
+  {
+    if (pair_src<S>(f) == f) return val[f * n + i];
+    return d == 0 ? T(0) : val[pair_src<S>(f) * n + i + d];
+  }
+  __device__ __forceinline__ T row(int64_t i, bool valid, const T *__restrict__ x) const
+  {
+    if (!valid) return T(0);
+    T acc = UNIT ? x[i] : ad[i] * x[i];
+#pragma unroll
+    for (int f = 0; f < S; f++) {
+      const int d = dcol[f * n + i];
+      acc = fmad(value(f, i, d), x[i + d], acc);
+    }
+    return acc;
+  }
+  __device__ __forceinline__ T row_comb(int64_t i, bool valid,
+      const T *__restrict__ r, const T *__restrict__ p, T beta) const
+  {
+    if (!valid) return T(0);
+    T acc = fmad(beta, p[i], r[i]);
+#pragma unroll
+    for (int f = 0; f < S; f++) {
+      const int d = dcol[f * n + i];
+      const int64_t j = i + d;
+      acc = fmad(value(f, i, d), fmad(beta, p[j], r[j]), acc);
+    }
     return acc;
   }
 };
@@ -220,14 +291,14 @@ __device__ __forceinline__ void ell_row_multi(const EllOp<T, S, UNIT, D16> &A,
       const T v = A.val[f * A.n + i];
       const int64_t j = A.cg(f, i, esc);
 #pragma unroll
-      for (int c = 0; c < NB; c++) acc[c] += v * x[j * NB + c];
+      for (int c = 0; c < NB; c++) acc[c] = fmad(v, x[j * NB + c], acc[c]);
     }
   } else {
     for (int f = 0; f < A.s; f++) {
       const T v = A.val[f * A.n + i];
       const int64_t j = A.cg(f, i, esc);
 #pragma unroll
-      for (int c = 0; c < NB; c++) acc[c] += v * x[j * NB + c];
+      for (int c = 0; c < NB; c++) acc[c] = fmad(v, x[j * NB + c], acc[c]);
     }
   }
   if constexpr (D16)
@@ -255,11 +326,11 @@ struct CsrOp {
     if (valid) {
       const int sub = threadIdx.x & (L - 1);
       const int e = ptr[i + 1];
-      for (int k = ptr[i] + sub; k < e; k += L) acc += val[k] * x[col[k]];
+      for (int k = ptr[i] + sub; k < e; k += L) acc = fmad(val[k], x[col[k]], acc);
     }
 #pragma unroll
     for (int off = L / 2; off > 0; off >>= 1) acc += __shfl_xor(acc, off, L);
-    if (valid) acc += UNIT ? x[i] : ad[i] * x[i];
+    if (valid) acc = UNIT ? acc + x[i] : fmad(ad[i], x[i], acc);
     return acc;
   }
   // (A (r + beta p))_i with a unit diagonal (see EllOp::row_comb)
@@ -272,12 +343,12 @@ struct CsrOp {
       const int e = ptr[i + 1];
       for (int k = ptr[i] + sub; k < e; k += L) {
         const int64_t j = col[k];
-        acc += val[k] * (r[j] + beta * p[j]);
+        acc = fmad(val[k], fmad(beta, p[j], r[j]), acc);
       }
     }
 #pragma unroll
     for (int off = L / 2; off > 0; off >>= 1) acc += __shfl_xor(acc, off, L);
-    if (valid) acc += r[i] + beta * p[i];
+    if (valid) acc += fmad(beta, p[i], r[i]);
     return acc;
   }
 };
@@ -296,7 +367,7 @@ __device__ __forceinline__ void csr_row_multi(const CsrOp<T, L, UNIT> &A,
       const T v = A.val[k];
       const int64_t j = A.col[k];
 #pragma unroll
-      for (int c = 0; c < NB; c++) acc[c] += v * x[j * NB + c];
+      for (int c = 0; c < NB; c++) acc[c] = fmad(v, x[j * NB + c], acc[c]);
     }
   }
 #pragma unroll
@@ -305,7 +376,7 @@ __device__ __forceinline__ void csr_row_multi(const CsrOp<T, L, UNIT> &A,
     for (int off = L / 2; off > 0; off >>= 1) acc[c] += __shfl_xor(acc[c], off, L);
   if (valid) {
 #pragma unroll
-    for (int c = 0; c < NB; c++) acc[c] += ad[i * NB + c] * x[i * NB + c];
+    for (int c = 0; c < NB; c++) acc[c] = fmad(ad[i * NB + c], x[i * NB + c], acc[c]);
   }
 }
 
@@ -346,7 +417,7 @@ __device__ __forceinline__ void row_multi(const PerSys<EllOp<T, S, UNIT, D16>> &
     const int64_t k = ((int64_t)f * A.n + i) * NB;
     const int64_t j = A.cg(f, i, esc);
 #pragma unroll
-    for (int c = 0; c < NB; c++) acc[c] += A.val[k + c] * x[j * NB + c];
+    for (int c = 0; c < NB; c++) acc[c] = fmad(A.val[k + c], x[j * NB + c], acc[c]);
   };
   if constexpr (S > 0) {
 #pragma unroll
@@ -380,7 +451,7 @@ __device__ __forceinline__ void row_multi(const PerSys<CsrOp<T, L, UNIT>> &A,
     for (int k = A.ptr[i] + sub; k < e; k += L) {
       const int64_t j = A.col[k];
 #pragma unroll
-      for (int c = 0; c < NB; c++) acc[c] += A.val[(int64_t)k * NB + c] * x[j * NB + c];
+      for (int c = 0; c < NB; c++) acc[c] = fmad(A.val[(int64_t)k * NB + c], x[j * NB + c], acc[c]);
     }
   }
 #pragma unroll
@@ -389,7 +460,7 @@ __device__ __forceinline__ void row_multi(const PerSys<CsrOp<T, L, UNIT>> &A,
     for (int off = L / 2; off > 0; off >>= 1) acc[c] += __shfl_xor(acc[c], off, L);
   if (valid) {
 #pragma unroll
-    for (int c = 0; c < NB; c++) acc[c] += ad[i * NB + c] * x[i * NB + c];
+    for (int c = 0; c < NB; c++) acc[c] = fmad(ad[i * NB + c], x[i * NB + c], acc[c]);
   }
 }
 
@@ -461,6 +532,27 @@ __host__ inline void with_op32(const sfr_mat_view *A, F &&f)
   case 4: f(CsrOp<float, 4, true>{A->val32, A->col, A->rowptr, nullptr, A->n, 0}); return;
   default: f(CsrOp<float, 8, true>{A->val32, A->col, A->rowptr, nullptr, A->n, 0}); return;
   }
+}
+
+// The FP64 PCG SpMV's operator: the face-pair read of the ELL when the view
+// allows it (the values checked to mirror), else with_op64.  (The FP32
+// iteration kernel keeps the full read: it saves 12 of its 60 B/row but
+// adds a third gather stream, and measured no faster at 128^3-160^3 and
+// 3 % slower at 200^3.)
+template <typename F>
+__host__ inline void with_op64_pair(const sfr_mat_view *A, F &&f)
+{
+  if (A->fmt == SFR_FMT_ELL && A->col16 && A->nexc == 0) {
+    if (A->pair == SFR_PAIR_HEX && A->slots == 6) {
+      f(PairOp<double, 6, false>{A->val, A->ad, A->n, A->col16});
+      return;
+    }
+    if (A->pair == SFR_PAIR_QUAD && A->slots == 4) {
+      f(PairOp<double, 4, false>{A->val, A->ad, A->n, A->col16});
+      return;
+    }
+  }
+  with_op64(A, f);
 }
 
 // Workgroups for n rows at rpt rows per thread and `lanes` lanes per row,

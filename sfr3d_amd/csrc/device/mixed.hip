@@ -242,14 +242,15 @@ __global__ __launch_bounds__(SFR_BLOCK) void k_mx_a(Op A,
       const int64_t i = c * RPC + r_ * (SFR_BLOCK / L) + threadIdx.x / L;
       const bool valid = i < A.n;
       if (fold) {
-        if (valid && lead) y[i] += fa * po[i];
+        if (valid && lead) y[i] = fmad(fa, po[i], y[i]);
         continue;
       }
       const float qi = A.row_comb(i, valid, r, po, fb);
       if (valid && lead) {
         const float pi = po[i];
-        const float pni = r[i] + fb * pi;
-        y[i] += fa * pi;
+        // the same rounding as each neighbour's r_j + beta p_j in row_comb
+        const float pni = fmad(fb, pi, r[i]);
+        y[i] = fmad(fa, pi, y[i]);
         pn[i] = pni;
         q[i] = qi;
         acc[0] += (double)qi * (double)pni;

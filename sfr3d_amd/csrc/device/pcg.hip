@@ -80,7 +80,9 @@ __global__ __launch_bounds__(SFR_BLOCK) void k_scale32_ell(int64_t n,
   const int64_t k = (int64_t)blockIdx.x * SFR_BLOCK + threadIdx.x;
   if (k < nnz) {
     const int64_t i = k % n;           // slot-major: row of entry k
-    val32[k] = (float)(val[k] * dscale[i] * dscale[col[k]]);
+    // a_ij (s_i s_j): the same bits for a_ji (s_j s_i) when a_ij == a_ji, so
+    // a symmetric A gives a symmetric copy
+    val32[k] = (float)(val[k] * (dscale[i] * dscale[col[k]]));
   }
 }
 
@@ -92,7 +94,7 @@ __global__ __launch_bounds__(SFR_BLOCK) void k_scale32_csr(int64_t n,
   const int64_t i = (int64_t)blockIdx.x * SFR_BLOCK + threadIdx.x;
   if (i < n)
     for (int k = ptr[i]; k < ptr[i + 1]; k++)
-      val32[k] = (float)(val[k] * dscale[i] * dscale[col[k]]);
+      val32[k] = (float)(val[k] * (dscale[i] * dscale[col[k]]));
 }
 
 template <class Op, int RPT>
@@ -413,6 +415,62 @@ extern "C" int sfr_k_fill32(hipStream_t s, int64_t n, float v, float *x)
   return hipGetLastError();
 }
 
+__device__ __forceinline__ uint64_t f64_bits(double v)
+{
+  uint64_t u;
+  __builtin_memcpy(&u, &v, sizeof u);
+  return u;
+}
+
+// Rows whose values do not mirror across the face pairs (see PairOp):
+// per row, the second slot g of each pair with offset d either faces no
+// neighbour (d == 0) and holds +0, or the neighbour j = i + d faces back
+// across g's partner f with the same value bits.  One count per workgroup.
+template <int S>
+__global__ __launch_bounds__(SFR_BLOCK) void k_pair_check(int64_t n,
+    const double *__restrict__ val, const int16_t *__restrict__ dcol,
+    double *pout)
+{
+  double acc[1] = {0.0};
+  for (int64_t i = (int64_t)blockIdx.x * SFR_BLOCK + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * SFR_BLOCK) {
+    bool bad = false;
+#pragma unroll
+    for (int g = 0; g < S; g++) {
+      const int f = pair_src<S>(g);
+      if (f == g) continue;
+      const int d = dcol[g * n + i];
+      const uint64_t vg = f64_bits(val[g * n + i]);
+      if (d == 0) {
+        bad |= vg != 0u;
+        continue;
+      }
+      const int64_t j = i + d;
+      if (j < 0 || j >= n) { bad = true; continue; }
+      bad |= dcol[f * n + j] != -d;
+      bad |= vg != f64_bits(val[f * n + j]);
+    }
+    acc[0] += bad ? 1.0 : 0.0;
+  }
+  store_partials<1>(acc, pout);
+}
+
+extern "C" int sfr_k_pair_check(hipStream_t s, const sfr_mat_view *A, int kind,
+                                double *part, int *nparts)
+{
+  const int64_t n = A->n;
+  const int64_t want = (n + SFR_BLOCK - 1) / SFR_BLOCK;
+  const int grid = want < 1 ? 1 : (want > 1024 ? 1024 : (int)want);
+  *nparts = grid;
+  if (kind == SFR_PAIR_HEX)
+    hipLaunchKernelGGL(k_pair_check<6>, dim3(grid), dim3(SFR_BLOCK), 0, s, n,
+                       A->val, A->col16, part);
+  else
+    hipLaunchKernelGGL(k_pair_check<4>, dim3(grid), dim3(SFR_BLOCK), 0, s, n,
+                       A->val, A->col16, part);
+  return hipGetLastError();
+}
+
 extern "C" int sfr_k_scale32(hipStream_t s, const sfr_mat_view *A,
                              double *dscale, float *val32)
 {
@@ -488,7 +546,7 @@ extern "C" int sfr_k_pcg_iter(hipStream_t s, const sfr_mat_view *A,
   const sfr_geom gv = {rv, grid_for(A->n, rv, 1)};
   const sfr_geom gm = {g0.rpt, grid_for(A->n, g0.rpt, op_lanes(A))};
   double *ra = REGION(part, 0), *rb = REGION(part, 1);
-  with_op64(A, [&](auto op) {
+  with_op64_pair(A, [&](auto op) {
     using Op = decltype(op);
     SFR_LAUNCH_RPT(k_pcg_spmv, Op, gm, s, op, p, q, ra, sc);
   });
@@ -520,7 +578,7 @@ extern "C" int sfr_k_pcg_spmv_only(hipStream_t s, const sfr_mat_view *A,
                                    sfr_scalars *sc, double *part)
 {
   const sfr_geom gm = {g0.rpt, grid_for(A->n, g0.rpt, op_lanes(A))};
-  with_op64(A, [&](auto op) {
+  with_op64_pair(A, [&](auto op) {
     using Op = decltype(op);
     SFR_LAUNCH_RPT(k_pcg_spmv, Op, gm, s, op, p, q, REGION(part, 0), sc);
   });

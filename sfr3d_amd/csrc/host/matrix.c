@@ -76,6 +76,7 @@ int sfr_init(sfr_context **out, int device)
   ctx->chunk = 16;
   ctx->ru_delta = 0.1;
   ctx->ru_delta_b = 0.1;
+  ctx->ell_pairs = 1;
   if (hipSetDevice(device) != hipSuccess) { free(ctx); return SFR_NO_DEVICE; }
   int rc = SFR_OK;
 #define INIT_TRY(call) do { if ((call) != hipSuccess) { rc = SFR_HIP_ERROR; goto fail; } } while (0)
@@ -256,7 +257,25 @@ sfr_mat_view sfr_view(const sfr_matrix *A)
   v.col16 = A->col16;
   v.exc_k = A->exc_k; v.exc_col = A->exc_col; v.nexc = A->nexc;
   v.valnb = NULL;
+  v.pair = A->pair_use ? A->pair_kind : SFR_PAIR_NONE;
   return v;
+}
+
+/* The face pairing of a slot-major ELL pattern (hcol[f*n + i]): the kind
+ * whose pairs (f, g) satisfy, for every row i with a neighbour j across f,
+ * that j's slot g faces back to i.  SFR3D's hex and quad face orders
+ * (Adjcency.c's face tables) give the two kinds recognised here. */
+static int ell_pair_kind(const int32_t *hcol, int32_t n, int s)
+{
+  static const int hex[6] = {1, 0, 4, 5, 2, 3}, quad[4] = {2, 3, 0, 1};
+  const int *opp = s == 6 ? hex : (s == 4 ? quad : NULL);
+  if (!opp) return SFR_PAIR_NONE;
+  for (int f = 0; f < s; f++)
+    for (int32_t i = 0; i < n; i++) {
+      const int32_t j = hcol[(int64_t)f * n + i];
+      if (j != i && hcol[(int64_t)opp[f] * n + j] != i) return SFR_PAIR_NONE;
+    }
+  return s == 6 ? SFR_PAIR_HEX : SFR_PAIR_QUAD;
 }
 
 /* ---- host conversion ------------------------------------------------------ */
@@ -477,6 +496,7 @@ static int upload_format(sfr_context *ctx, sfr_matrix *A, int32_t n,
         goto done;
       }
       A->nexc = (int)nexc;
+      if (nexc == 0) A->pair_kind = ell_pair_kind(hcol, n, A->slots);
     }
   }
   MTRY(hipMemcpyAsync(A->vmap, hmap, sizeof(int32_t) * (size_t)A->nnz, hipMemcpyHostToDevice, ctx->stream));
@@ -662,7 +682,37 @@ int sfr_matrix_refresh(sfr_context *ctx, sfr_matrix *A)
   TRY(sfr_k_diag(ctx->stream, A->n, A->diag_in_al ? A->dmap : NULL, A->raw, A->raw_ad, A->ad, A->dinv));
   A->have32 = 0;
   A->have32r = 0;
+  A->pair_ok = -1;
+  A->pair_use = 0;
   return SFR_OK;
+}
+
+int sfr_matrix_pair_check(sfr_context *ctx, sfr_matrix *A)
+{
+  if (A->pair_kind == SFR_PAIR_NONE || !ctx->ell_pairs) {
+    A->pair_use = 0;
+    return SFR_OK;
+  }
+  if (A->pair_ok < 0) {
+    sfr_mat_view v = sfr_view(A);
+    int np = 0;
+    TRY(sfr_k_pair_check(ctx->stream, &v, A->pair_kind, ctx->part, &np));
+    double h[1024];
+    TRY(hipMemcpyAsync(h, ctx->part, sizeof(double) * (size_t)np,
+                       hipMemcpyDeviceToHost, ctx->stream));
+    TRY(hipStreamSynchronize(ctx->stream));
+    double bad = 0.0;
+    for (int k = 0; k < np; k++) bad += h[k];
+    A->pair_ok = bad == 0.0;
+  }
+  A->pair_use = A->pair_ok == 1;
+  return SFR_OK;
+}
+
+int sfr_matrix_pair_planes(const sfr_matrix *A)
+{
+  if (!A) return SFR_BAD_ARG;
+  return A->pair_use ? A->pair_kind : SFR_PAIR_NONE;
 }
 
 /* FP32 copy of the device-format values, unscaled (mixed BiCGSTAB) */

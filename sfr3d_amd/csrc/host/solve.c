@@ -100,7 +100,7 @@ static int get_graph(sfr_context *ctx, int kind, const void *k0,
         e->key[1] == k1 && e->key[2] == k2 &&
         e->key[3] == (const void *)ctx->w64[0] && e->key[4] == (const void *)(intptr_t)g.rpt &&
         e->key[5] == (const void *)ctx->gm_buf && e->key[6] == (const void *)(intptr_t)g.cap &&
-        e->key[7] == (const void *)(intptr_t)v->lanes) {
+        e->key[7] == (const void *)(intptr_t)(v->lanes | v->pair << 8)) {
       /* least recently used first: a hit moves to the end, so the graph a
        * caller has just been handed is never the one a second lookup for
        * the same solve evicts */
@@ -179,7 +179,7 @@ static int get_graph(sfr_context *ctx, int kind, const void *k0,
   e->key[4] = (const void *)(intptr_t)g.rpt;
   e->key[5] = (const void *)ctx->gm_buf;
   e->key[6] = (const void *)(intptr_t)g.cap;
-  e->key[7] = (const void *)(intptr_t)v->lanes;
+  e->key[7] = (const void *)(intptr_t)(v->lanes | v->pair << 8);
   *exec = ex;
   return SFR_OK;
 }
@@ -365,6 +365,7 @@ static int solve_pcg64(sfr_context *ctx, sfr_matrix *A, double tol,
                        sfr_stats *st)
 {
   CHECK(sfr_workspace(ctx, A->n));
+  CHECK(sfr_matrix_pair_check(ctx, A));
   sfr_mat_view v = sfr_view(A);
   sfr_geom g = {sfr_pcg_rpt(A), 0, 0};
   chunks ch;
@@ -1061,6 +1062,7 @@ int sfr_kernel_time(sfr_context *ctx, sfr_matrix *A, int which, int reps,
   TRY(hipSetDevice(ctx->device));
   CHECK(sfr_workspace(ctx, A->n));
   if (which == 1) CHECK(sfr_matrix_prepare_mixed(ctx, A));
+  if (which == 0) CHECK(sfr_matrix_pair_check(ctx, A));
   sfr_mat_view v = sfr_view(A);
   sfr_geom g = {which == 0 ? sfr_pcg_rpt(A) : A->rpt, 0, 0};
   if (which == 1) g = mixed_geom(A);
@@ -1288,6 +1290,11 @@ int sfr_set_option(sfr_context *ctx, const char *name, double value)
   if (!strcmp(name, "ru_delta")) {
     if (!(value > 0 && value < 1)) return sfr_set_error(ctx, SFR_BAD_ARG, "ru_delta %g", value);
     ctx->ru_delta = value;
+    return SFR_OK;
+  }
+  if (!strcmp(name, "ell_pairs")) {
+    if (value != 0 && value != 1) return sfr_set_error(ctx, SFR_BAD_ARG, "ell_pairs %g", value);
+    ctx->ell_pairs = (int)value;
     return SFR_OK;
   }
   if (!strcmp(name, "ru_delta_bicg")) {

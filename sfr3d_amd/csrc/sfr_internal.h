@@ -82,7 +82,14 @@ typedef struct sfr_mat_view {
   int            nexc;
   const double  *valnb;     /* batched kernels: per-system values          */
                             /*   interleaved, valnb[k*nb + c], or NULL     */
+  int            pair;      /* ELL face pairs the FP64 PCG SpMV may read   */
+                            /*   (SFR_PAIR_HEX / SFR_PAIR_QUAD), or 0      */
 } sfr_mat_view;
+
+/* ELL face pairs: SFR3D's hex face order pairs slots (0,1) (2,4) (3,5),
+ * its quad order (0,2) (1,3); a row's second slot of a pair holds the value
+ * of the neighbour's first slot facing back */
+enum { SFR_PAIR_NONE = 0, SFR_PAIR_HEX = 1, SFR_PAIR_QUAD = 2 };
 
 /* launch geometry: rows per thread, workgroups (derived), and a cap on the
  * workgroups below SFR_GRIDCAP (0: SFR_GRIDCAP) */
@@ -105,6 +112,10 @@ struct sfr_matrix {
   int64_t *exc_k;        /*   do not fit 16 bits, listed in exc_k/exc_col  */
   int32_t *exc_col;
   int      nexc;
+  int      pair_kind;    /* face pairing of the ELL pattern (SFR_PAIR_*)   */
+  int      pair_ok;      /* values mirror across the pairs: 1 yes, 0 no,   */
+                         /*   -1 not checked since they last changed       */
+  int      pair_use;     /* the FP64 PCG SpMV reads them (last check)      */
   double  *raw, *raw_ad, *val, *ad, *dinv, *dscale;
   float   *val32;
   float   *val32r;       /* FP32 copy of val (unscaled), mixed BiCGSTAB     */
@@ -161,6 +172,7 @@ struct sfr_context {
   int64_t      nb_val_n, nb_raw_n;
   double       ru_delta;        /* MIXED PCG: reliable-update residual drop */
   double       ru_delta_b;      /* MIXED BiCGSTAB: the same                */
+  int          ell_pairs;       /* FP64 PCG SpMV may read ELL face pairs   */
   void        *scratch;         /* device address of the static scratch    */
   int          precision;       /* host-pointer and batch entry points     */
   float       *mb32;            /* mixed BiCGSTAB: 7 vectors, diagonal and */
@@ -177,6 +189,9 @@ void sfr_drop_graphs_keyed(sfr_context *ctx, int slot, const void *key);
 /* solve statistics from a scalar block (ms = device time of the solve) */
 void sfr_fill_stats(const sfr_scalars *sc, sfr_stats *st, float ms);
 int  sfr_matrix_prepare_mixed(sfr_context *ctx, sfr_matrix *A);
+/* whether the FP64 PCG SpMV of A may read its ELL face pairs: checks the
+ * values once after they change (a device pass and a sync), sets pair_use */
+int  sfr_matrix_pair_check(sfr_context *ctx, sfr_matrix *A);
 /* raw values changed on the device: device-format values, diagonal */
 int  sfr_matrix_refresh(sfr_context *ctx, sfr_matrix *A);
 sfr_mat_view sfr_view(const sfr_matrix *A);
@@ -193,6 +208,10 @@ int sfr_k_gather(hipStream_t s, int64_t nnz, const int32_t *vmap,
 int sfr_k_diag(hipStream_t s, int64_t n, const int32_t *dmap,
                const double *raw, const double *raw_ad, double *ad,
                double *dinv);
+/* mismatches of A's values across its face pairs (kind = SFR_PAIR_*):
+ * per-workgroup counts in part[0..*nparts) */
+int sfr_k_pair_check(hipStream_t s, const sfr_mat_view *A, int kind,
+                     double *part, int *nparts);
 int sfr_k_scale32(hipStream_t s, const sfr_mat_view *A, double *dscale,
                   float *val32);
 int sfr_k_fill(hipStream_t s, int64_t n, double v, double *x);

@@ -98,6 +98,7 @@ class DeviceMatrix:
         return {"neq": n.value, "format": f.value, "slots": s.value,
                 "nnz": nnz.value,
                 "index_bytes": int(lib.sfr_matrix_index_bytes(self.h)),
+                "pair_planes": int(lib.sfr_matrix_pair_planes(self.h)),
                 "rows_per_thread": rpt.value, "rows_per_thread_pcg": rpt_pcg.value}
 
     def destroy(self):

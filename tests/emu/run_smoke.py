@@ -1,7 +1,7 @@
 """TEST INFRASTRUCTURE -- runs the emulated library (argv[1]) on the small
 cases of argv[2] (written by tests/test_emulated_kernels.py with the
 oracle's solutions) in a process of its own: FP64 and mixed PCG, BiCGSTAB,
-BiCGSTAB(2), GMRES, MINRES, the species batch,
+BiCGSTAB(2), GMRES, MINRES, ELL face-pair reads, the species batch,
 the shared-pattern batch, both adjoint paths, 16-bit column offsets with
 escaped far columns, a tuned uneven batch and re-tuned rows per thread.
 Prints PASS/FAIL lines and exits non-zero on any failure."""
@@ -51,6 +51,40 @@ rc = lib.sfr_solve_device(ctx, A, 5, 0, 1e-8, maxit, b.ctypes.data, x.ctypes.dat
 check(f"minres pois rc {rc} it {st.iters}/{itm} rel {rel(x, z['pois_mr_xo']):.1e}",
       rc == 0 and abs(int(st.iters) - itm) <= 2 and rel(x, z["pois_mr_xo"]) < 1e-6)
 lib.sfr_matrix_destroy(ctx, A)
+# ELLPACK in SFR3D's face order (hex, quad rows): the FP64 PCG SpMV reads
+# the face pairs once the values are seen to mirror, with the same bits as
+# with the pairs off (the mixed loop keeps the full read: the option leaves
+# it unchanged), and values that stop mirroring turn them off
+lib.sfr_set_option.argtypes = [vp, C.c_char_p, C.c_double]
+lib.sfr_matrix_set_values.argtypes = [vp, vp, vp, vp, vp]
+lib.sfr_matrix_pair_planes.argtypes = [vp]
+for name, kind in (("pE3", 1), ("pE2", 2)):
+    s, maxit = mk(name); A = vp(); assert lib.sfr_matrix_create(ctx, C.byref(s), 0, C.byref(A)) == 0
+    b = np.ascontiguousarray(z[f"{name}_b"]); xo = z[f"{name}_xo"]
+    for prec in (0, 1):
+        res = []
+        for pairs in (1, 0):
+            assert lib.sfr_set_option(ctx, b"ell_pairs", pairs) == 0
+            x = np.zeros(s.neq); st = St()
+            rc = lib.sfr_solve_device(ctx, A, 1, prec, 1e-8, maxit, b.ctypes.data, x.ctypes.data, 1, C.byref(st))
+            res.append((rc, st.iters, x, lib.sfr_matrix_pair_planes(A)))
+        (r1, i1, x1, p1), (r0, i0, x0, p0) = res
+        check(f"{name} pairs prec {prec} rc {r1},{r0} it {i1},{i0} planes {p1},{p0} rel {rel(x1, xo):.1e}",
+              r1 == 0 and r0 == 0 and (prec == 1 or (p1 == kind and p0 == 0)) and i1 == i0
+              and np.array_equal(x1, x0) and rel(x1, xo) < 1e-6)
+    assert lib.sfr_set_option(ctx, b"ell_pairs", 1) == 0
+    al = np.array(z[f"{name}_al"]); ad = np.ascontiguousarray(z[f"{name}_ad"])
+    k = int(np.flatnonzero(al)[len(np.flatnonzero(al)) // 2])
+    al[k] *= 1.0 + 2.0 ** -40                 # one value no longer mirrored
+    planes = []
+    for vals in (al, np.ascontiguousarray(z[f"{name}_al"])):
+        assert lib.sfr_matrix_set_values(ctx, A, vals.ctypes.data, ad.ctypes.data, None) == 0
+        x = np.zeros(s.neq); st = St()
+        rc = lib.sfr_solve_device(ctx, A, 1, 0, 1e-8, maxit, b.ctypes.data, x.ctypes.data, 1, C.byref(st))
+        planes.append((rc, lib.sfr_matrix_pair_planes(A), rel(x, xo)))
+    check(f"{name} pairs after value changes {planes}",
+          planes[0][1] == 0 and planes[1][1] == kind and planes[1][0] == 0 and planes[1][2] < 1e-6)
+    lib.sfr_matrix_destroy(ctx, A)
 # momentum: BiCGSTAB, shared batch, adjoint (transposed operator)
 s, maxit = mk("mom"); A = vp(); assert lib.sfr_matrix_create(ctx, C.byref(s), 0, C.byref(A)) == 0
 bb = np.ascontiguousarray(z["mom_b"]); xo = z["mom_xo"]

@@ -41,6 +41,11 @@ def write_cases(path):
 
     B = systems.poisson3d(6, seed=11)
     put("pois", B.system, B.b, 1, 1000)
+    # ELLPACK in SFR3D's face order, hex and quad rows (face-pair reads)
+    B = systems.poisson3d(6, seed=12, storage=S.ELLPACK)
+    put("pE3", B.system, B.b, 1, 1000)
+    B = systems.poisson2d(9, 7, seed=3, storage=S.ELLPACK, pinned_cell=None)
+    put("pE2", B.system, B.b, 1, 1000)
     M = systems.momentum3d(5)
     put("mom", M.system, M.extra["b_all"], 2, 1000)
     d["mom_xo_uvw"] = np.stack([oracle.pbicgstab(M.system, M.extra["b_all"][c], 1e-8, 1000)[0]
@@ -120,7 +125,7 @@ def test_kernels_under_cpu_emulation(emu_lib, tmp_path):
     print(res.stdout)
     assert res.returncode == 0, res.stdout + res.stderr[-3000:]
     assert "SMOKE OK" in res.stdout
-    assert res.stdout.count("PASS") == 21
+    assert res.stdout.count("PASS") == 27
 
 
 @pytest.fixture(scope="module")
