@@ -71,3 +71,4 @@ inline unsigned long long wall_clock64() { return emu::clock(); }
 
 #define hipLaunchKernelGGL(K, G, B, SHM, S, ...) emu::launch(K, dim3(G), dim3(B), __VA_ARGS__)
 inline int __builtin_amdgcn_readfirstlane(int v) { return v; }
+template <class T> inline T __builtin_nontemporal_load(const T *p) { return *p; }
