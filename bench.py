@@ -129,11 +129,11 @@ def main():
         wall = time.perf_counter() - t_start
         return st, wall / steps, times
 
-    st64, t64, _ = run(S.PREC_FP64, args.steps, args.warmup)
+    st64, t64, times64 = run(S.PREC_FP64, args.steps, args.warmup)
     x64 = dx.get()
     log(f"[bench] fp64  : {1.0 / t64:8.3f} solves/s  iters={st64['iters']} "
         f"relres={st64['relres']:.3e} status={st64['status']}")
-    stmx, tmx, _ = run(S.PREC_MIXED, args.steps, args.warmup)
+    stmx, tmx, timesmx = run(S.PREC_MIXED, args.steps, args.warmup)
     xmx = dx.get()
     log(f"[bench] mixed : {1.0 / tmx:8.3f} solves/s  inner iters={stmx['iters']} "
         f"outer={stmx['outer']} relres={stmx['relres']:.3e} status={stmx['status']}")
@@ -267,6 +267,7 @@ def main():
             "solver": "PCG", "precision": "mixed",
         },
         "mixed": {"solves_per_s": round(1.0 / tmx, 4),
+                  "ms_median": round(float(np.median(timesmx)) * 1e3, 3),
                   "inner_iters": itmx, "outer_steps": outer,
                   "relres_true": stmx["relres"],
                   "ref_criterion": [stmx["norm_r_m"], stmx["conv"]],
@@ -276,6 +277,7 @@ def main():
                   "solve_frac": round(solve_bytesmx / tmx / 1e9 / HBM_PEAK_GBS, 4),
                   "solve_GBs_canonical": round(canonmx / tmx / 1e9, 1)},
         "fp64": {"solves_per_s": round(1.0 / t64, 4), "iters": it64,
+                 "ms_median": round(float(np.median(times64)) * 1e3, 3),
                  "relres_true": st64["relres"],
                  "ref_criterion": [st64["norm_r_m"], st64["conv"]],
                  "relerr_vs_tol1e-13": finite(err64),
