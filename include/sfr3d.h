@@ -182,7 +182,7 @@ void sfr_matrix_cache_clear(sfr_context *ctx);
  * the residual, the solution update and the stopping test are FP64, the
  * test is the FP64 solver's own.  That test bounds the residual, not the
  * error: on an ill-conditioned system the mixed solution meets it with a
- * larger error than the FP64 one (7.2e-7 against 1.0e-7 on a 128^3
+ * larger error than the FP64 one (8.5e-7 against 1.0e-7 on a 128^3
  * variable-coefficient Poisson system at tol 1e-8), since FP32 rounding
  * leaves error in the lowest modes, which the test weighs least; a caller
  * who needs the solution itself to FP64 accuracy keeps SFR_PREC_FP64.
