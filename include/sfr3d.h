@@ -127,7 +127,7 @@ void sfr_matrix_destroy(sfr_context *ctx, sfr_matrix *A);
  * the CSR format, lanes per row (1, 2, 4, 8).  An explicit rpt applies to
  * every loop; without one, a matrix of 2^21 rows or more runs its mixed
  * loop at 2 rows per thread on at most 1,024 workgroups and its FP64 PCG
- * loop at 1 row per thread below 2^22 rows and 2 from there.  rpt = 0
+ * loop at 1 row per thread on at most 1,024 workgroups.  rpt = 0
  * measures:
  * short FP64 PCG runs at each rows-per-thread choice pick the FP64 PCG
  * loop's (the other loops keep theirs).  It runs solves on the context's

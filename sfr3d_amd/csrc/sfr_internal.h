@@ -269,20 +269,18 @@ int sfr_k_simple_update(hipStream_t s, int64_t n, double alpha, double *w,
 /* measured rows per thread of the FP64 PCG loop -> A->rpt_pcg */
 int sfr_tune_pcg(sfr_context *ctx, sfr_matrix *A);
 /* Rows per thread of the FP64 PCG loop: the measured choice when there is
- * one; else, unless the caller fixed the shape, 1 for 2^21 <= n < 2^22
- * rows and 2 from 2^22 up; else the matrix's default.  Measured on the
- * 128^3 / 160^3 / 200^3 Poisson loops (ms per solve at 1, 2, 4, 8 rows per
- * thread): with face pairs 63.9 / 65.9 / 69.5 / 68.7, 187.8 / 188.2 /
- * 193.4 / 195.8 and 466.1 / 453.8 / 452.3 / 469.3; with the full read the
- * tuner also picks 1 at 128^3, and 160^3 runs 215.6 / 208.4 / 213.7 /
- * 215.8.  The shape does not depend on whether the face pairs are read,
- * so a solve that has not checked them (the allocation-free entry point)
- * sums in the same order as one that has. */
+ * one; else, unless the caller fixed the shape, 1 from 2^21 rows up (with
+ * the SpMV capped at 1,024 workgroups there, pcg.hip); else the matrix's
+ * default.  Measured on the 128^3 / 160^3 / 200^3 Poisson loops (ms per
+ * solve at 1, 2, 4, 8 rows per thread): 63.2 / 64.8 / 65.4 / 69.2,
+ * 180.0 / 183.7 / 183.8 / 187.4 and 415.9 / 424.7 / 430.4 / 439.9; the
+ * tuner picks 1 at all three.  The shape does not depend on whether the
+ * face pairs are read, so a solve that has not checked them (the
+ * allocation-free entry point) sums in the same order as one that has. */
 static inline int sfr_pcg_rpt(const sfr_matrix *A)
 {
   if (A->rpt_pcg) return A->rpt_pcg;
-  if (!A->rpt_set && A->n >= ((int64_t)1 << 21))
-    return A->n < ((int64_t)1 << 22) ? 1 : 2;
+  if (!A->rpt_set && A->n >= ((int64_t)1 << 21)) return 1;
   return A->rpt;
 }
 
