@@ -76,10 +76,27 @@ __device__ __forceinline__ void reduce_partials(const double *__restrict__ part,
 {
   __shared__ double bc[NV];
   double t[NV];
+  // Up to 8 partials a thread (2,048 a workgroup) have their loads issued
+  // together before any is added: the partials were written by workgroups
+  // on every XCD, so most loads leave this XCD's L2, and one round trip per
+  // partial in turn was most of a short kernel's fixed cost.  They are
+  // added in the same ascending order as one at a time.
+  constexpr int U = 8;
 #pragma unroll
   for (int k = 0; k < NV; k++) {
+    const double *pk = part + (size_t)k * g;
     double s = 0.0;
-    for (int j = threadIdx.x; j < g; j += SFR_BLOCK) s += part[(size_t)k * g + j];
+    for (int base = 0; base < g; base += U * SFR_BLOCK) {
+      double v[U];
+#pragma unroll
+      for (int u = 0; u < U; u++) {
+        const int j = base + u * SFR_BLOCK + (int)threadIdx.x;
+        v[u] = j < g ? pk[j] : 0.0;
+      }
+#pragma unroll
+      for (int u = 0; u < U; u++)
+        if (base + u * SFR_BLOCK + (int)threadIdx.x < g) s += v[u];
+    }
     t[k] = s;
   }
   block_sum<NV>(t);
@@ -93,18 +110,6 @@ __device__ __forceinline__ void reduce_partials(const double *__restrict__ part,
   __syncthreads();
 }
 
-// ---- sparse row operators ---------------------------------------------------
-// row(i, valid, x) returns (A x)_i for the L lanes that share row i; lanes of
-// an invalid row return 0 but still take part in the cross-lane shuffles.
-
-// Slot-major ELL: val[f*n + i], col[f*n + i]; one lane per row.  S > 0 fixes
-// the slot count at compile time, S == 0 reads it at run time.  UNIT = the
-// diagonal is 1 (Jacobi-scaled copy) and ad is not read.  D16 = the columns
-// are read as 16-bit offsets from the row, dcol[f*n + i] = col - i (half the
-// index traffic).  The few entries whose offset does not fit hold
-// SFR_D16_ESC: the gather loop reads the row's own x there (a select, no
-// branch), and a row that has one adds a_k (x[col] - x[i]) afterwards, with
-// the column from a short exception list (k = f*n + i, col).
 // a*b + c rounded once, written out rather than left to contraction: where
 // the compiler fuses a product into an add depends on the basic blocks the
 // two land in, so two kernels with the same source expression could round
@@ -119,6 +124,18 @@ __device__ __forceinline__ double fmad(double a, double b, double c)
   return __builtin_fma(a, b, c);
 }
 
+// ---- sparse row operators ---------------------------------------------------
+// row(i, valid, x) returns (A x)_i for the L lanes that share row i; lanes of
+// an invalid row return 0 but still take part in the cross-lane shuffles.
+
+// Slot-major ELL: val[f*n + i], col[f*n + i]; one lane per row.  S > 0 fixes
+// the slot count at compile time, S == 0 reads it at run time.  UNIT = the
+// diagonal is 1 (Jacobi-scaled copy) and ad is not read.  D16 = the columns
+// are read as 16-bit offsets from the row, dcol[f*n + i] = col - i (half the
+// index traffic).  The few entries whose offset does not fit hold
+// SFR_D16_ESC: the gather loop reads the row's own x there (a select, no
+// branch), and a row that has one adds a_k (x[col] - x[i]) afterwards, with
+// the column from a short exception list (k = f*n + i, col).
 __device__ __forceinline__ int64_t d16_col(const int16_t *__restrict__ dcol,
     const int64_t *xk, const int32_t *xc, int64_t nx, int64_t k, int64_t i)
 {
