@@ -80,12 +80,19 @@ __device__ __forceinline__ void reduce_partials(const double *__restrict__ part,
   // together before any is added: the partials were written by workgroups
   // on every XCD, so most loads leave this XCD's L2, and one round trip per
   // partial in turn was most of a short kernel's fixed cost.  They are
-  // added in the same ascending order as one at a time.
+  // added in the same ascending order as one at a time.  Up to SFR_BLOCK
+  // partials (one a thread; the per-system sets of the uneven batch) take
+  // a single load.
   constexpr int U = 8;
 #pragma unroll
   for (int k = 0; k < NV; k++) {
     const double *pk = part + (size_t)k * g;
     double s = 0.0;
+    if (g <= SFR_BLOCK) {
+      if ((int)threadIdx.x < g) s += pk[threadIdx.x];
+      t[k] = s;
+      continue;
+    }
     for (int base = 0; base < g; base += U * SFR_BLOCK) {
       double v[U];
 #pragma unroll
@@ -107,6 +114,33 @@ __device__ __forceinline__ void reduce_partials(const double *__restrict__ part,
   __syncthreads();
 #pragma unroll
   for (int k = 0; k < NV; k++) out[k] = bc[k];
+  __syncthreads();
+}
+
+// The same sum with one load at a time: for kernels that reduce many small
+// partial sets in turn (the uneven batch reduces its system's set for every
+// chunk it takes), where the unrolled loads of reduce_partials measured
+// slower (260 -> 246 batch solves/s).  The same order and bits.
+template <int NV>
+__device__ __forceinline__ void reduce_partials_seq(const double *__restrict__ part,
+                                                    int g, double (&out)[NV])
+{
+  __shared__ double bs[NV];
+  double t[NV];
+#pragma unroll
+  for (int k = 0; k < NV; k++) {
+    double s = 0.0;
+    for (int j = threadIdx.x; j < g; j += SFR_BLOCK) s += part[(size_t)k * g + j];
+    t[k] = s;
+  }
+  block_sum<NV>(t);
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int k = 0; k < NV; k++) bs[k] = t[k];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < NV; k++) out[k] = bs[k];
   __syncthreads();
 }
 

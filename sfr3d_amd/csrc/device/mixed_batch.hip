@@ -305,7 +305,7 @@ __global__ __launch_bounds__(SFR_BLOCK) void k_ub_init2(sfr_ub_view v,
     if (lb != 0) return;
     const UbSys S = v.sys[s];
     double nb[2];
-    reduce_partials<2>(slots_of(v.partB, S), S.g, nb);
+    reduce_partials_seq<2>(slots_of(v.partB, S), S.g, nb);
     if (threadIdx.x == 0) {
       sfr_scalars *c = &v.sc[s];
       c->norm_b = sqrt(fabs(nb[0]));
@@ -398,7 +398,7 @@ __global__ __launch_bounds__(SFR_BLOCK) void k_ub_s2(sfr_ub_view v)
     const int64_t base = (int64_t)lb * S.rpt * SFR_BLOCK;
     if (run == UB_CG) {
       double pq[1];
-      reduce_partials<1>(slots_of(v.partA, S), S.g, pq);
+      reduce_partials_seq<1>(slots_of(v.partA, S), S.g, pq);
       const double d = c->d, alpha = d / pq[0];
       if (!(pq[0] > 0.0) || !isfinite(alpha)) {
         // FP32 lost positive definiteness: fold, then restart with p = r
@@ -446,7 +446,7 @@ __global__ __launch_bounds__(SFR_BLOCK) void k_ub_s2(sfr_ub_view v)
     }
     if (lb != 0) return;
     double t[4];
-    reduce_partials<4>(slots_of(v.partA, S), S.g, t);
+    reduce_partials_seq<4>(slots_of(v.partA, S), S.g, t);
     if (threadIdx.x == 0) {
       const double rmr = t[0];
       const bool conv = sqrt(fabs(rmr)) < c->conv || rmr == 0.0;
@@ -481,7 +481,7 @@ __global__ __launch_bounds__(SFR_BLOCK) void k_ub_s3(sfr_ub_view v)
     const float *__restrict__ r = v.r + S.row0;
     if (run == UB_CG) {
       double t[1];
-      reduce_partials<1>(slots_of(v.partB, S), S.g, t);
+      reduce_partials_seq<1>(slots_of(v.partB, S), S.g, t);
       const double rr = t[0];
       const double conv = c->conv;
       const bool fold = rr < c->delta2 * c->rho_max || rr <= conv * conv ||
@@ -519,7 +519,7 @@ __global__ __launch_bounds__(SFR_BLOCK) void k_ub_s3(sfr_ub_view v)
     }
     // after a fold: new direction from the refreshed residual
     double t[4];
-    reduce_partials<4>(slots_of(v.partA, S), S.g, t);
+    reduce_partials_seq<4>(slots_of(v.partA, S), S.g, t);
     const double c2 = t[2];
     const bool fresh = run != UB_FOLD;          // INIT or RESTART: p = r
     const float fb = (float)(c2 / c->d_prev), fg = (float)(t[3] / c2);
@@ -568,7 +568,7 @@ __global__ __launch_bounds__(SFR_BLOCK) void k_ub_final2(sfr_ub_view v)
     if (lb != 0) return;
     const UbSys S = v.sys[s];
     double t[4];
-    reduce_partials<4>(slots_of(v.partB, S), S.g, t);
+    reduce_partials_seq<4>(slots_of(v.partB, S), S.g, t);
     if (threadIdx.x == 0)
       for (int k = 0; k < 4; k++) v.sc[s].res[k] = t[k];
   });
