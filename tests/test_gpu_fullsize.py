@@ -7,6 +7,8 @@ not depend on size:
     1e-9; mixed precision: the reference's stop test met on the true FP64
     residual, its error against a tol-1e-13 solution below a sanity bound
     (it is about 7x the FP64 solve's), bitwise repeatable.
+  - the same system in the device CSR format (several lanes per row):
+    FP64 PCG against the oracle the same way, bitwise repeatable.
   - 96^3 Taylor-Green momentum, shared CSRD pattern (884,736 rows x 3):
     the batched u/v/w BiCGSTAB bitwise equal to three single solves and
     within 1e-6 of the oracle's serial BiCGSTAB, every stop test met.
@@ -76,6 +78,29 @@ def test_poisson_128cube_fp64_and_mixed(ctx, poisson128):
     assert e64 < 1e-6 and em < 1e-5
     stm2 = ctx.solve_device(A, S.PCG, TOL, 100000, db, dx, precision=S.PREC_MIXED)
     assert stm2["iters"] == stm["iters"] and np.array_equal(dx.get(), xm)
+    A.destroy(); db.free(); dx.free()
+
+
+def test_poisson_128cube_fp64_csr_format(ctx, poisson128):
+    """The same system with the device CSR format forced (several lanes per
+    row) through the FP64 PCG loop's launch shape from 2^21 rows (1 row per
+    thread, the SpMV on at most 1,024 workgroups): the oracle's iteration
+    count to within 2 and its solution to 1e-9, and bitwise repeatable."""
+    B = poisson128
+    n = B.system.neq
+    A = ctx.matrix(B.system, S.FMT_CSR)
+    assert A.info()["format"] == S.FMT_CSR
+    db, dx = ctx.array(B.b), ctx.empty(n)
+    st = ctx.solve_device(A, S.PCG, TOL, 100000, db, dx)
+    x = dx.get()
+    xo, so = oracle.pcg_omp(oracle.full_rows(B.system), B.system.ad[:n], B.b, TOL, 100000)
+    print(f"128^3 CSR format: iters {st['iters']} oracle {so['iters']} "
+          f"rows per thread {A.info()['rows_per_thread_pcg']}", flush=True)
+    assert st["status"] == S._lib.OK
+    assert abs(int(st["iters"]) - int(so["iters"])) <= 2
+    assert relerr(x, xo) < 1e-9
+    st2 = ctx.solve_device(A, S.PCG, TOL, 100000, db, dx)
+    assert st2["iters"] == st["iters"] and np.array_equal(dx.get(), x)
     A.destroy(); db.free(); dx.free()
 
 
