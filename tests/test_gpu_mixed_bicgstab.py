@@ -99,6 +99,35 @@ def test_mixed_uvw_distinct_diagonals(ctx, n):
         assert relerr(xs[c], xo) < 1e-6, (c, relerr(xs[c], xo), sts[c], so["iters"])
 
 
+@pytest.mark.parametrize("delta", [1e-2, 1e-4])
+def test_mixed_uvw_fold_threshold(ctx, delta):
+    """The fold threshold ru_delta_bicg below its default on the 96^3 u/v/w
+    batch: each system still meets the reference's stop test and agrees with
+    the oracle, with fewer folds than at the default 0.1 (one per iteration
+    there: these systems drop the residual about tenfold per iteration)."""
+    M = systems.momentum3d(96)
+    s = M.system
+    ads = [s.ad * (1.0 + 0.05 * c) for c in range(3)]
+    bs = list(M.extra["b_all"])
+    print(f"mixed uvw 96^3 ru_delta_bicg {delta:g}", flush=True)
+    with mixed(ctx):
+        _, st_def = ctx.solve_shared(s, S.PBICGSTAB, TOL, 5000, bs, ads=ads)
+        ctx.set_option("ru_delta_bicg", delta)
+        try:
+            xs, sts = ctx.solve_shared(s, S.PBICGSTAB, TOL, 5000, bs, ads=ads)
+        finally:
+            ctx.set_option("ru_delta_bicg", 0.1)
+    print(f"  folds {[st['outer'] for st in sts]} against "
+          f"{[st['outer'] for st in st_def]} at 0.1", flush=True)
+    for c in range(3):
+        sc = with_ad(s, ads[c])
+        xo, so = oracle.pbicgstab(sc, bs[c], TOL, 5000)
+        assert sts[c]["status"] == S._lib.OK
+        assert np.linalg.norm(bs[c] - oracle.matvec(sc, xs[c])) < TOL * np.linalg.norm(bs[c])
+        assert relerr(xs[c], xo) < 1e-6, (c, relerr(xs[c], xo), sts[c], so["iters"])
+        assert sts[c]["outer"] < st_def[c]["outer"]
+
+
 def test_mixed_species_batch(ctx):
     sysm, sp = systems.species3d(16, 5, seed=2)
     als = [a for a, _, _ in sp]
