@@ -59,6 +59,9 @@ def main():
     ap.add_argument("--rpt", type=int, default=0, choices=[0, 1, 2, 4, 8],
                     help="with --profile-kernels: rows per thread of the "
                          "kernels (0: the library's default)")
+    ap.add_argument("--nt-loads", type=int, default=-1, choices=[-1, 0, 1],
+                    help="non-temporal matrix loads in the PCG loops' SpMV "
+                         "kernels: -1 by size (the library's rule), 0 off, 1 on")
     ap.add_argument("--only", choices=["solver_ids", "species", "uvw", "uneven",
                                        "adjoint", "cells", "poisson160", "poisson200"],
                     help="run one of the other workloads alone and print its "
@@ -79,6 +82,7 @@ def main():
 
     if args.only:
         ctx = S.Context(0)
+        ctx.set_option("nt_loads", args.nt_loads)
         leg = {"solver_ids": lambda: bench_solvers(ctx, S, systems, cpu=not args.no_cpu,
                                                    names=args.solvers.split(",")),
                "species": lambda: bench_species(ctx, S, systems),
@@ -101,6 +105,7 @@ def main():
         f"({time.time() - t0:.1f}s)")
 
     ctx = S.Context(0)
+    ctx.set_option("nt_loads", args.nt_loads)
     A = ctx.matrix(sysm)
     info = A.info()
     db = ctx.array(B.b)
@@ -304,7 +309,8 @@ def main():
                                    "launch_us": round(t_k64 * 1e6, 2),
                                    "bytes_per_launch": b64}},
         "cpu_baseline": cpu,
-        "hbm_copy_GBs": copy_gbs,
+        "hbm_copy_GBs": copy_gbs[0] if copy_gbs else None,
+        "hbm_copy_GBs_hipmemcpy": copy_gbs[1] if copy_gbs else None,
         "device_assembly": assembly,
     }
     out.update(extra)
@@ -472,12 +478,17 @@ def _hip():
 
 def hbm_copy_ceiling(ctx, gib=2, reps=5):
     """Device-to-device copy of `gib` GiB (far above the 256 MB last-level
-    cache), timed with HIP events: the streaming rate this HBM sustains,
-    counted as read + write bytes."""
+    cache), timed with HIP events, counted as read + write bytes: the
+    library's streaming copy kernel (sfr_copy_time), the HBM rate a pure
+    stream attains, and hipMemcpyAsync's device copy beside it."""
+    from sfr3d_amd._lib import lib
     hip = _hip()
     n = gib << 30
     src = ctx.empty(n // 8)
     dst = ctx.empty(n // 8)
+    ms_k = C.c_double()
+    rc = lib.sfr_copy_time(ctx.h, src.ptr, dst.ptr, n // 8, reps, C.byref(ms_k))
+    assert rc == 0, rc
     e0, e1 = C.c_void_p(), C.c_void_p()
     hip.hipEventCreate(C.byref(e0))
     hip.hipEventCreate(C.byref(e1))
@@ -493,9 +504,11 @@ def hbm_copy_ceiling(ctx, gib=2, reps=5):
     hip.hipEventDestroy(e1)
     src.free()
     dst.free()
-    gbs = 2.0 * n * reps / (ms.value * 1e-3) / 1e9
-    log(f"[bench] HBM copy ceiling ({gib} GiB D2D): {gbs:.0f} GB/s")
-    return round(gbs, 1)
+    gbs = 2.0 * n / (ms_k.value * 1e-3) / 1e9
+    gbs_mc = 2.0 * n * reps / (ms.value * 1e-3) / 1e9
+    log(f"[bench] HBM copy ceiling ({gib} GiB D2D): streaming kernel {gbs:.0f} GB/s, "
+        f"hipMemcpyAsync {gbs_mc:.0f} GB/s")
+    return round(gbs, 1), round(gbs_mc, 1)
 
 
 def bench_assembly(ctx, S, A, B, db, reps=5):

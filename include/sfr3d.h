@@ -484,6 +484,14 @@ int  sfr_simple_update(sfr_context *ctx, int numel, double alphaPres,
  * the updated direction, direction and correction updates, p.q). */
 int sfr_kernel_time(sfr_context *ctx, sfr_matrix *A, int which, int reps,
                     double *ms_per_launch);
+/* Measurement hook, no counterpart in the reference: average device time of
+ * one copy of n doubles d_src -> d_dst (16-byte aligned, not overlapping)
+ * by the library's streaming copy kernel (16-byte non-temporal loads and
+ * stores), HIP events on the context stream.  2 n 8 bytes / time is the
+ * HBM rate a pure stream attains, the ceiling the loop kernels are held
+ * against; on MI355X it is above what hipMemcpy's device copy reaches. */
+int sfr_copy_time(sfr_context *ctx, const double *d_src, double *d_dst,
+                  int64_t n, int reps, double *ms_per_copy);
 /* Options of a context: "chunk" = iterations per full captured graph
  * chunk (1..256; solves start on short chunks; GMRES captures chunk/16
  * restart cycles, at least one); "precision" = SFR_PREC_FP64 or
@@ -493,7 +501,12 @@ int sfr_kernel_time(sfr_context *ctx, sfr_matrix *A, int which, int reps,
  * largest value since the last fold) that triggers an FP64 reliable update
  * (both default to 0.1); "ell_pairs" = 1 (default) lets the FP64 PCG SpMV
  * read mirrored ELL values through face pairs (sfr_matrix_pair_planes), 0 reads
- * every slot (the results are the same bits either way).  A mixed BiCGSTAB
+ * every slot (the results are the same bits either way); "nt_loads" = -1
+ * (default) lets the PCG loops' SpMV kernels (FP64 and the mixed FP32 loop)
+ * stream an ELL matrix with non-temporal loads once the loop's footprint,
+ * about 84 B/row in FP64 with face pairs and 56 B/row in FP32, exceeds the
+ * 256 MiB last-level cache, 0 / 1 force plain / non-temporal loads (a cache
+ * policy only: the same bits either way).  A mixed BiCGSTAB
  * solve whose FP64 residual stops
  * improving (or breaks down) is finished in FP64 from its last x, within
  * the same call; stats.iters then counts both parts. */

@@ -141,6 +141,8 @@ _sig("sfr_solve_adjoint", C.c_int, vp, vp, C.c_int, C.c_double, C.c_uint, vp,
      vp, vp, vp, vp, C.POINTER(SfrStats))
 _sig("sfr_kernel_time", C.c_int, vp, vp, C.c_int, C.c_int,
      C.POINTER(C.c_double))
+_sig("sfr_copy_time", C.c_int, vp, vp, vp, C.c_int64, C.c_int,
+     C.POINTER(C.c_double))
 _sig("sfr_set_option", C.c_int, vp, C.c_char_p, C.c_double)
 # device assembly
 _sig("sfr_assembly_create", C.c_int, vp, vp, C.POINTER(SfrPattern), i32p,

@@ -64,6 +64,35 @@ __global__ __launch_bounds__(SFR_BLOCK) void k_fill32(int64_t n, float v,
   if (i < n) x[i] = v;
 }
 
+// Streaming copy of n2 16-byte elements, four per thread per pass with all
+// loads issued before the stores, non-temporal both ways: the HBM rate a
+// pure stream attains (the measurement hook sfr_copy_time).  An odd
+// trailing double is copied by the first thread.
+typedef double sfr_v2d __attribute__((vector_size(16)));
+
+__global__ __launch_bounds__(SFR_BLOCK) void k_stream_copy(int64_t n2,
+    const sfr_v2d *__restrict__ src, sfr_v2d *__restrict__ dst,
+    const double *__restrict__ tsrc, double *__restrict__ tdst)
+{
+  constexpr int U = 4;
+  const int64_t stride = (int64_t)gridDim.x * SFR_BLOCK * U;
+  for (int64_t base = (int64_t)blockIdx.x * SFR_BLOCK * U + threadIdx.x;
+       base < n2; base += stride) {
+    sfr_v2d v[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const int64_t i = base + (int64_t)u * SFR_BLOCK;
+      if (i < n2) v[u] = __builtin_nontemporal_load(src + i);
+    }
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const int64_t i = base + (int64_t)u * SFR_BLOCK;
+      if (i < n2) __builtin_nontemporal_store(v[u], dst + i);
+    }
+  }
+  if (tsrc && blockIdx.x == 0 && threadIdx.x == 0) *tdst = *tsrc;
+}
+
 // Jacobi-scaled FP32 copy for the mixed solve: a_ij / sqrt(a_ii a_jj).
 __global__ __launch_bounds__(SFR_BLOCK) void k_dscale(int64_t n,
     const double *__restrict__ ad, double *__restrict__ dscale)
@@ -412,6 +441,23 @@ extern "C" int sfr_k_fill32(hipStream_t s, int64_t n, float v, float *x)
 {
   if (n <= 0) return hipSuccess;
   hipLaunchKernelGGL(k_fill32, dim3(blocks1(n)), dim3(SFR_BLOCK), 0, s, n, v, x);
+  return hipGetLastError();
+}
+
+// grid: 16,384 workgroups of 4 x 256 elements per pass (the fastest shape
+// of a sweep over 1,024-65,536 workgroups on 2 GiB buffers)
+extern "C" int sfr_k_stream_copy(hipStream_t s, int64_t n, const double *src,
+                                 double *dst)
+{
+  if (n <= 0) return hipSuccess;
+  const int64_t n2 = n / 2;
+  const int64_t per = (int64_t)SFR_BLOCK * 4;
+  const int64_t want = (n2 + per - 1) / per;
+  const unsigned grid = (unsigned)(want < 16384 ? (want > 0 ? want : 1) : 16384);
+  hipLaunchKernelGGL(k_stream_copy, dim3(grid), dim3(SFR_BLOCK), 0, s, n2,
+                     (const sfr_v2d *)src, (sfr_v2d *)dst,
+                     (n & 1) ? src + n - 1 : (const double *)nullptr,
+                     (n & 1) ? dst + n - 1 : (double *)nullptr);
   return hipGetLastError();
 }
 

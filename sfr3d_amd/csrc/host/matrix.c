@@ -77,6 +77,7 @@ int sfr_init(sfr_context **out, int device)
   ctx->ru_delta = 0.1;
   ctx->ru_delta_b = 0.1;
   ctx->ell_pairs = 1;
+  ctx->nt_loads = -1;
   if (hipSetDevice(device) != hipSuccess) { free(ctx); return SFR_NO_DEVICE; }
   int rc = SFR_OK;
 #define INIT_TRY(call) do { if ((call) != hipSuccess) { rc = SFR_HIP_ERROR; goto fail; } } while (0)
@@ -258,6 +259,7 @@ sfr_mat_view sfr_view(const sfr_matrix *A)
   v.exc_k = A->exc_k; v.exc_col = A->exc_col; v.nexc = A->nexc;
   v.valnb = NULL;
   v.pair = A->pair_use ? A->pair_kind : SFR_PAIR_NONE;
+  v.nt = 0;
   return v;
 }
 

@@ -100,7 +100,7 @@ static int get_graph(sfr_context *ctx, int kind, const void *k0,
         e->key[1] == k1 && e->key[2] == k2 &&
         e->key[3] == (const void *)ctx->w64[0] && e->key[4] == (const void *)(intptr_t)g.rpt &&
         e->key[5] == (const void *)ctx->gm_buf && e->key[6] == (const void *)(intptr_t)g.cap &&
-        e->key[7] == (const void *)(intptr_t)(v->lanes | v->pair << 8)) {
+        e->key[7] == (const void *)(intptr_t)(v->lanes | v->pair << 8 | v->nt << 12)) {
       /* least recently used first: a hit moves to the end, so the graph a
        * caller has just been handed is never the one a second lookup for
        * the same solve evicts */
@@ -179,7 +179,7 @@ static int get_graph(sfr_context *ctx, int kind, const void *k0,
   e->key[4] = (const void *)(intptr_t)g.rpt;
   e->key[5] = (const void *)ctx->gm_buf;
   e->key[6] = (const void *)(intptr_t)g.cap;
-  e->key[7] = (const void *)(intptr_t)(v->lanes | v->pair << 8);
+  e->key[7] = (const void *)(intptr_t)(v->lanes | v->pair << 8 | v->nt << 12);
   *exec = ex;
   return SFR_OK;
 }
@@ -367,6 +367,7 @@ static int solve_pcg64(sfr_context *ctx, sfr_matrix *A, double tol,
   CHECK(sfr_workspace(ctx, A->n));
   CHECK(sfr_matrix_pair_check(ctx, A));
   sfr_mat_view v = sfr_view(A);
+  v.nt = sfr_loop_nt(ctx, A, 0);
   sfr_geom g = {sfr_pcg_rpt(A), 0, 0};
   chunks ch;
   CHECK(get_chunks(ctx, G_PCG, A, d_x, NULL, &v, g, &ch));
@@ -636,6 +637,7 @@ static int solve_pcg_mixed(sfr_context *ctx, sfr_matrix *A, double tol,
   CHECK(sfr_workspace(ctx, A->n));
   CHECK(sfr_matrix_prepare_mixed(ctx, A));
   sfr_mat_view v = sfr_view(A);
+  v.nt = sfr_loop_nt(ctx, A, 1);
   const sfr_geom g = mixed_geom(A);
   chunks ch;
   CHECK(get_chunks(ctx, G_MX, A, NULL, NULL, &v, g, &ch));
@@ -1032,6 +1034,7 @@ int sfr_solve_device_noalloc(sfr_context *ctx, sfr_matrix *A, int solver,
   memset(&d, 0, sizeof d);
   d.solver = solver;
   d.v = sfr_view(A);
+  if (solver == SFR_PCG) d.v.nt = sfr_loop_nt(ctx, A, 0);
   d.g = (sfr_geom){solver == SFR_PCG ? sfr_pcg_rpt(A) : A->rpt, 0, 0};
   d.x = d_x;
   d.sc = (sfr_scalars *)base;
@@ -1064,6 +1067,7 @@ int sfr_kernel_time(sfr_context *ctx, sfr_matrix *A, int which, int reps,
   if (which == 1) CHECK(sfr_matrix_prepare_mixed(ctx, A));
   if (which == 0) CHECK(sfr_matrix_pair_check(ctx, A));
   sfr_mat_view v = sfr_view(A);
+  v.nt = sfr_loop_nt(ctx, A, which == 1);
   sfr_geom g = {which == 0 ? sfr_pcg_rpt(A) : A->rpt, 0, 0};
   if (which == 1) g = mixed_geom(A);
   hipStream_t s = ctx->stream;
@@ -1103,6 +1107,30 @@ int sfr_kernel_time(sfr_context *ctx, sfr_matrix *A, int which, int reps,
   float ms = 0.0f;
   TRY(hipEventElapsedTime(&ms, ctx->ev_start, ctx->ev_stop));
   *ms_per_launch = ms / reps;
+  return SFR_OK;
+}
+
+int sfr_copy_time(sfr_context *ctx, const double *d_src, double *d_dst,
+                  int64_t n, int reps, double *ms_per_copy)
+{
+  if (!ctx || !d_src || !d_dst || n <= 0 || reps <= 0 || !ms_per_copy)
+    return SFR_BAD_ARG;
+  if ((((uintptr_t)d_src | (uintptr_t)d_dst) & 15) != 0)
+    return sfr_set_error(ctx, SFR_BAD_ARG, "sfr_copy_time: buffers not 16-byte aligned");
+  const uintptr_t a = (uintptr_t)d_src, b = (uintptr_t)d_dst,
+                  len = (uintptr_t)n * sizeof(double);
+  if (a < b + len && b < a + len)
+    return sfr_set_error(ctx, SFR_BAD_ARG, "sfr_copy_time: buffers overlap");
+  TRY(hipSetDevice(ctx->device));
+  hipStream_t s = ctx->stream;
+  TRY(sfr_k_stream_copy(s, n, d_src, d_dst));
+  TRY(hipEventRecord(ctx->ev_start, s));
+  for (int r = 0; r < reps; r++) TRY(sfr_k_stream_copy(s, n, d_src, d_dst));
+  TRY(hipEventRecord(ctx->ev_stop, s));
+  TRY(hipEventSynchronize(ctx->ev_stop));
+  float ms = 0.0f;
+  TRY(hipEventElapsedTime(&ms, ctx->ev_start, ctx->ev_stop));
+  *ms_per_copy = ms / reps;
   return SFR_OK;
 }
 
@@ -1290,6 +1318,12 @@ int sfr_set_option(sfr_context *ctx, const char *name, double value)
   if (!strcmp(name, "ru_delta")) {
     if (!(value > 0 && value < 1)) return sfr_set_error(ctx, SFR_BAD_ARG, "ru_delta %g", value);
     ctx->ru_delta = value;
+    return SFR_OK;
+  }
+  if (!strcmp(name, "nt_loads")) {
+    if (value != -1 && value != 0 && value != 1)
+      return sfr_set_error(ctx, SFR_BAD_ARG, "nt_loads %g", value);
+    ctx->nt_loads = (int)value;
     return SFR_OK;
   }
   if (!strcmp(name, "ell_pairs")) {

@@ -84,6 +84,8 @@ typedef struct sfr_mat_view {
                             /*   interleaved, valnb[k*nb + c], or NULL     */
   int            pair;      /* ELL face pairs the FP64 PCG SpMV may read   */
                             /*   (SFR_PAIR_HEX / SFR_PAIR_QUAD), or 0      */
+  int            nt;        /* the PCG loops' SpMV kernels stream the      */
+                            /*   matrix with non-temporal loads            */
 } sfr_mat_view;
 
 /* ELL face pairs: SFR3D's hex face order pairs slots (0,1) (2,4) (3,5),
@@ -173,6 +175,8 @@ struct sfr_context {
   double       ru_delta;        /* MIXED PCG: reliable-update residual drop */
   double       ru_delta_b;      /* MIXED BiCGSTAB: the same                */
   int          ell_pairs;       /* FP64 PCG SpMV may read ELL face pairs   */
+  int          nt_loads;        /* -1 by size, 0 / 1: non-temporal matrix   */
+                                /*   loads in the PCG loops' SpMV kernels   */
   void        *scratch;         /* device address of the static scratch    */
   int          precision;       /* host-pointer and batch entry points     */
   float       *mb32;            /* mixed BiCGSTAB: 7 vectors, diagonal and */
@@ -218,6 +222,8 @@ int sfr_k_fill(hipStream_t s, int64_t n, double v, double *x);
 int sfr_k_spmv(hipStream_t s, const sfr_mat_view *A, const double *x,
                double *y);
 int sfr_k_fill32(hipStream_t s, int64_t n, float v, float *x);
+int sfr_k_stream_copy(hipStream_t s, int64_t n, const double *src,
+                      double *dst);
 /* single SpMV kernels of the PCG / inner loops (measurement hooks) */
 int sfr_k_pcg_spmv_only(hipStream_t s, const sfr_mat_view *A, sfr_geom g,
                         const double *p, double *q, sfr_scalars *sc,
@@ -282,6 +288,28 @@ static inline int sfr_pcg_rpt(const sfr_matrix *A)
   if (A->rpt_pcg) return A->rpt_pcg;
   if (!A->rpt_set && A->n >= ((int64_t)1 << 21)) return 1;
   return A->rpt;
+}
+
+/* Non-temporal matrix loads in the PCG loops' SpMV kernels (FP64
+ * k_pcg_spmv, FP32 k_mx_a): the context option "nt_loads" when it is set,
+ * else once the loop's footprint -- the matrix bytes its SpMV reads plus
+ * its five vectors -- exceeds the 256 MiB last-level cache.  Below that
+ * the plain loads keep the matrix cached from one iteration to the next;
+ * above it they only displace the vectors, which the vector kernels then
+ * fetch from HBM.  Measured on MI355X, solves/s with the loads plain /
+ * non-temporal, 128^3 / 160^3 / 200^3 Poisson (84 B/row FP64 with face
+ * pairs, 56 B/row FP32): FP64 15.6 / 14.5, 5.29 / 5.79, 2.38 / 2.41;
+ * mixed 27.2 / 23.3, 12.0 / 10.5, 4.69 / 5.08.  The values are the same
+ * bits either way. */
+#define SFR_LLC_BYTES ((int64_t)256 << 20)
+static inline int sfr_loop_nt(const sfr_context *ctx, const sfr_matrix *A,
+                              int fp32)
+{
+  if (ctx->nt_loads >= 0) return ctx->nt_loads;
+  const int64_t s = A->slots;
+  const int64_t row = fp32 ? s * 4 + s * 2 + 5 * 4
+                           : 8 + (A->pair_use ? s / 2 : s) * 8 + s * 2 + 5 * 8;
+  return (int64_t)A->n * row > SFR_LLC_BYTES;
 }
 
 /* GMRES(SFR_GM_K), FP64 (reference src/Iters.c:587-790) */

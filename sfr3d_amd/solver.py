@@ -292,8 +292,10 @@ class Context:
         return e.value, c.value, h.value
 
     def set_option(self, name, value):
-        """sfr_set_option: "chunk" (iterations per captured graph chunk) or
-        "ru_delta" (mixed-precision reliable-update threshold)."""
+        """sfr_set_option: "chunk" (iterations per captured graph chunk),
+        "precision", "ru_delta" / "ru_delta_bicg" (mixed-precision
+        reliable-update thresholds), "ell_pairs" (0/1) or "nt_loads"
+        (-1 by size, 0/1); an unknown name or value raises SolverError."""
         self._check(lib.sfr_set_option(self.h, name.encode(), float(value)))
 
     def __enter__(self):

@@ -72,3 +72,4 @@ inline unsigned long long wall_clock64() { return emu::clock(); }
 #define hipLaunchKernelGGL(K, G, B, SHM, S, ...) emu::launch(K, dim3(G), dim3(B), __VA_ARGS__)
 inline int __builtin_amdgcn_readfirstlane(int v) { return v; }
 template <class T> inline T __builtin_nontemporal_load(const T *p) { return *p; }
+template <class T> inline void __builtin_nontemporal_store(T v, T *p) { *p = v; }

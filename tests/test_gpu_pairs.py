@@ -10,7 +10,9 @@ x and the iteration count of FP64 PCG equal with the context option
 unchanged by it -- on the 128^3 bench system (hex rows, RCM) and a
 128 x 128 quad system; a value that stops mirroring turns the pairs off and
 mirroring values turn them on again, with the solution of the full read
-each time.
+each time.  The same bar holds for the context option "nt_loads" (the
+matrix streamed by non-temporal loads in the loops' SpMV kernels, a cache
+policy that must not change a bit).
 """
 import contextlib
 
@@ -107,5 +109,39 @@ def test_pairs_follow_the_values(ctx):
         assert st3["iters"] == st["iters"] and np.array_equal(dx.get(), x_pair)
         xo, so = oracle.pcg(B.system, B.b, TOL, 100000)
         assert abs(int(st["iters"]) - int(so["iters"])) <= 1 and relerr(x_pair, xo) < 1e-9
+    finally:
+        A.destroy(); db.free(); dx.free()
+
+
+@pytest.mark.parametrize("case", ["hex64", "quad128"])
+def test_nt_loads_same_bits(ctx, case):
+    """FP64 (face-pair read) and mixed PCG with the matrix loads forced
+    plain and non-temporal: the same iterations and the same x."""
+    if case == "hex64":
+        B = systems.poisson3d(64, seed=3, kmin=1e-2, reorder=True, storage=S.ELLPACK)
+    else:
+        B = systems.poisson2d(128, 128, seed=4, storage=S.ELLPACK, pinned_cell=None)
+    n = B.system.neq
+    print(f"nt_loads {case}: {n} rows", flush=True)
+    A = ctx.matrix(B.system)
+    db, dx = ctx.array(B.b), ctx.empty(n)
+    try:
+        for prec in (S.PREC_FP64, S.PREC_MIXED):
+            out = []
+            for nt in (0, 1):
+                ctx.set_option("nt_loads", nt)
+                try:
+                    st = ctx.solve_device(A, S.PCG, TOL, 100000, db, dx, precision=prec)
+                finally:
+                    ctx.set_option("nt_loads", -1)
+                out.append((st, dx.get()))
+            (s0, x0), (s1, x1) = out
+            print(f"  precision {prec}: iters {s0['iters']} / {s1['iters']}", flush=True)
+            assert s0["status"] == s1["status"] == S._lib.OK
+            assert s0["iters"] == s1["iters"] and s0["outer"] == s1["outer"]
+            assert np.array_equal(x0, x1)
+        assert A.info()["pair_planes"] in (1, 2)
+        with pytest.raises(S.SolverError):
+            ctx.set_option("nt_loads", 2)
     finally:
         A.destroy(); db.free(); dx.free()
